@@ -1,0 +1,139 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the gfx950 plain cellular-automaton track finder.
+
+Workload: central Pb-Pb-like events (8000 primaries + 2% secondaries + 2% noise,
+about 8.3e3 clusters per layer, one primary vertex), generated from seeds and
+uploaded to HBM before timing.  One step = one plain-CA pass (index tables,
+tracklets, cells, neighbours, levels, roads) over a batch of such events.
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tracking-itsu_amd"))
+
+HBM_PEAK_GBS = 8000.0
+STAGES = ["prep+index tables", "tracklets", "cells", "neighbours", "levels+roads"]
+
+
+def bytes_alg(c):
+    """Compulsory HBM bytes of one (event, vertex): every stage reads its inputs
+    once and writes its outputs once, at the reference record sizes (Cluster 36,
+    Tracklet 16, Cell 40, Road 32, index/level/edge 4 bytes)."""
+    N, T, C, E, R = list(c.clusters), list(c.tracklets), list(c.cells), list(c.neighbours), c.roads
+    b = sum(n * (16 + 36) for n in N) + 6 * 401 * 4
+    b += sum((N[l] + N[l + 1]) * 36 + T[l] * 16 + N[l + 1] * 4 for l in range(6))
+    b += sum((T[l] + T[l + 1]) * 16 + (N[l] + N[l + 1] + N[l + 2]) * 36 + C[l] * 40 + T[l + 1] * 4 for l in range(5))
+    b += sum((C[l] + C[l + 1]) * 40 + E[l] * 4 + C[l + 1] * 4 for l in range(4))
+    b += sum(C) * 4 + sum(E) * 4 + R * 32
+    return b
+
+
+def cpu_baseline(events_path, n_events):
+    """The reference CPU tracker itself (built from its sources into
+    oracle/_ref/), single thread, on a bounded sample of the same workload."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if os.path.exists(exe):
+        out = subprocess.run([exe, "time", events_path, "1"], check=True, capture_output=True, text=True).stdout
+        r = json.loads(out.strip().splitlines()[-1])
+        return {"value": r["events"] / r["seconds"], "unit": "events/s", "cores": 1, "kind": "reference",
+                "sample": "%d central events (seeds 2..%d), single pass after one warm-up event" % (n_events, n_events + 1)}
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle  # noqa: E402
+    import itsca  # noqa: E402
+    evs = itsca.load_events(events_path)
+    s = pyoracle.time_events(evs, 1, 1)
+    return {"value": len(evs) / s, "unit": "events/s", "cores": 1, "kind": "port",
+            "sample": "%d central events, single pass" % len(evs)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=32, help="central events per step")
+    ap.add_argument("--cpu-events", type=int, default=40, help="events in the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every (event, vertex) is independent, so N GPUs would each run "
+              "their own share; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import itsca
+
+    t_gen = time.time()
+    events = itsca.generate_events(2, 2, args.batch)
+    tracker = itsca.GpuTracker(device=0)
+    tracker.set_events(events)
+    print("# generated and uploaded %d events in %.1f s" % (len(events), time.time() - t_gen), file=sys.stderr,
+          flush=True)
+
+    tracker.run()  # sizes the buffers
+    for _ in range(args.warmup):
+        tracker.run_async()
+    tracker.finish()
+
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tracker.run_async()
+    tracker.finish()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    events_per_s = args.batch * args.steps / elapsed
+
+    # algorithmic bytes per step and per-stage device time (separate profiled run)
+    balg = sum(bytes_alg(tracker.counts(i)) for i in range(tracker.num_items()))
+    tracker.set_profiling(True)
+    prof_runs = 5
+    stage_ms = [0.0] * 5
+    for _ in range(prof_runs):
+        tracker.run()
+        stage_ms = [a + b for a, b in zip(stage_ms, tracker.stage_times().tolist())]
+    stage_ms = [v / prof_runs for v in stage_ms]
+    tracker.set_profiling(False)
+    c0 = tracker.counts(0)
+    dominant = max(range(5), key=lambda k: stage_ms[k])
+    achieved = balg / (ms_per_step * 1e-3) / 1e9
+
+    line = {
+        "metric": "events/s on 1 MI355X, Pb-Pb-like 8k tracks: plain GPU vs batched+learned filter",
+        "value": round(events_per_s, 2),
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "plain CA, central Pb-Pb-like events (8000 tracks, ~8.3e3 clusters/layer), "
+                               "%d events per step, inputs resident in HBM" % args.batch,
+                   "events_per_step": args.batch, "form": "plain GPU (batched launch)"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "whole pipeline", "bytes_alg_per_step": balg},
+        "stage_ms": {STAGES[k]: round(stage_ms[k], 4) for k in range(5)},
+        "dominant_stage": STAGES[dominant],
+        "counts_event0": {"tracklets": list(c0.tracklets), "cells": list(c0.cells), "roads": c0.roads},
+    }
+    if not args.no_cpu:
+        with tempfile.TemporaryDirectory() as tmp:
+            p = os.path.join(tmp, "cpu_sample.txt")
+            itsca.write_events(2, 2, args.cpu_events, p)
+            line["cpu_baseline"] = cpu_baseline(p, args.cpu_events)
+    print(json.dumps(line), flush=True)
+    tracker.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,119 @@
+/* itsca.h -- C ABI of libitsca_hip.so, the gfx950 cellular-automaton track
+ * finder of the ALICE ITS (plain CA: index tables, tracklets, cells,
+ * neighbours, CA levels and roads).
+ *
+ * The reference plugs its back ends in at compile time, through
+ * TrackerTraits<IsGPU>::computeLayerTracklets / computeLayerCells
+ * (include/ITSReconstruction/CA/Tracker.h:39-48) with the device state hung off
+ * PrimaryVertexContext (PrimaryVertexContext.h:59-77,
+ * gpu/PrimaryVertexContext.h:39-42).  This library replaces that whole back
+ * end -- and the stages the reference keeps on the host (findCellsNeighbours,
+ * findTracks, computeMontecarloLabels, Tracker.cxx:416-652) -- behind plain C
+ * entry points.  The C++ mirror of the reference API (Tracker<true>) and any
+ * foreign-function binding sit on top of it.
+ *
+ * Conventions: every call returns ITSCA_OK (0) or an error code, with the text
+ * in itsca_last_error(ctx).  Device memory is owned by the context; host
+ * buffers passed in are copied before the call returns; output buffers are
+ * owned by the caller, sized from itsca_get_counts().  A context is not
+ * thread-safe; independent contexts are.
+ */
+#ifndef ITSCA_H
+#define ITSCA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct itsca_ctx itsca_ctx;
+typedef int itsca_status;
+
+enum {
+  ITSCA_OK = 0,
+  ITSCA_ERR_ARGUMENT = 1,
+  ITSCA_ERR_HIP = 2,      /* a HIP runtime call failed (no device, launch failure, ...) */
+  ITSCA_ERR_CAPACITY = 3, /* an output did not fit even after growing the buffers */
+  ITSCA_ERR_STATE = 4     /* call out of order (e.g. download before run) */
+};
+
+typedef struct itsca_config {
+  int device;            /* HIP device ordinal */
+  int persistent_blocks; /* workgroups of the persistent stage kernels; 0 = 8 per CU */
+  double capacity_scale; /* multiplier on the built-in output-size estimates; 0 = 1 */
+} itsca_config;
+
+/* Output sizes of one work item (one primary vertex of one event). */
+typedef struct itsca_counts {
+  int clusters[7];
+  int tracklets[6];
+  int cells[5];
+  int neighbours[4]; /* accepted edges between cell layers l and l+1 */
+  int roads;
+} itsca_counts;
+
+/* Record kinds for itsca_download (rows of 4-byte fields, little endian). */
+enum {
+  ITSCA_CLUSTERS = 0,          /* sorted clusters of `layer`: id, phi(f32), r(f32), bin, x, y, z (f32), mc */
+  ITSCA_INDEX_TABLE = 1,       /* 401 int: first sorted position of each z-phi bin of `layer` (1..6) */
+  ITSCA_TRACKLETS = 2,         /* first, second, tanLambda(f32), phi(f32) */
+  ITSCA_TRACKLET_LOOKUP = 3,   /* per cluster of layer+1: first tracklet of layer+1 starting there, or -1 */
+  ITSCA_CELLS = 4,             /* c1, c2, c3, t1, t2, nx, ny, nz, curvature (f32), level */
+  ITSCA_CELL_LOOKUP = 5,       /* per tracklet of layer+1: first cell of layer+1 starting there, or -1 */
+  ITSCA_NEIGHBOUR_OFFSETS = 6, /* cells(layer+1)+1 int: start of each cell's list in ITSCA_NEIGHBOURS */
+  ITSCA_NEIGHBOURS = 7,        /* ascending cells of `layer` neighbouring each cell of layer+1 */
+  ITSCA_ROADS = 8              /* cell index per layer 0..4, size, label, fake (8 int) */
+};
+
+/* Creates a context on cfg->device (cfg may be NULL: device 0, defaults).  On
+ * failure *out still receives a context that holds the error text for
+ * itsca_last_error; destroy it as usual. */
+itsca_status itsca_create(itsca_ctx** out, const itsca_config* cfg);
+void itsca_destroy(itsca_ctx* ctx);
+const char* itsca_last_error(const itsca_ctx* ctx);
+
+/* Input: events are staged on the host, then uploaded to HBM together.
+ * Replaces Event/Layer::addCluster + PrimaryVertexContext::initialize's input
+ * (Event.h:35-44, PrimaryVertexContext.cxx:31-50).  Arrays hold every layer's
+ * clusters, layer after layer, in file order; layer_offsets has 8 entries
+ * (layer l occupies [layer_offsets[l], layer_offsets[l+1])); vertices holds 3
+ * floats per primary vertex. */
+itsca_status itsca_clear_events(itsca_ctx* ctx);
+itsca_status itsca_add_event(itsca_ctx* ctx, const float* x, const float* y, const float* z,
+                             const int32_t* cluster_id, const int32_t* mc_id, const int32_t* layer_offsets,
+                             const float* vertices, int num_vertices, int* event_index);
+itsca_status itsca_upload(itsca_ctx* ctx);
+
+/* Runs the plain CA on every (event, vertex) work item of the uploaded batch
+ * and waits for it.  Counterpart of Tracker::clustersToTracks
+ * (Tracker.cxx:244-265) for the whole batch; work item i is the i-th vertex in
+ * event order. */
+itsca_status itsca_run(itsca_ctx* ctx);
+
+/* Enqueues the same run on the context's stream without waiting; call
+ * itsca_finish before reading results.  Used to time back-to-back runs. */
+itsca_status itsca_run_async(itsca_ctx* ctx);
+itsca_status itsca_finish(itsca_ctx* ctx);
+
+int itsca_num_items(const itsca_ctx* ctx);
+itsca_status itsca_get_counts(itsca_ctx* ctx, int item, itsca_counts* out);
+
+/* Copies one record kind of one work item into dst (cap_bytes long);
+ * *written receives the number of bytes. */
+itsca_status itsca_download(itsca_ctx* ctx, int item, int what, int layer, void* dst, size_t cap_bytes,
+                            size_t* written);
+
+/* Per-stage device time of the last run, from HIP events on the context's
+ * stream (milliseconds): [0] cluster preparation and index tables,
+ * [1] tracklets, [2] cells, [3] neighbours, [4] levels and roads.
+ * Requires itsca_set_profiling(ctx, 1) before the run. */
+itsca_status itsca_set_profiling(itsca_ctx* ctx, int enable);
+itsca_status itsca_stage_times(itsca_ctx* ctx, float ms[5]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ITSCA_H */

@@ -1,0 +1,39 @@
+/* itsca_host.h -- C entry points of libitsca.so for event batches: the seeded
+ * workload generator and the reference text formats (src/CA/IOUtils.cxx:38-123),
+ * for callers that do not use the C++ API (Python, foreign-function bindings).
+ */
+#ifndef ITSCA_HOST_H
+#define ITSCA_HOST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct itsca_events itsca_events;
+
+/* preset: 1 small (100 tracks), 2 central (8000 tracks), 3 mixed multiplicity
+ * (log-uniform 50..8000), 5 pile-up (4 collisions); event i uses seed + i. */
+itsca_events* itsca_events_generate(int preset, uint64_t seed, int num_events);
+/* Reads the reference event format; NULL if the file cannot be opened. */
+itsca_events* itsca_events_load(const char* path);
+/* Writes the reference event format (and the labels format if labels_path is
+ * not NULL); returns 0 on success. */
+int itsca_events_write(const itsca_events* ev, const char* events_path, const char* labels_path);
+void itsca_events_free(itsca_events* ev);
+
+int itsca_events_count(const itsca_events* ev);
+int itsca_events_num_vertices(const itsca_events* ev, int event);
+/* offsets[8]: layer l occupies [offsets[l], offsets[l+1]) of the copy below. */
+int itsca_events_layer_offsets(const itsca_events* ev, int event, int32_t offsets[8]);
+/* Copies the event's clusters (all layers, layer after layer, file order) and
+ * its vertices (3 floats each); returns 0 on success. */
+int itsca_events_copy(const itsca_events* ev, int event, float* x, float* y, float* z, int32_t* cluster_id,
+                      int32_t* mc_id, float* vertices);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ITSCA_HOST_H */

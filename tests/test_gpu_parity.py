@@ -1,0 +1,101 @@
+"""gfx950 pipeline against the reference: bit-exact, position for position,
+with the stable-order reference build (the reference with std::stable_sort in
+place of std::sort for the per-layer bin order; see golden/make_golden.py)."""
+import hashlib
+import os
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+import itsca
+import pyoracle
+from golden_cases import central, fixture, small_cases
+
+pytestmark = pytest.mark.gpu
+CASES = small_cases()
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def tracker():
+    t = itsca.GpuTracker(device=0)
+    yield t
+    t.close()
+
+
+@pytest.mark.parametrize("name,path,nv", CASES, ids=[c[0] for c in CASES])
+def test_single_event_bit_exact(tracker, name, path, nv):
+    say(f"gpu case {name}")
+    ev = itsca.load_events(path)[0]
+    tracker.set_events([ev])
+    tracker.run()
+    assert tracker.num_items() == nv
+    for v in range(nv):
+        assert itsca.dump_item(tracker, v, ev.vertices[v]) == fixture(name, "stable", v)
+
+
+def test_batch_of_all_cases_bit_exact(tracker):
+    say("gpu batch of all golden cases")
+    evs = [itsca.load_events(p)[0] for _, p, _ in CASES]
+    tracker.set_events(evs)
+    tracker.run()
+    item = 0
+    for (name, _, nv), ev in zip(CASES, evs):
+        for v in range(nv):
+            assert itsca.dump_item(tracker, item, ev.vertices[v]) == fixture(name, "stable", v), name
+            item += 1
+    assert item == tracker.num_items()
+
+
+def test_central_event_matches_reference_digest(tracker):
+    say("gpu central event (8000 tracks)")
+    info = central()
+    with tempfile.TemporaryDirectory() as tmp:
+        p = os.path.join(tmp, "c.txt")
+        itsca.write_events(info["preset"], info["seed"], 1, p)
+        assert hashlib.sha256(open(p, "rb").read()).hexdigest() == info["events_sha256"]
+        ev = itsca.load_events(p)[0]
+    tracker.set_events([ev])
+    tracker.run()
+    c = tracker.counts(0)
+    assert list(c.tracklets) == info["counts"]["tracklets"]
+    assert list(c.cells) == info["counts"]["cells"]
+    text = itsca.dump_item(tracker, 0, ev.vertices[0])
+    assert hashlib.sha256(text.encode()).hexdigest() == info["stable_sha256"]
+
+
+def test_mixed_multiplicity_batch_matches_oracle(tracker):
+    say("gpu mixed batch vs oracle")
+    evs = itsca.generate_events(3, 3000, 6)
+    tracker.set_events(evs)
+    tracker.run()
+    for i, ev in enumerate(evs):
+        assert itsca.dump_item(tracker, i, ev.vertices[0]) == pyoracle.OracleVertex(ev, 0, stable=True).dump()
+
+
+def test_runs_are_deterministic(tracker):
+    say("gpu determinism")
+    evs = itsca.generate_events(2, 5, 2)
+    tracker.set_events(evs)
+    tracker.run()
+    a = [tracker.roads(i).copy() for i in range(2)]
+    tracker.run()
+    b = [tracker.roads(i) for i in range(2)]
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_capacity_growth_keeps_results(golden_dir):
+    say("gpu capacity growth")
+    ev = itsca.load_events(os.path.join(golden_dir, "small_s1.events.txt"))[0]
+    t = itsca.GpuTracker(device=0, capacity_scale=0.001)  # forces the overflow/grow path
+    t.set_events([ev])
+    t.run()
+    assert itsca.dump_item(t, 0, ev.vertices[0]) == fixture("small_s1", "stable", 0)
+    t.close()

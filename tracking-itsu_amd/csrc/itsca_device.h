@@ -1,0 +1,97 @@
+// itsca_device.h -- device-side state of one batch of (event, vertex) work
+// items, shared by the kernels and the runtime.
+//
+// HBM layout (all structure-of-arrays, 4-byte elements):
+//   raw clusters      per event, layer after layer, file order (upload once)
+//   sorted clusters   per work item and layer at clSeg[w*7+l].base; a work item
+//                     is one primary vertex of one event (Tracker.cxx:251-262)
+//   index tables      401 int per (item, layer): first sorted position of each
+//                     z-phi bin (PrimaryVertexContext.cxx:90-119)
+//   tracklets         compact over all (item, layer) segments, trkSeg[w*6+l]
+//   cells             compact, celSeg[w*5+l]
+//   predecessor lists compact, edgSeg[w*4+l]: for every cell of layer l+1 the
+//                     ascending list of neighbouring cells of layer l
+//   roads             compact, roadSeg[w]
+// Indices stored inside records are local to their (item, layer), exactly as
+// the reference stores positions in its per-layer vectors.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "itsca_math.h"
+
+namespace itsca {
+
+struct Seg {
+  int base;
+  int count;
+};
+
+// Stage plans: each stage's persistent kernels walk chunks of kChunk
+// elements; chunkStart[s] is the first chunk of segment s.
+constexpr int kChunk = 256;
+constexpr int kBlock = 256;
+
+enum Stage { kStTracklets = 0, kStCells = 1, kStEdges = 2, kStLevels = 3, kStRoads = 7, kNumStages = 8 };
+// kStLevels + (l-1) for cell layers 1..4 (levels of layer 0 are all 1).
+
+enum StatusSlot {
+  kStatusOverflowTracklets = 0,
+  kStatusOverflowCells = 1,
+  kStatusOverflowEdges = 2,
+  kStatusOverflowRoads = 3,
+  kNumStatus = 8
+};
+
+struct DevState {
+  int W;  // work items
+  int maxSegs;
+  // work items
+  const int* itemEvent;
+  const float* itemVertex;  // 3 per item
+  // raw clusters of the uploaded events
+  const float *rx, *ry, *rz;
+  const int *rid, *rmc;
+  const int* evLayerOff;  // 8 per event, absolute into the raw arrays
+  // sorted clusters
+  const Seg* clSeg;  // W*7
+  float *cx, *cy, *cz, *cphi, *cr;
+  int *cid, *cmc, *cbin;
+  float *sPhi, *sR;  // prep scratch, original order
+  int *sBin, *sTmp;
+  int* table;  // W*7*401
+  // tracklets
+  int *trkCnt, *trkOff;  // per sorted cluster
+  Seg* trkSeg;           // W*6
+  int *tFirst, *tSecond;
+  float *tTanL, *tPhi;
+  int tCap;
+  // cells
+  int *cellCnt, *cellOff;  // per tracklet
+  Seg* celSeg;             // W*5
+  int *c1, *c2, *c3, *ct1, *ct2, *clevel;
+  float *cnx, *cny, *cnz, *ccurv;
+  uint32_t* cpaths;  // number of level-descending paths from layer 0
+  int cCap;
+  // tracklets of layer l grouped by their second cluster
+  int* inOff;   // per sorted cluster (layers 1..4): start in inList of layer l-1
+  int* inList;  // aligned with the tracklets (trkSeg of layer l)
+  // predecessor (neighbour) lists
+  int *edgeCnt, *edgeOff;  // per cell
+  Seg* edgSeg;             // W*4
+  int* pred;
+  int eCap;
+  // roads
+  int* roadOff;  // per cell (roots in layer 4)
+  Seg* roadSeg;  // W
+  int *rCells, *rLabel, *rFake;
+  int rCap;
+  // plans
+  int* chunkStart;  // kNumStages * (maxSegs + 1)
+  int *chunkSum, *chunkOff;  // kNumStages * maxChunks
+  int maxChunks;
+  int* status;  // kNumStatus
+};
+
+}  // namespace itsca

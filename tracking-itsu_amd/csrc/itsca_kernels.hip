@@ -1,0 +1,871 @@
+// itsca_kernels.hip -- gfx950 kernels of the plain cellular-automaton track
+// finder.  Every stage is count -> scan -> fill: a count kernel computes each
+// element's number of outputs and a per-chunk sum, a single-workgroup scan
+// turns chunk sums into global offsets, and a fill kernel recomputes the
+// outputs and writes them in the reference's emission order.  There are no
+// global atomics; LDS atomics are used only where the result is made
+// order-independent afterwards.
+//
+// Compiled with -ffp-contract=off and correctly rounded fp32 divide and sqrt,
+// so each float expression rounds as the reference's x86-64 SSE build does.
+#include <hip/hip_runtime.h>
+
+#include "itsca_device.h"
+#include "itsca_kernels.h"
+
+namespace itsca {
+namespace {
+
+// ---------------------------------------------------------------------------
+// Block-level helpers (wave64)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int waveInclusiveScan(int v)
+{
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan over the block; `scratch` holds blockDim/64 + 1 ints.
+__device__ __forceinline__ int blockExclusiveScan(int v, int* scratch, int& total)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nWaves = blockDim.x >> 6;
+  const int inc = waveInclusiveScan(v);
+  if (lane == 63) scratch[wave] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int w = 0; w < nWaves; ++w) {
+      const int t = scratch[w];
+      scratch[w] = s;
+      s += t;
+    }
+    scratch[nWaves] = s;
+  }
+  __syncthreads();
+  const int ex = inc - v + scratch[wave];
+  total = scratch[nWaves];
+  __syncthreads();
+  return ex;
+}
+
+__device__ __forceinline__ int blockSum(int v, int* scratch)
+{
+  int total;
+  blockExclusiveScan(v, scratch, total);
+  return total;
+}
+
+// Chunks of a stage are dealt so that each XCD (8 per device, blocks dealt
+// round-robin) walks a contiguous range, which keeps neighbouring chunks --
+// which read the same next-layer data -- in the same L2.
+__device__ __forceinline__ int xcdBlockIndex()
+{
+  const int g = gridDim.x;
+  if ((g & 7) != 0) return blockIdx.x;
+  return (blockIdx.x & 7) * (g >> 3) + (blockIdx.x >> 3);
+}
+
+__device__ __forceinline__ int segmentOfChunk(const int* chunkStart, int S, int chunk)
+{
+  int lo = 0, hi = S;  // chunkStart[lo] <= chunk < chunkStart[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (chunkStart[mid] <= chunk)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------------------
+// Stage (a): per-layer cluster transform and stable counting sort by bin.
+// PrimaryVertexContext.cxx:31-142, Cluster.cxx:35-45.
+// One workgroup per (item, layer).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) kPrep(DevState s)
+{
+  __shared__ int hist[kTableSize];
+  __shared__ int tab[kTableSize];
+  __shared__ int scratch[1024 / 64 + 1];
+  const int w = blockIdx.x / kLayers, l = blockIdx.x % kLayers;
+  if (blockIdx.x == 0 && threadIdx.x < kNumStatus) s.status[threadIdx.x] = 0;
+  const Seg seg = s.clSeg[w * kLayers + l];
+  const int e = s.itemEvent[w];
+  const int rawBase = s.evLayerOff[e * 8 + l];
+  const float vx = s.itemVertex[3 * w], vy = s.itemVertex[3 * w + 1];
+  const int n = seg.count, base = seg.base;
+  for (int b = threadIdx.x; b < kTableSize; b += blockDim.x) hist[b] = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const float x = s.rx[rawBase + i], y = s.ry[rawBase + i], z = s.rz[rawBase + i];
+    const float dx = x - vx, dy = y - vy;
+    const float phi = clusterPhi(dx, dy);
+    const float r = sqrtf(dx * dx + dy * dy);
+    const int b = binIndex(zBinIndex(l, z), phiBinIndex(phi));
+    s.sPhi[base + i] = phi;
+    s.sR[base + i] = r;
+    s.sBin[base + i] = b;
+    s.trkCnt[base + i] = 0;
+    atomicAdd(&hist[b], 1);
+  }
+  __syncthreads();
+  // Exclusive scan of the 401 bin counts: table[b] = #clusters with bin < b.
+  int* table = s.table + (size_t)(w * kLayers + l) * kTableSize;
+  {
+    const int v = threadIdx.x < kTableSize ? hist[threadIdx.x] : 0;
+    int total;
+    const int ex = blockExclusiveScan(v, scratch, total);
+    if (threadIdx.x < kTableSize) {
+      hist[threadIdx.x] = ex;  // becomes the insertion cursor
+      tab[threadIdx.x] = ex;
+      table[threadIdx.x] = ex;
+    }
+  }
+  __syncthreads();
+  // Bucket the original indices (order inside a bucket is arbitrary here)...
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int pos = atomicAdd(&hist[s.sBin[base + i]], 1);
+    s.sTmp[base + pos] = i;
+  }
+  __threadfence_block();
+  __syncthreads();
+  // ...then place each cluster at table[bin] + (number of bucket members with a
+  // smaller original index): a stable order, independent of the atomics.
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int b = s.sBin[base + i];
+    const int lo = tab[b];
+    const int hi = b < kBins ? tab[b + 1] : n;
+    int rank = 0;
+    for (int j = lo; j < hi; ++j) rank += s.sTmp[base + j] < i ? 1 : 0;
+    const int p = base + lo + rank;
+    s.cx[p] = s.rx[rawBase + i];
+    s.cy[p] = s.ry[rawBase + i];
+    s.cz[p] = s.rz[rawBase + i];
+    s.cphi[p] = s.sPhi[base + i];
+    s.cr[p] = s.sR[base + i];
+    s.cid[p] = s.rid[rawBase + i];
+    s.cmc[p] = s.rmc[rawBase + i];
+    s.cbin[p] = b;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Plans.  A segment is one (item, layer); plan kernels turn per-segment element
+// counts into chunk ranges.  Single workgroup.
+// ---------------------------------------------------------------------------
+__device__ void planChunks(const int* counts, int S, int* chunkStart, int* scratch)
+{
+  int carry = 0;
+  for (int s0 = 0; s0 < S; s0 += blockDim.x) {
+    const int sIdx = s0 + threadIdx.x;
+    const int c = sIdx < S ? (counts[sIdx] + kChunk - 1) / kChunk : 0;
+    int total;
+    const int ex = blockExclusiveScan(c, scratch, total);
+    if (sIdx < S) chunkStart[sIdx] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) chunkStart[S] = carry;
+}
+
+// Tracklet layer l is computed only while every layer 0..l+1 has clusters
+// (early return, Tracker.cxx:49-52).
+__global__ void __launch_bounds__(1024) kPlanTracklets(DevState s, int* counts)
+{
+  __shared__ int scratch[1024 / 64 + 1];
+  const int S = s.W * kTrackletLayers;
+  for (int i = threadIdx.x; i < S; i += blockDim.x) {
+    const int w = i / kTrackletLayers, l = i % kTrackletLayers;
+    bool ok = true;
+    for (int k = 0; k <= l + 1; ++k) ok = ok && s.clSeg[w * kLayers + k].count > 0;
+    counts[i] = ok ? s.clSeg[w * kLayers + l].count : 0;
+  }
+  __syncthreads();
+  planChunks(counts, S, s.chunkStart + kStTracklets * (s.maxSegs + 1), scratch);
+}
+
+// Single-workgroup scan of a stage's chunk sums into global output offsets and
+// per-segment (base, count); flags overflow of the output capacity.
+__global__ void __launch_bounds__(1024) kScanChunks(DevState s, int stage, int S, Seg* outSeg, int cap,
+                                                    int statusSlot)
+{
+  __shared__ int scratch[1024 / 64 + 1];
+  const int* chunkStart = s.chunkStart + stage * (s.maxSegs + 1);
+  const int* sums = s.chunkSum + (size_t)stage * s.maxChunks;
+  int* offs = s.chunkOff + (size_t)stage * s.maxChunks;
+  const int nChunks = chunkStart[S];
+  int carry = 0;
+  for (int c0 = 0; c0 < nChunks; c0 += blockDim.x) {
+    const int c = c0 + threadIdx.x;
+    const int v = c < nChunks ? sums[c] : 0;
+    int total;
+    const int ex = blockExclusiveScan(v, scratch, total);
+    if (c < nChunks) offs[c] = carry + ex;
+    carry += total;
+  }
+  __syncthreads();
+  const int grand = carry;
+  // On overflow every segment of the stage is emptied, so no later stage reads
+  // past the capacity; the host grows the buffers and runs again.
+  const bool overflow = grand > cap;
+  for (int i = threadIdx.x; i < S; i += blockDim.x) {
+    const int a = chunkStart[i], b = chunkStart[i + 1];
+    const int base = a < nChunks ? offs[a] : grand;
+    const int end = b < nChunks ? offs[b] : grand;
+    outSeg[i].base = overflow ? 0 : base;
+    outSeg[i].count = overflow ? 0 : end - base;
+  }
+  if (threadIdx.x == 0 && overflow) s.status[statusSlot] = grand;
+}
+
+// ---------------------------------------------------------------------------
+// Stage (b): tracklets.  Tracker.cxx:44-117, TrackingUtils.cxx:30-48,
+// Tracklet.cxx:32-40.  One thread per cluster of layer l.
+// ---------------------------------------------------------------------------
+struct TrackletWindow {
+  int x, y, z, w;
+  int rows;
+  bool empty;
+};
+
+__device__ __forceinline__ TrackletWindow trackletWindow(float z, float r, float phi, float vz, int l)
+{
+  TrackletWindow win;
+  const float tanLambda = (z - vz) / r;
+  const float dirZ = tanLambda * (layerR(l + 1) - r) + z;
+  const float zMin = dirZ - 2 * kZCut;
+  const float phiMin = phi - kPhiCut;
+  const float zMax = dirZ + 2 * kZCut;
+  const float phiMax = phi + kPhiCut;
+  const float hz = layerHalfZ(l + 1);
+  if (zMax < -hz || zMin > hz || zMin > zMax) {
+    win.x = win.y = win.z = win.w = 0;
+  } else {
+    const int zlo = zBinIndex(l + 1, zMin), zhi = zBinIndex(l + 1, zMax);
+    win.x = zlo > 0 ? zlo : 0;
+    win.y = phiBinIndex(normalizedPhi(phiMin));
+    win.z = zhi < kZBins - 1 ? zhi : kZBins - 1;
+    win.w = phiBinIndex(normalizedPhi(phiMax));
+  }
+  // (0,0,0,0) doubles as the empty rectangle (Tracker.cxx:68-71).
+  win.empty = win.x == 0 && win.y == 0 && win.z == 0 && win.w == 0;
+  int rows = win.w - win.y + 1;
+  if (rows < 0) rows += kPhiBins;
+  win.rows = rows;
+  return win;
+}
+
+// Visits every accepted candidate of cluster i in the reference's order
+// (window row order, then ascending candidate position).
+template <typename F>
+__device__ __forceinline__ void forEachTracklet(const DevState& s, const int* table, int baseNext, int nNext,
+                                                float z, float r, float phi, float vz, int l, F&& emit)
+{
+  const TrackletWindow win = trackletWindow(z, r, phi, vz, l);
+  if (win.empty) return;
+  const float tanLambda = (z - vz) / r;
+  const float maxDeltaZ = trackletMaxDeltaZ(l);
+  int row = win.y;
+  for (int count = 0; count < win.rows; ++count) {
+    const int firstBin = binIndex(win.x, row);
+    int maxBin = firstBin + win.z - win.x + 1;
+    // Window rows past the last phi row would read past the table; the
+    // reference reads the following memory there (only reachable when phi
+    // lies within 2 ulp below 0.3).  Clamp to the table end.
+    if (maxBin > kBins) maxBin = kBins;
+    const int jFirst = table[firstBin];
+    const int jLast = table[maxBin];
+    // Inclusive upper bound as in the reference, guarded by the layer size.
+    for (int j = jFirst; j <= jLast && j < nNext; ++j) {
+      const float rn = s.cr[baseNext + j];
+      const float zn = s.cz[baseNext + j];
+      const float phin = s.cphi[baseNext + j];
+      const float deltaZ = fabsf(tanLambda * (rn - r) + z - zn);
+      const float deltaPhi = fabsf(phi - phin);
+      if (deltaZ < maxDeltaZ && (deltaPhi < kPhiCut || fabsf(deltaPhi - kTwoPi) < kPhiCut)) emit(j);
+    }
+    ++row;
+    if (row == kPhiBins) row = 0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) kTrackletsCount(DevState s)
+{
+  __shared__ int scratch[kBlock / 64 + 1];
+  const int S = s.W * kTrackletLayers;
+  const int* chunkStart = s.chunkStart + kStTracklets * (s.maxSegs + 1);
+  const int nChunks = chunkStart[S];
+  for (int chunk = xcdBlockIndex(); chunk < nChunks; chunk += gridDim.x) {
+    const int sg = segmentOfChunk(chunkStart, S, chunk);
+    const int w = sg / kTrackletLayers, l = sg % kTrackletLayers;
+    const Seg cur = s.clSeg[w * kLayers + l], next = s.clSeg[w * kLayers + l + 1];
+    const int i = (chunk - chunkStart[sg]) * kChunk + threadIdx.x;
+    int cnt = 0;
+    if (i < cur.count) {
+      const int* table = s.table + (size_t)(w * kLayers + l + 1) * kTableSize;
+      const int p = cur.base + i;
+      forEachTracklet(s, table, next.base, next.count, s.cz[p], s.cr[p], s.cphi[p], s.itemVertex[3 * w + 2], l,
+                      [&](int) { ++cnt; });
+      s.trkCnt[p] = cnt;
+    }
+    const int total = blockSum(cnt, scratch);
+    if (threadIdx.x == 0) s.chunkSum[(size_t)kStTracklets * s.maxChunks + chunk] = total;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) kTrackletsFill(DevState s)
+{
+  __shared__ int scratch[kBlock / 64 + 1];
+  const int S = s.W * kTrackletLayers;
+  const int* chunkStart = s.chunkStart + kStTracklets * (s.maxSegs + 1);
+  const int nChunks = chunkStart[S];
+  for (int chunk = xcdBlockIndex(); chunk < nChunks; chunk += gridDim.x) {
+    const int sg = segmentOfChunk(chunkStart, S, chunk);
+    const int w = sg / kTrackletLayers, l = sg % kTrackletLayers;
+    const Seg cur = s.clSeg[w * kLayers + l], next = s.clSeg[w * kLayers + l + 1];
+    const int i = (chunk - chunkStart[sg]) * kChunk + threadIdx.x;
+    const int p = cur.base + i;
+    const int cnt = i < cur.count ? s.trkCnt[p] : 0;
+    int total;
+    const int ex = blockExclusiveScan(cnt, scratch, total);
+    const int out = s.chunkOff[(size_t)kStTracklets * s.maxChunks + chunk] + ex;  // global position
+    const int segBase = s.trkSeg[sg].base;
+    if (i < cur.count) {
+      s.trkOff[p] = out - segBase;
+      if (cnt > 0) {
+        const int* table = s.table + (size_t)(w * kLayers + l + 1) * kTableSize;
+        const float x = s.cx[p], y = s.cy[p], z = s.cz[p], r = s.cr[p];
+        int k = out;
+        forEachTracklet(s, table, next.base, next.count, z, r, s.cphi[p], s.itemVertex[3 * w + 2], l, [&](int j) {
+          if (k < s.tCap) {
+            const int q = next.base + j;
+            s.tFirst[k] = i;
+            s.tSecond[k] = j;
+            s.tTanL[k] = (z - s.cz[q]) / (r - s.cr[q]);
+            s.tPhi[k] = fdlibmAtan2f(y - s.cy[q], x - s.cx[q]);
+            s.cellCnt[k] = 0;  // stays 0 where the cell stage does not run
+          }
+          ++k;
+        });
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stage (c): cells.  Tracker.cxx:119-235, Cell.cxx:24-32, MathUtils.h:53-60.
+// One thread per tracklet of layer l; candidates are the tracklets of layer
+// l+1 that start at its second cluster (a contiguous range).
+// ---------------------------------------------------------------------------
+// Cell layer l is computed only while every tracklet layer 0..l+1 is
+// non-empty (early return, Tracker.cxx:124-128).
+__global__ void __launch_bounds__(1024) kPlanCells(DevState s, int* counts)
+{
+  __shared__ int scratch[1024 / 64 + 1];
+  const int S = s.W * kCellLayers;
+  for (int i = threadIdx.x; i < S; i += blockDim.x) {
+    const int w = i / kCellLayers, l = i % kCellLayers;
+    bool ok = true;
+    for (int k = 0; k <= l + 1; ++k) ok = ok && s.trkSeg[w * kTrackletLayers + k].count > 0;
+    counts[i] = ok ? s.trkSeg[w * kTrackletLayers + l].count : 0;
+  }
+  __syncthreads();
+  planChunks(counts, S, s.chunkStart + kStCells * (s.maxSegs + 1), scratch);
+}
+
+struct CellOut {
+  float nx, ny, nz, curvature;
+};
+
+template <typename F>
+__device__ __forceinline__ void forEachCell(const DevState& s, int w, int l, int t /*global tracklet*/, F&& emit)
+{
+  const Seg tl = s.trkSeg[w * kTrackletLayers + l];
+  const Seg tn = s.trkSeg[w * kTrackletLayers + l + 1];
+  const Seg cl0 = s.clSeg[w * kLayers + l], cl1 = s.clSeg[w * kLayers + l + 1], cl2 = s.clSeg[w * kLayers + l + 2];
+  const int k = s.tSecond[t];
+  const int nCand = s.trkCnt[cl1.base + k];
+  if (nCand == 0) return;  // no tracklet of layer l+1 starts at this cluster
+  const int uFirst = s.trkOff[cl1.base + k];
+  const float vx = s.itemVertex[3 * w], vy = s.itemVertex[3 * w + 1], vz = s.itemVertex[3 * w + 2];
+  const int p1 = cl0.base + s.tFirst[t], p2 = cl1.base + k;
+  const float x1 = s.cx[p1], y1 = s.cy[p1], z1 = s.cz[p1], r1 = s.cr[p1];
+  const float x2 = s.cx[p2], y2 = s.cy[p2], r2 = s.cr[p2];
+  const float r1sq = r1 * r1;
+  const float r2sq = r2 * r2;
+  const float d1x = x2 - x1, d1y = y2 - y1, d1z = r2sq - r1sq;
+  const float tTanL = s.tTanL[t], tPhi = s.tPhi[t];
+  const float maxDeltaZ = cellMaxDeltaZ(l), maxDCA = cellMaxDCA(l);
+  (void)tl;
+  for (int iu = uFirst; iu < uFirst + nCand; ++iu) {
+    const int u = tn.base + iu;
+    const float uTanL = s.tTanL[u], uPhi = s.tPhi[u];
+    const float deltaTanLambda = fabsf(tTanL - uTanL);
+    const float deltaPhi = fabsf(tPhi - uPhi);
+    if (!(deltaTanLambda < kCellMaxDeltaTanLambda &&
+          (deltaPhi < kCellMaxDeltaPhi || fabsf(deltaPhi - kTwoPi) < kCellMaxDeltaPhi)))
+      continue;
+    const float avgTanLambda = 0.5f * (tTanL + uTanL);
+    const float dirZ = -avgTanLambda * r1 + z1;
+    const float deltaZ = fabsf(dirZ - vz);
+    if (!(deltaZ < maxDeltaZ)) continue;
+    const int p3 = cl2.base + s.tSecond[u];
+    const float x3 = s.cx[p3], y3 = s.cy[p3], r3 = s.cr[p3];
+    const float r3sq = r3 * r3;
+    const float d2x = x3 - x1, d2y = y3 - y1, d2z = r3sq - r1sq;
+    const float nx = (d1y * d2z) - (d1z * d2y);
+    const float ny = (d1z * d2x) - (d1x * d2z);
+    const float nz = (d1x * d2y) - (d1y * d2x);
+    const float norm = sqrtf(nx * nx + ny * ny + nz * nz);
+    if (norm < kFloatMin || fabsf(nz) < kFloatMin) continue;
+    const float inv = 1.0f / norm;
+    const float hx = nx * inv, hy = ny * inv, hz = nz * inv;
+    const float planeDistance = -hx * (x2 - vx) - (hy * y2 - vy) - hz * r2sq;
+    const float hz2 = hz * hz;
+    const float radius = sqrtf((1.0f - hz2 - 4.0f * planeDistance * hz) / (4.0f * hz2));
+    const float cx = -0.5f * hx / hz, cy = -0.5f * hy / hz;
+    const float dca = fabsf(radius - sqrtf(cx * cx + cy * cy));
+    if (dca > maxDCA) continue;  // NaN passes, as in the reference CPU path
+    CellOut o;
+    o.nx = hx;
+    o.ny = hy;
+    o.nz = hz;
+    o.curvature = 1.0f / radius;
+    emit(iu, o);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) kCellsCount(DevState s)
+{
+  __shared__ int scratch[kBlock / 64 + 1];
+  const int S = s.W * kCellLayers;
+  const int* chunkStart = s.chunkStart + kStCells * (s.maxSegs + 1);
+  const int nChunks = chunkStart[S];
+  for (int chunk = xcdBlockIndex(); chunk < nChunks; chunk += gridDim.x) {
+    const int sg = segmentOfChunk(chunkStart, S, chunk);
+    const int w = sg / kCellLayers, l = sg % kCellLayers;
+    const Seg tl = s.trkSeg[w * kTrackletLayers + l];
+    const int it = (chunk - chunkStart[sg]) * kChunk + threadIdx.x;
+    int cnt = 0;
+    if (it < tl.count) {
+      forEachCell(s, w, l, tl.base + it, [&](int, const CellOut&) { ++cnt; });
+      s.cellCnt[tl.base + it] = cnt;
+    }
+    const int total = blockSum(cnt, scratch);
+    if (threadIdx.x == 0) s.chunkSum[(size_t)kStCells * s.maxChunks + chunk] = total;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) kCellsFill(DevState s)
+{
+  __shared__ int scratch[kBlock / 64 + 1];
+  const int S = s.W * kCellLayers;
+  const int* chunkStart = s.chunkStart + kStCells * (s.maxSegs + 1);
+  const int nChunks = chunkStart[S];
+  for (int chunk = xcdBlockIndex(); chunk < nChunks; chunk += gridDim.x) {
+    const int sg = segmentOfChunk(chunkStart, S, chunk);
+    const int w = sg / kCellLayers, l = sg % kCellLayers;
+    const Seg tl = s.trkSeg[w * kTrackletLayers + l];
+    const int it = (chunk - chunkStart[sg]) * kChunk + threadIdx.x;
+    const int t = tl.base + it;
+    const int cnt = it < tl.count ? s.cellCnt[t] : 0;
+    int total;
+    const int ex = blockExclusiveScan(cnt, scratch, total);
+    const int out = s.chunkOff[(size_t)kStCells * s.maxChunks + chunk] + ex;
+    const int segBase = s.celSeg[sg].base;
+    if (it < tl.count) {
+      s.cellOff[t] = out - segBase;
+      if (cnt > 0) {
+        const int first = s.tFirst[t], second = s.tSecond[t];
+        const int tn = s.trkSeg[w * kTrackletLayers + l + 1].base;
+        int k = out;
+        forEachCell(s, w, l, t, [&](int iu, const CellOut& o) {
+          if (k < s.cCap) {
+            s.c1[k] = first;
+            s.c2[k] = second;
+            s.c3[k] = s.tSecond[tn + iu];
+            s.ct1[k] = it;
+            s.ct2[k] = iu;
+            s.cnx[k] = o.nx;
+            s.cny[k] = o.ny;
+            s.cnz[k] = o.nz;
+            s.ccurv[k] = o.curvature;
+            s.clevel[k] = 1;
+            s.cpaths[k] = 1;
+            s.edgeCnt[k] = 0;
+          }
+          ++k;
+        });
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tracklets of layer l grouped by second cluster, ascending within a group:
+// a counting sort in LDS, key range in passes of kKeysPerPass.  One workgroup
+// per (item, layer), layers 0..3.  Bucket order from the LDS atomics is made
+// deterministic by sorting each (small) bucket afterwards.
+// ---------------------------------------------------------------------------
+constexpr int kKeysPerPass = 8192;
+
+__global__ void __launch_bounds__(1024) kInIndex(DevState s)
+{
+  __shared__ int hist[kKeysPerPass];
+  __shared__ int scratch[1024 / 64 + 1];
+  const int w = blockIdx.x / (kCellLayers - 1), l = blockIdx.x % (kCellLayers - 1);
+  const Seg tl = s.trkSeg[w * kTrackletLayers + l];
+  const Seg keys = s.clSeg[w * kLayers + l + 1];
+  int* inOff = s.inOff + keys.base;
+  int* list = s.inList + tl.base;
+  int carry = 0;
+  for (int k0 = 0; k0 < keys.count; k0 += kKeysPerPass) {
+    const int nk = min(kKeysPerPass, keys.count - k0);
+    for (int k = threadIdx.x; k < nk; k += blockDim.x) hist[k] = 0;
+    __syncthreads();
+    for (int t = threadIdx.x; t < tl.count; t += blockDim.x) {
+      const int key = s.tSecond[tl.base + t] - k0;
+      if (key >= 0 && key < nk) atomicAdd(&hist[key], 1);
+    }
+    __syncthreads();
+    for (int k1 = 0; k1 < nk; k1 += blockDim.x) {
+      const int k = k1 + threadIdx.x;
+      const int v = k < nk ? hist[k] : 0;
+      int total;
+      const int ex = blockExclusiveScan(v, scratch, total);
+      if (k < nk) {
+        hist[k] = carry + ex;
+        inOff[k0 + k] = carry + ex;
+      }
+      carry += total;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < tl.count; t += blockDim.x) {
+      const int key = s.tSecond[tl.base + t] - k0;
+      if (key >= 0 && key < nk) list[atomicAdd(&hist[key], 1)] = t;
+    }
+    __syncthreads();
+  }
+  __threadfence_block();
+  __syncthreads();
+  for (int k = threadIdx.x; k < keys.count; k += blockDim.x) {
+    const int lo = inOff[k], hi = k + 1 < keys.count ? inOff[k + 1] : tl.count;
+    for (int a = lo + 1; a < hi; ++a) {  // insertion sort of one small bucket
+      const int v = list[a];
+      int b = a - 1;
+      while (b >= lo && list[b] > v) {
+        list[b + 1] = list[b];
+        --b;
+      }
+      list[b + 1] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stage (c): neighbours.  Tracker.cxx:416-474.  One thread per cell n of layer
+// l+1 gathers its predecessors: cells (t, u) of layer l with u = n's first
+// tracklet, for t in the tracklets of layer l ending at u's first cluster, in
+// ascending order -- the order the reference pushes them.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) kPlanEdges(DevState s, int* counts)
+{
+  __shared__ int scratch[1024 / 64 + 1];
+  const int S = s.W * (kCellLayers - 1);
+  for (int i = threadIdx.x; i < S; i += blockDim.x) {
+    const int w = i / (kCellLayers - 1), l = i % (kCellLayers - 1);
+    // Every cell of layer l+1 gets its (possibly empty) predecessor list.
+    counts[i] = s.celSeg[w * kCellLayers + l + 1].count;
+  }
+  __syncthreads();
+  planChunks(counts, S, s.chunkStart + kStEdges * (s.maxSegs + 1), scratch);
+}
+
+template <typename F>
+__device__ __forceinline__ void forEachPredecessor(const DevState& s, int w, int l, int n /*global cell l+1*/,
+                                                   F&& emit)
+{
+  const Seg tl = s.trkSeg[w * kTrackletLayers + l];
+  const Seg tn = s.trkSeg[w * kTrackletLayers + l + 1];
+  const Seg cl = s.celSeg[w * kCellLayers + l];
+  const int keyBase = s.clSeg[w * kLayers + l + 1].base, keyCount = s.clSeg[w * kLayers + l + 1].count;
+  const int u = s.ct1[n];
+  const int k = s.tFirst[tn.base + u];
+  const int lo = s.inOff[keyBase + k];
+  const int hi = k + 1 < keyCount ? s.inOff[keyBase + k + 1] : tl.count;
+  const float nx = s.cnx[n], ny = s.cny[n], nz = s.cnz[n], curv = s.ccurv[n];
+  const float maxNormal = neighbourMaxNormalDelta(l), maxCurv = neighbourMaxCurvatureDelta(l);
+  for (int a = lo; a < hi; ++a) {
+    const int t = s.inList[tl.base + a];
+    const int cnt = s.cellCnt[tl.base + t];
+    if (cnt == 0) continue;
+    // Cells of tracklet t are ordered by their second tracklet; find u.
+    int b0 = s.cellOff[tl.base + t], b1 = b0 + cnt;
+    while (b0 < b1) {
+      const int mid = (b0 + b1) >> 1;
+      if (s.ct2[cl.base + mid] < u)
+        b0 = mid + 1;
+      else
+        b1 = mid;
+    }
+    if (b0 >= s.cellOff[tl.base + t] + cnt || s.ct2[cl.base + b0] != u) continue;
+    const int c = cl.base + b0;
+    const float dx = s.cnx[c] - nx, dy = s.cny[c] - ny, dz = s.cnz[c] - nz;
+    const float deltaNormal = (dx * dx) + (dy * dy) + (dz * dz);
+    const float deltaCurvature = fabsf(s.ccurv[c] - curv);
+    if (deltaNormal < maxNormal && deltaCurvature < maxCurv) emit(b0);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) kEdgesCount(DevState s)
+{
+  __shared__ int scratch[kBlock / 64 + 1];
+  const int S = s.W * (kCellLayers - 1);
+  const int* chunkStart = s.chunkStart + kStEdges * (s.maxSegs + 1);
+  const int nChunks = chunkStart[S];
+  for (int chunk = xcdBlockIndex(); chunk < nChunks; chunk += gridDim.x) {
+    const int sg = segmentOfChunk(chunkStart, S, chunk);
+    const int w = sg / (kCellLayers - 1), l = sg % (kCellLayers - 1);
+    const Seg cn = s.celSeg[w * kCellLayers + l + 1];
+    const int in = (chunk - chunkStart[sg]) * kChunk + threadIdx.x;
+    int cnt = 0;
+    if (in < cn.count) {
+      forEachPredecessor(s, w, l, cn.base + in, [&](int) { ++cnt; });
+      s.edgeCnt[cn.base + in] = cnt;
+    }
+    const int total = blockSum(cnt, scratch);
+    if (threadIdx.x == 0) s.chunkSum[(size_t)kStEdges * s.maxChunks + chunk] = total;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) kEdgesFill(DevState s)
+{
+  __shared__ int scratch[kBlock / 64 + 1];
+  const int S = s.W * (kCellLayers - 1);
+  const int* chunkStart = s.chunkStart + kStEdges * (s.maxSegs + 1);
+  const int nChunks = chunkStart[S];
+  for (int chunk = xcdBlockIndex(); chunk < nChunks; chunk += gridDim.x) {
+    const int sg = segmentOfChunk(chunkStart, S, chunk);
+    const int w = sg / (kCellLayers - 1), l = sg % (kCellLayers - 1);
+    const Seg cn = s.celSeg[w * kCellLayers + l + 1];
+    const int in = (chunk - chunkStart[sg]) * kChunk + threadIdx.x;
+    const int n = cn.base + in;
+    const int cnt = in < cn.count ? s.edgeCnt[n] : 0;
+    int total;
+    const int ex = blockExclusiveScan(cnt, scratch, total);
+    const int out = s.chunkOff[(size_t)kStEdges * s.maxChunks + chunk] + ex;
+    if (in < cn.count) {
+      s.edgeOff[n] = out - s.edgSeg[sg].base;
+      if (cnt > 0) {
+        int k = out;
+        forEachPredecessor(s, w, l, n, [&](int c) {
+          if (k < s.eCap) s.pred[k] = c;
+          ++k;
+        });
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stage (d): CA levels and road counts.  Tracker.cxx:461-468 computes, layer
+// by layer, level(n) = max(level(n), level(c) + 1) over accepted neighbours c;
+// since layer l is final before layer l+1 is visited this equals
+// 1 + max(0, max level of predecessors).  paths(n) counts the predecessor
+// chains with levels stepping down by exactly one (the chains findTracks
+// follows, Tracker.cxx:476-571).  One launch per cell layer 1..4.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) kPlanLayer(DevState s, int* counts, int layer, int stage)
+{
+  __shared__ int scratch[1024 / 64 + 1];
+  const int S = s.W;
+  for (int w = threadIdx.x; w < S; w += blockDim.x) counts[w] = s.celSeg[w * kCellLayers + layer].count;
+  __syncthreads();
+  planChunks(counts, S, s.chunkStart + stage * (s.maxSegs + 1), scratch);
+}
+
+__global__ void __launch_bounds__(kBlock) kLevels(DevState s, int layer, int stage)
+{
+  __shared__ int scratch[kBlock / 64 + 1];
+  const int S = s.W;
+  const int* chunkStart = s.chunkStart + stage * (s.maxSegs + 1);
+  const int nChunks = chunkStart[S];
+  for (int chunk = xcdBlockIndex(); chunk < nChunks; chunk += gridDim.x) {
+    const int w = segmentOfChunk(chunkStart, S, chunk);
+    const Seg cn = s.celSeg[w * kCellLayers + layer];
+    const Seg cp = s.celSeg[w * kCellLayers + layer - 1];
+    const int in = (chunk - chunkStart[w]) * kChunk + threadIdx.x;
+    int roots = 0;
+    if (in < cn.count) {
+      const int n = cn.base + in;
+      const int cnt = s.edgeCnt[n];
+      int level = 1;
+      uint32_t paths = 1;
+      if (cnt > 0) {
+        const int off = s.edgSeg[w * (kCellLayers - 1) + layer - 1].base + s.edgeOff[n];
+        int best = 0;
+        for (int a = 0; a < cnt; ++a) best = max(best, s.clevel[cp.base + s.pred[off + a]]);
+        level = best + 1;
+        paths = 0;
+        for (int a = 0; a < cnt; ++a) {
+          const int c = cp.base + s.pred[off + a];
+          if (s.clevel[c] == level - 1) {
+            const uint32_t p = s.cpaths[c];
+            paths = paths + p < paths ? 0xffffffffu : paths + p;  // saturate
+          }
+        }
+      }
+      s.clevel[n] = level;
+      s.cpaths[n] = paths;
+      if (layer == kCellLayers - 1 && level == kCellsMinLevel) roots = (int)min(paths, 0x3fffffffu);
+    }
+    if (layer == kCellLayers - 1) {
+      const int total = blockSum(roots, scratch);
+      if (threadIdx.x == 0) s.chunkSum[(size_t)kStRoads * s.maxChunks + chunk] = total;
+    }
+  }
+}
+
+// Roads: every root (layer-4 cell of level 5) emits its chains in the order of
+// the reference's depth-first traversal -- ascending predecessor at every
+// layer -- with Moore's majority vote of the MC labels (Tracker.cxx:573-652).
+__global__ void __launch_bounds__(kBlock) kRoadsFill(DevState s)
+{
+  __shared__ int scratch[kBlock / 64 + 1];
+  constexpr int L4 = kCellLayers - 1;
+  const int S = s.W;
+  const int* chunkStart = s.chunkStart + kStRoads * (s.maxSegs + 1);
+  const int nChunks = chunkStart[S];
+  for (int chunk = xcdBlockIndex(); chunk < nChunks; chunk += gridDim.x) {
+    const int w = segmentOfChunk(chunkStart, S, chunk);
+    const Seg c4 = s.celSeg[w * kCellLayers + L4];
+    const int in = (chunk - chunkStart[w]) * kChunk + threadIdx.x;
+    int cnt = 0;
+    if (in < c4.count && s.clevel[c4.base + in] == kCellsMinLevel) cnt = (int)min(s.cpaths[c4.base + in], 0x3fffffffu);
+    int total;
+    const int ex = blockExclusiveScan(cnt, scratch, total);
+    int out = s.chunkOff[(size_t)kStRoads * s.maxChunks + chunk] + ex;
+    if (cnt == 0) continue;
+    s.roadOff[c4.base + in] = out - s.roadSeg[w].base;
+    Seg cs[kCellLayers];
+    for (int k = 0; k < kCellLayers; ++k) cs[k] = s.celSeg[w * kCellLayers + k];
+    const Seg cl[kLayers] = {s.clSeg[w * kLayers + 0], s.clSeg[w * kLayers + 1], s.clSeg[w * kLayers + 2],
+                             s.clSeg[w * kLayers + 3], s.clSeg[w * kLayers + 4], s.clSeg[w * kLayers + 5],
+                             s.clSeg[w * kLayers + 6]};
+    // Iterative depth-first walk; pos[k] indexes the predecessor list of the
+    // cell chosen at layer k+1.
+    int cell[kCellLayers];
+    int pos[kCellLayers];
+    cell[L4] = in;
+    int layer = L4;
+    pos[L4 - 1] = -1;
+    const int end = out + cnt;
+    while (layer <= L4 && out < end) {
+      if (layer == 0) {
+        if (out < s.rCap) {
+          for (int k = 0; k < kCellLayers; ++k) s.rCells[(size_t)out * kCellLayers + k] = cell[k];
+          // Moore's voting over the 7 clusters of the chain.
+          const int base0 = cs[0].base + cell[0];
+          int value = s.cmc[cl[0].base + s.c1[base0]];
+          int count = 1;
+          bool fake = false;
+          const int second = s.cmc[cl[1].base + s.c2[base0]];
+          if (second == value) {
+            ++count;
+          } else {
+            value = second;
+            count = 1;
+            fake = true;
+          }
+          for (int k = 0; k < kCellLayers; ++k) {
+            const int current = s.cmc[cl[k + 2].base + s.c3[cs[k].base + cell[k]]];
+            if (current == value) {
+              ++count;
+            } else {
+              --count;
+              fake = true;
+            }
+            if (count == 0) {
+              value = current;
+              count = 1;
+            }
+          }
+          s.rLabel[out] = value;
+          s.rFake[out] = fake ? 1 : 0;
+        }
+        ++out;
+        layer = 1;
+        continue;
+      }
+      // advance to the next predecessor of cell[layer] whose level is one less
+      const int n = cs[layer].base + cell[layer];
+      const int lvl = s.clevel[n];
+      const int off = s.edgSeg[w * (kCellLayers - 1) + layer - 1].base + s.edgeOff[n];
+      const int cnt2 = s.edgeCnt[n];
+      int a = pos[layer - 1] + 1;
+      while (a < cnt2 && s.clevel[cs[layer - 1].base + s.pred[off + a]] != lvl - 1) ++a;
+      if (a >= cnt2) {
+        ++layer;  // exhausted: back up
+        continue;
+      }
+      pos[layer - 1] = a;
+      cell[layer - 1] = s.pred[off + a];
+      --layer;
+      if (layer > 0) pos[layer - 1] = -1;
+    }
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launch sequence of one plain-CA run over the whole batch.
+// ---------------------------------------------------------------------------
+void launchPipeline(const DevState& s, const LaunchConfig& cfg, hipStream_t stream, StageTimer* timer)
+{
+  const int G = cfg.persistentBlocks;
+  int* counts = cfg.planScratch;
+  auto mark = [&](int stage) {
+    if (timer) timer->mark(stage, stream);
+  };
+  mark(0);
+  hipLaunchKernelGGL(kPrep, dim3(s.W * kLayers), dim3(1024), 0, stream, s);
+  mark(1);
+  hipLaunchKernelGGL(kPlanTracklets, dim3(1), dim3(1024), 0, stream, s, counts);
+  hipLaunchKernelGGL(kTrackletsCount, dim3(G), dim3(kBlock), 0, stream, s);
+  hipLaunchKernelGGL(kScanChunks, dim3(1), dim3(1024), 0, stream, s, (int)kStTracklets, s.W * kTrackletLayers,
+                     s.trkSeg, s.tCap, (int)kStatusOverflowTracklets);
+  hipLaunchKernelGGL(kTrackletsFill, dim3(G), dim3(kBlock), 0, stream, s);
+  mark(2);
+  hipLaunchKernelGGL(kPlanCells, dim3(1), dim3(1024), 0, stream, s, counts);
+  hipLaunchKernelGGL(kCellsCount, dim3(G), dim3(kBlock), 0, stream, s);
+  hipLaunchKernelGGL(kScanChunks, dim3(1), dim3(1024), 0, stream, s, (int)kStCells, s.W * kCellLayers, s.celSeg,
+                     s.cCap, (int)kStatusOverflowCells);
+  hipLaunchKernelGGL(kCellsFill, dim3(G), dim3(kBlock), 0, stream, s);
+  mark(3);
+  hipLaunchKernelGGL(kInIndex, dim3(s.W * (kCellLayers - 1)), dim3(1024), 0, stream, s);
+  hipLaunchKernelGGL(kPlanEdges, dim3(1), dim3(1024), 0, stream, s, counts);
+  hipLaunchKernelGGL(kEdgesCount, dim3(G), dim3(kBlock), 0, stream, s);
+  hipLaunchKernelGGL(kScanChunks, dim3(1), dim3(1024), 0, stream, s, (int)kStEdges, s.W * (kCellLayers - 1),
+                     s.edgSeg, s.eCap, (int)kStatusOverflowEdges);
+  hipLaunchKernelGGL(kEdgesFill, dim3(G), dim3(kBlock), 0, stream, s);
+  mark(4);
+  for (int layer = 1; layer < kCellLayers; ++layer) {
+    // The last level launch shares its chunk plan with the roads stage: it also
+    // writes each chunk's number of roads.
+    const int stage = layer == kCellLayers - 1 ? (int)kStRoads : kStLevels + layer - 1;
+    hipLaunchKernelGGL(kPlanLayer, dim3(1), dim3(1024), 0, stream, s, counts, layer, stage);
+    hipLaunchKernelGGL(kLevels, dim3(G), dim3(kBlock), 0, stream, s, layer, stage);
+  }
+  hipLaunchKernelGGL(kScanChunks, dim3(1), dim3(1024), 0, stream, s, (int)kStRoads, s.W, s.roadSeg, s.rCap,
+                     (int)kStatusOverflowRoads);
+  hipLaunchKernelGGL(kRoadsFill, dim3(G), dim3(kBlock), 0, stream, s);
+  mark(5);
+}
+
+}  // namespace itsca

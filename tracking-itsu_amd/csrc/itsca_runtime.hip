@@ -1,0 +1,672 @@
+// itsca_runtime.hip -- context, HBM buffers and the C ABI of libitsca_hip.so.
+//
+// Replaces the reference's CUDA runtime layer (gpu/Vector.h, gpu/Context.h,
+// gpu/Stream.h, src/CA/gpu/cuda/{Context,Stream,Utils}.cu) and its per-layer
+// host round trips (Tracker.cu:285-473): one upload per batch, a device-
+// resident pipeline with no host synchronisation inside a run, and one small
+// read-back of the output sizes per run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/itsca.h"
+#include "itsca_device.h"
+#include "itsca_kernels.h"
+
+using namespace itsca;
+
+namespace {
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf()
+  {
+    if (p) (void)hipFree(p);
+  }
+  // Grow-only; contents are not preserved.
+  hipError_t reserve(size_t want)
+  {
+    if (want <= n && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    const size_t cnt = std::max<size_t>(want, 1);
+    hipError_t e = hipMalloc(&p, cnt * sizeof(T));
+    if (e == hipSuccess) n = cnt;
+    return e;
+  }
+};
+
+struct HipError {
+  hipError_t code;
+  std::string where;
+};
+
+#define HIP_TRY(expr)                                   \
+  do {                                                  \
+    hipError_t e_ = (expr);                             \
+    if (e_ != hipSuccess) throw HipError{e_, #expr};    \
+  } while (0)
+
+}  // namespace
+
+struct itsca_ctx {
+  int device = 0;
+  int cus = 0;
+  int persistentBlocks = 0;
+  double capacityScale = 1.0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool profiling = false;
+  StageTimer timer{};
+  bool timerReady = false;
+  bool ran = false;
+
+  // staged host input
+  std::vector<float> hx, hy, hz, hvert;
+  std::vector<int> hid, hmc, hLayerOff, hEvVert;  // hLayerOff: 8 per event; hEvVert: first vertex of each event
+
+  // batch shape
+  int W = 0, E = 0;
+  std::vector<int> itemEvent;
+  std::vector<Seg> clSeg;  // 7 per item
+  long sortedClusters = 0;
+  int tCap = 0, cCap = 0, eCap = 0, rCap = 0;
+  int maxSegs = 0, maxChunks = 0;
+
+  // results of the last run
+  std::vector<Seg> trkSeg, celSeg, edgSeg, roadSeg;
+  int status[kNumStatus] = {};
+
+  // device buffers
+  DBuf<float> rx, ry, rz, itemVertex;
+  DBuf<int> rid, rmc, evLayerOff, dItemEvent;
+  DBuf<Seg> dClSeg, dTrkSeg, dCelSeg, dEdgSeg, dRoadSeg;
+  DBuf<float> cx, cy, cz, cphi, cr, sPhi, sR;
+  DBuf<int> cid, cmc, cbin, sBin, sTmp, trkCnt, trkOff, inOff, table;
+  DBuf<int> tFirst, tSecond, cellCnt, cellOff, inList;
+  DBuf<float> tTanL, tPhi;
+  DBuf<int> c1, c2, c3, ct1, ct2, clevel, edgeCnt, edgeOff, roadOff;
+  DBuf<float> cnx, cny, cnz, ccurv;
+  DBuf<uint32_t> cpaths;
+  DBuf<int> pred, rCells, rLabel, rFake;
+  DBuf<int> chunkStart, chunkSum, chunkOff, planScratch, dStatus;
+
+  ~itsca_ctx()
+  {
+    if (timerReady)
+      for (auto& e : timer.ev) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  itsca_status fail(int code, const std::string& msg)
+  {
+    err = msg;
+    return code;
+  }
+
+  void estimateCapacities()
+  {
+    double t = 0, c = 0;
+    for (int w = 0; w < W; ++w) {
+      double tw = 0;
+      for (int l = 0; l < kTrackletLayers; ++l) {
+        const double a = clSeg[w * kLayers + l].count, b = clSeg[w * kLayers + l + 1].count;
+        tw += std::max({4096.0, 24.0 * std::max(a, b), 3e-3 * a * b});
+      }
+      t += tw;
+      c += std::max(4096.0, 0.25 * tw);
+    }
+    const double sc = capacityScale > 0 ? capacityScale : 1.0;
+    auto clampCap = [](double v) { return (int)std::min(v, 2.0e9); };
+    tCap = clampCap(t * sc);
+    cCap = clampCap(c * sc);
+    eCap = cCap;
+    rCap = clampCap(std::max(4096.0, 0.5 * c * sc));
+  }
+
+  void allocateOutputs()
+  {
+    const size_t SC = (size_t)std::max<long>(sortedClusters, 1);
+    maxSegs = kLayers * std::max(W, 1);
+    maxChunks = (int)((std::max<size_t>({SC, (size_t)tCap, (size_t)cCap}) + kChunk - 1) / kChunk) + 8 * W + 1;
+    HIP_TRY(cx.reserve(SC));
+    HIP_TRY(cy.reserve(SC));
+    HIP_TRY(cz.reserve(SC));
+    HIP_TRY(cphi.reserve(SC));
+    HIP_TRY(cr.reserve(SC));
+    HIP_TRY(sPhi.reserve(SC));
+    HIP_TRY(sR.reserve(SC));
+    HIP_TRY(cid.reserve(SC));
+    HIP_TRY(cmc.reserve(SC));
+    HIP_TRY(cbin.reserve(SC));
+    HIP_TRY(sBin.reserve(SC));
+    HIP_TRY(sTmp.reserve(SC));
+    HIP_TRY(trkCnt.reserve(SC));
+    HIP_TRY(trkOff.reserve(SC));
+    HIP_TRY(inOff.reserve(SC));
+    HIP_TRY(table.reserve((size_t)W * kLayers * kTableSize));
+    HIP_TRY(tFirst.reserve(tCap));
+    HIP_TRY(tSecond.reserve(tCap));
+    HIP_TRY(tTanL.reserve(tCap));
+    HIP_TRY(tPhi.reserve(tCap));
+    HIP_TRY(cellCnt.reserve(tCap));
+    HIP_TRY(cellOff.reserve(tCap));
+    HIP_TRY(inList.reserve(tCap));
+    HIP_TRY(c1.reserve(cCap));
+    HIP_TRY(c2.reserve(cCap));
+    HIP_TRY(c3.reserve(cCap));
+    HIP_TRY(ct1.reserve(cCap));
+    HIP_TRY(ct2.reserve(cCap));
+    HIP_TRY(clevel.reserve(cCap));
+    HIP_TRY(cnx.reserve(cCap));
+    HIP_TRY(cny.reserve(cCap));
+    HIP_TRY(cnz.reserve(cCap));
+    HIP_TRY(ccurv.reserve(cCap));
+    HIP_TRY(cpaths.reserve(cCap));
+    HIP_TRY(edgeCnt.reserve(cCap));
+    HIP_TRY(edgeOff.reserve(cCap));
+    HIP_TRY(roadOff.reserve(cCap));
+    HIP_TRY(pred.reserve(eCap));
+    HIP_TRY(rCells.reserve((size_t)rCap * kCellLayers));
+    HIP_TRY(rLabel.reserve(rCap));
+    HIP_TRY(rFake.reserve(rCap));
+    HIP_TRY(dTrkSeg.reserve((size_t)W * kTrackletLayers));
+    HIP_TRY(dCelSeg.reserve((size_t)W * kCellLayers));
+    HIP_TRY(dEdgSeg.reserve((size_t)W * (kCellLayers - 1)));
+    HIP_TRY(dRoadSeg.reserve((size_t)std::max(W, 1)));
+    HIP_TRY(chunkStart.reserve((size_t)kNumStages * (maxSegs + 1)));
+    HIP_TRY(chunkSum.reserve((size_t)kNumStages * maxChunks));
+    HIP_TRY(chunkOff.reserve((size_t)kNumStages * maxChunks));
+    HIP_TRY(planScratch.reserve(maxSegs));
+    HIP_TRY(dStatus.reserve(kNumStatus));
+  }
+
+  DevState state() const
+  {
+    DevState s{};
+    s.W = W;
+    s.maxSegs = maxSegs;
+    s.itemEvent = dItemEvent.p;
+    s.itemVertex = itemVertex.p;
+    s.rx = rx.p;
+    s.ry = ry.p;
+    s.rz = rz.p;
+    s.rid = rid.p;
+    s.rmc = rmc.p;
+    s.evLayerOff = evLayerOff.p;
+    s.clSeg = dClSeg.p;
+    s.cx = cx.p;
+    s.cy = cy.p;
+    s.cz = cz.p;
+    s.cphi = cphi.p;
+    s.cr = cr.p;
+    s.cid = cid.p;
+    s.cmc = cmc.p;
+    s.cbin = cbin.p;
+    s.sPhi = sPhi.p;
+    s.sR = sR.p;
+    s.sBin = sBin.p;
+    s.sTmp = sTmp.p;
+    s.table = table.p;
+    s.trkCnt = trkCnt.p;
+    s.trkOff = trkOff.p;
+    s.trkSeg = dTrkSeg.p;
+    s.tFirst = tFirst.p;
+    s.tSecond = tSecond.p;
+    s.tTanL = tTanL.p;
+    s.tPhi = tPhi.p;
+    s.tCap = tCap;
+    s.cellCnt = cellCnt.p;
+    s.cellOff = cellOff.p;
+    s.celSeg = dCelSeg.p;
+    s.c1 = c1.p;
+    s.c2 = c2.p;
+    s.c3 = c3.p;
+    s.ct1 = ct1.p;
+    s.ct2 = ct2.p;
+    s.clevel = clevel.p;
+    s.cnx = cnx.p;
+    s.cny = cny.p;
+    s.cnz = cnz.p;
+    s.ccurv = ccurv.p;
+    s.cpaths = cpaths.p;
+    s.cCap = cCap;
+    s.inOff = inOff.p;
+    s.inList = inList.p;
+    s.edgeCnt = edgeCnt.p;
+    s.edgeOff = edgeOff.p;
+    s.edgSeg = dEdgSeg.p;
+    s.pred = pred.p;
+    s.eCap = eCap;
+    s.roadOff = roadOff.p;
+    s.roadSeg = dRoadSeg.p;
+    s.rCells = rCells.p;
+    s.rLabel = rLabel.p;
+    s.rFake = rFake.p;
+    s.rCap = rCap;
+    s.chunkStart = chunkStart.p;
+    s.chunkSum = chunkSum.p;
+    s.chunkOff = chunkOff.p;
+    s.maxChunks = maxChunks;
+    s.status = dStatus.p;
+    return s;
+  }
+
+  void enqueue()
+  {
+    LaunchConfig cfg{persistentBlocks, planScratch.p};
+    if (profiling && !timerReady) {
+      for (auto& e : timer.ev) HIP_TRY(hipEventCreate(&e));
+      timerReady = true;
+    }
+    launchPipeline(state(), cfg, stream, profiling ? &timer : nullptr);
+    HIP_TRY(hipGetLastError());
+  }
+
+  void readBack()
+  {
+    trkSeg.resize((size_t)W * kTrackletLayers);
+    celSeg.resize((size_t)W * kCellLayers);
+    edgSeg.resize((size_t)W * (kCellLayers - 1));
+    roadSeg.resize(W);
+    HIP_TRY(hipMemcpyAsync(trkSeg.data(), dTrkSeg.p, trkSeg.size() * sizeof(Seg), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(celSeg.data(), dCelSeg.p, celSeg.size() * sizeof(Seg), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(edgSeg.data(), dEdgSeg.p, edgSeg.size() * sizeof(Seg), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(roadSeg.data(), dRoadSeg.p, roadSeg.size() * sizeof(Seg), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(status, dStatus.p, sizeof status, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+  }
+};
+
+namespace {
+
+template <typename F>
+itsca_status guarded(itsca_ctx* ctx, const char* what, F&& body)
+{
+  if (!ctx) return ITSCA_ERR_ARGUMENT;
+  try {
+    return body();
+  } catch (const HipError& e) {
+    return ctx->fail(ITSCA_ERR_HIP, std::string(what) + ": " + e.where + ": " + hipGetErrorString(e.code));
+  } catch (const std::bad_alloc&) {
+    return ctx->fail(ITSCA_ERR_ARGUMENT, std::string(what) + ": host allocation failed");
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+itsca_status itsca_create(itsca_ctx** out, const itsca_config* cfg)
+{
+  if (!out) return ITSCA_ERR_ARGUMENT;
+  *out = nullptr;
+  itsca_ctx* ctx = new itsca_ctx();
+  if (cfg) {
+    ctx->device = cfg->device;
+    ctx->persistentBlocks = cfg->persistent_blocks;
+    ctx->capacityScale = cfg->capacity_scale;
+  }
+  const itsca_status st = guarded(ctx, "itsca_create", [&]() -> itsca_status {
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (ctx->device < 0 || ctx->device >= n)
+      return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_create: no HIP device " + std::to_string(ctx->device));
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, ctx->device));
+    ctx->cus = prop.multiProcessorCount;
+    if (ctx->persistentBlocks <= 0) ctx->persistentBlocks = 8 * ctx->cus;
+    HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    return ITSCA_OK;
+  });
+  // On failure the context is still returned so that the caller can read
+  // itsca_last_error; it must be destroyed like any other.
+  *out = ctx;
+  return st;
+}
+
+void itsca_destroy(itsca_ctx* ctx)
+{
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  delete ctx;
+}
+
+const char* itsca_last_error(const itsca_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+itsca_status itsca_clear_events(itsca_ctx* ctx)
+{
+  if (!ctx) return ITSCA_ERR_ARGUMENT;
+  ctx->hx.clear();
+  ctx->hy.clear();
+  ctx->hz.clear();
+  ctx->hid.clear();
+  ctx->hmc.clear();
+  ctx->hvert.clear();
+  ctx->hLayerOff.clear();
+  ctx->hEvVert.clear();
+  ctx->E = 0;
+  ctx->ran = false;
+  return ITSCA_OK;
+}
+
+itsca_status itsca_add_event(itsca_ctx* ctx, const float* x, const float* y, const float* z,
+                             const int32_t* cluster_id, const int32_t* mc_id, const int32_t* layer_offsets,
+                             const float* vertices, int num_vertices, int* event_index)
+{
+  if (!ctx) return ITSCA_ERR_ARGUMENT;
+  if (!layer_offsets || num_vertices < 0 || (num_vertices > 0 && !vertices))
+    return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_add_event: missing layer offsets or vertices");
+  const int total = layer_offsets[kLayers] - layer_offsets[0];
+  for (int l = 0; l < kLayers; ++l)
+    if (layer_offsets[l + 1] < layer_offsets[l])
+      return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_add_event: layer offsets must be non-decreasing");
+  if (total > 0 && (!x || !y || !z || !cluster_id || !mc_id))
+    return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_add_event: missing cluster arrays");
+  const int base = (int)ctx->hx.size();
+  if ((long)base + total > 0x7fffffffL) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_add_event: batch too large");
+  const int o = layer_offsets[0];
+  ctx->hx.insert(ctx->hx.end(), x + o, x + o + total);
+  ctx->hy.insert(ctx->hy.end(), y + o, y + o + total);
+  ctx->hz.insert(ctx->hz.end(), z + o, z + o + total);
+  ctx->hid.insert(ctx->hid.end(), cluster_id + o, cluster_id + o + total);
+  ctx->hmc.insert(ctx->hmc.end(), mc_id + o, mc_id + o + total);
+  for (int l = 0; l < 8; ++l) ctx->hLayerOff.push_back(base + layer_offsets[l] - o);
+  ctx->hEvVert.push_back((int)ctx->hvert.size() / 3);
+  ctx->hvert.insert(ctx->hvert.end(), vertices, vertices + 3 * num_vertices);
+  if (event_index) *event_index = ctx->E;
+  ++ctx->E;
+  ctx->ran = false;
+  return ITSCA_OK;
+}
+
+itsca_status itsca_upload(itsca_ctx* ctx)
+{
+  return guarded(ctx, "itsca_upload", [&]() -> itsca_status {
+    HIP_TRY(hipSetDevice(ctx->device));
+    // Work items: every vertex of every event, in event order.
+    ctx->itemEvent.clear();
+    ctx->clSeg.clear();
+    std::vector<float> itemVtx;
+    long sc = 0;
+    const int nv = (int)ctx->hvert.size() / 3;
+    for (int e = 0; e < ctx->E; ++e) {
+      const int v0 = ctx->hEvVert[e];
+      const int v1 = e + 1 < ctx->E ? ctx->hEvVert[e + 1] : nv;
+      for (int v = v0; v < v1; ++v) {
+        ctx->itemEvent.push_back(e);
+        itemVtx.insert(itemVtx.end(), {ctx->hvert[3 * v], ctx->hvert[3 * v + 1], ctx->hvert[3 * v + 2]});
+        for (int l = 0; l < kLayers; ++l) {
+          const int n = ctx->hLayerOff[8 * e + l + 1] - ctx->hLayerOff[8 * e + l];
+          ctx->clSeg.push_back(Seg{(int)sc, n});
+          sc += n;
+        }
+      }
+    }
+    if (sc > 0x7fffffffL) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_upload: batch too large");
+    ctx->W = (int)ctx->itemEvent.size();
+    ctx->sortedClusters = sc;
+    const size_t NR = std::max<size_t>(ctx->hx.size(), 1);
+    HIP_TRY(ctx->rx.reserve(NR));
+    HIP_TRY(ctx->ry.reserve(NR));
+    HIP_TRY(ctx->rz.reserve(NR));
+    HIP_TRY(ctx->rid.reserve(NR));
+    HIP_TRY(ctx->rmc.reserve(NR));
+    HIP_TRY(ctx->evLayerOff.reserve(std::max<size_t>(ctx->hLayerOff.size(), 1)));
+    HIP_TRY(ctx->dItemEvent.reserve(std::max(ctx->W, 1)));
+    HIP_TRY(ctx->itemVertex.reserve(std::max<size_t>(itemVtx.size(), 1)));
+    HIP_TRY(ctx->dClSeg.reserve(std::max<size_t>(ctx->clSeg.size(), 1)));
+    auto put = [&](void* dst, const void* src, size_t bytes) {
+      if (bytes) HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    };
+    put(ctx->rx.p, ctx->hx.data(), ctx->hx.size() * 4);
+    put(ctx->ry.p, ctx->hy.data(), ctx->hy.size() * 4);
+    put(ctx->rz.p, ctx->hz.data(), ctx->hz.size() * 4);
+    put(ctx->rid.p, ctx->hid.data(), ctx->hid.size() * 4);
+    put(ctx->rmc.p, ctx->hmc.data(), ctx->hmc.size() * 4);
+    put(ctx->evLayerOff.p, ctx->hLayerOff.data(), ctx->hLayerOff.size() * 4);
+    put(ctx->dItemEvent.p, ctx->itemEvent.data(), ctx->itemEvent.size() * 4);
+    put(ctx->itemVertex.p, itemVtx.data(), itemVtx.size() * 4);
+    put(ctx->dClSeg.p, ctx->clSeg.data(), ctx->clSeg.size() * sizeof(Seg));
+    ctx->estimateCapacities();
+    ctx->allocateOutputs();
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->ran = false;
+    return ITSCA_OK;
+  });
+}
+
+itsca_status itsca_run_async(itsca_ctx* ctx)
+{
+  return guarded(ctx, "itsca_run_async", [&]() -> itsca_status {
+    if (ctx->W == 0) return ctx->fail(ITSCA_ERR_STATE, "itsca_run_async: nothing uploaded");
+    HIP_TRY(hipSetDevice(ctx->device));
+    ctx->enqueue();
+    return ITSCA_OK;
+  });
+}
+
+itsca_status itsca_finish(itsca_ctx* ctx)
+{
+  return guarded(ctx, "itsca_finish", [&]() -> itsca_status {
+    HIP_TRY(hipSetDevice(ctx->device));
+    ctx->readBack();
+    for (int k = 0; k < kNumStatus; ++k)
+      if (ctx->status[k] != 0)
+        return ctx->fail(ITSCA_ERR_CAPACITY, "itsca_finish: output capacity exceeded (stage " + std::to_string(k) +
+                                                 ", needed " + std::to_string(ctx->status[k]) + ")");
+    ctx->ran = true;
+    return ITSCA_OK;
+  });
+}
+
+itsca_status itsca_run(itsca_ctx* ctx)
+{
+  return guarded(ctx, "itsca_run", [&]() -> itsca_status {
+    if (ctx->W == 0) return ctx->fail(ITSCA_ERR_STATE, "itsca_run: nothing uploaded");
+    HIP_TRY(hipSetDevice(ctx->device));
+    for (int attempt = 0; attempt < 6; ++attempt) {
+      ctx->enqueue();
+      ctx->readBack();
+      bool grow = false;
+      auto need = [&](int slot, int& cap) {
+        if (ctx->status[slot] != 0) {
+          cap = (int)std::min(2.0e9, 1.25 * (double)ctx->status[slot] + 4096);
+          grow = true;
+        }
+      };
+      need(kStatusOverflowTracklets, ctx->tCap);
+      need(kStatusOverflowCells, ctx->cCap);
+      need(kStatusOverflowEdges, ctx->eCap);
+      need(kStatusOverflowRoads, ctx->rCap);
+      if (!grow) {
+        ctx->ran = true;
+        return ITSCA_OK;
+      }
+      ctx->allocateOutputs();
+    }
+    return ctx->fail(ITSCA_ERR_CAPACITY, "itsca_run: outputs kept exceeding the grown capacity");
+  });
+}
+
+int itsca_num_items(const itsca_ctx* ctx) { return ctx ? ctx->W : 0; }
+
+itsca_status itsca_get_counts(itsca_ctx* ctx, int item, itsca_counts* out)
+{
+  if (!ctx || !out) return ITSCA_ERR_ARGUMENT;
+  if (!ctx->ran) return ctx->fail(ITSCA_ERR_STATE, "itsca_get_counts: no completed run");
+  if (item < 0 || item >= ctx->W) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_get_counts: bad item");
+  for (int l = 0; l < kLayers; ++l) out->clusters[l] = ctx->clSeg[item * kLayers + l].count;
+  for (int l = 0; l < kTrackletLayers; ++l) out->tracklets[l] = ctx->trkSeg[item * kTrackletLayers + l].count;
+  for (int l = 0; l < kCellLayers; ++l) out->cells[l] = ctx->celSeg[item * kCellLayers + l].count;
+  for (int l = 0; l < kCellLayers - 1; ++l) out->neighbours[l] = ctx->edgSeg[item * (kCellLayers - 1) + l].count;
+  out->roads = ctx->roadSeg[item].count;
+  return ITSCA_OK;
+}
+
+itsca_status itsca_download(itsca_ctx* ctx, int item, int what, int layer, void* dst, size_t cap_bytes,
+                            size_t* written)
+{
+  return guarded(ctx, "itsca_download", [&]() -> itsca_status {
+    if (!ctx->ran) return ctx->fail(ITSCA_ERR_STATE, "itsca_download: no completed run");
+    if (item < 0 || item >= ctx->W) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_download: bad item");
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto layerOk = [&](int lo, int hi) { return layer >= lo && layer <= hi; };
+    std::vector<int32_t> rows;
+    auto fetch = [&](const auto* src, size_t n) {
+      using T = std::remove_cv_t<std::remove_pointer_t<decltype(src)>>;
+      std::vector<T> h(n);
+      if (n) HIP_TRY(hipMemcpyAsync(h.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+      return h;
+    };
+    auto interleave = [&](std::initializer_list<const std::vector<int32_t>*> cols, size_t n) {
+      rows.resize(n * cols.size());
+      size_t c = 0;
+      for (const auto* col : cols) {
+        for (size_t i = 0; i < n; ++i) rows[i * cols.size() + c] = (*col)[i];
+        ++c;
+      }
+    };
+    auto asInt = [](const std::vector<float>& f) {
+      std::vector<int32_t> v(f.size());
+      std::memcpy(v.data(), f.data(), f.size() * 4);
+      return v;
+    };
+    switch (what) {
+      case ITSCA_CLUSTERS: {
+        if (!layerOk(0, 6)) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_download: bad layer");
+        const Seg g = ctx->clSeg[item * kLayers + layer];
+        auto id = fetch(ctx->cid.p + g.base, g.count), bin = fetch(ctx->cbin.p + g.base, g.count),
+             mc = fetch(ctx->cmc.p + g.base, g.count);
+        auto phi = asInt(fetch(ctx->cphi.p + g.base, g.count)), r = asInt(fetch(ctx->cr.p + g.base, g.count)),
+             x = asInt(fetch(ctx->cx.p + g.base, g.count)), y = asInt(fetch(ctx->cy.p + g.base, g.count)),
+             z = asInt(fetch(ctx->cz.p + g.base, g.count));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        interleave({&id, &phi, &r, &bin, &x, &y, &z, &mc}, g.count);
+        break;
+      }
+      case ITSCA_INDEX_TABLE: {
+        if (!layerOk(1, 6)) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_download: bad layer");
+        rows = fetch(ctx->table.p + (size_t)(item * kLayers + layer) * kTableSize, kTableSize);
+        break;
+      }
+      case ITSCA_TRACKLETS: {
+        if (!layerOk(0, 5)) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_download: bad layer");
+        const Seg g = ctx->trkSeg[item * kTrackletLayers + layer];
+        auto a = fetch(ctx->tFirst.p + g.base, g.count), b = fetch(ctx->tSecond.p + g.base, g.count);
+        auto t = asInt(fetch(ctx->tTanL.p + g.base, g.count)), p = asInt(fetch(ctx->tPhi.p + g.base, g.count));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        interleave({&a, &b, &t, &p}, g.count);
+        break;
+      }
+      case ITSCA_TRACKLET_LOOKUP: {
+        if (!layerOk(0, 4)) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_download: bad layer");
+        const Seg g = ctx->clSeg[item * kLayers + layer + 1];
+        auto cnt = fetch(ctx->trkCnt.p + g.base, g.count), off = fetch(ctx->trkOff.p + g.base, g.count);
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        rows.resize(g.count);
+        for (int i = 0; i < g.count; ++i) rows[i] = cnt[i] > 0 ? off[i] : -1;
+        break;
+      }
+      case ITSCA_CELLS: {
+        if (!layerOk(0, 4)) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_download: bad layer");
+        const Seg g = ctx->celSeg[item * kCellLayers + layer];
+        auto a = fetch(ctx->c1.p + g.base, g.count), b = fetch(ctx->c2.p + g.base, g.count),
+             c = fetch(ctx->c3.p + g.base, g.count), t1 = fetch(ctx->ct1.p + g.base, g.count),
+             t2 = fetch(ctx->ct2.p + g.base, g.count), lv = fetch(ctx->clevel.p + g.base, g.count);
+        auto nx = asInt(fetch(ctx->cnx.p + g.base, g.count)), ny = asInt(fetch(ctx->cny.p + g.base, g.count)),
+             nz = asInt(fetch(ctx->cnz.p + g.base, g.count)), cv = asInt(fetch(ctx->ccurv.p + g.base, g.count));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        interleave({&a, &b, &c, &t1, &t2, &nx, &ny, &nz, &cv, &lv}, g.count);
+        break;
+      }
+      case ITSCA_CELL_LOOKUP: {
+        if (!layerOk(0, 3)) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_download: bad layer");
+        // indexed by the tracklets of layer+1; set only where cell layer
+        // layer+1 was computed
+        const Seg g = ctx->trkSeg[item * kTrackletLayers + layer + 1];
+        bool computed = true;
+        for (int k = 0; k <= layer + 2; ++k) computed = computed && ctx->trkSeg[item * kTrackletLayers + k].count > 0;
+        rows.assign(g.count, -1);
+        if (computed) {
+          auto cnt = fetch(ctx->cellCnt.p + g.base, g.count), off = fetch(ctx->cellOff.p + g.base, g.count);
+          HIP_TRY(hipStreamSynchronize(ctx->stream));
+          for (int i = 0; i < g.count; ++i) rows[i] = cnt[i] > 0 ? off[i] : -1;
+        }
+        break;
+      }
+      case ITSCA_NEIGHBOUR_OFFSETS: {
+        if (!layerOk(0, 3)) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_download: bad layer");
+        const Seg g = ctx->celSeg[item * kCellLayers + layer + 1];
+        auto off = fetch(ctx->edgeOff.p + g.base, g.count);
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        rows = off;
+        rows.push_back(ctx->edgSeg[item * (kCellLayers - 1) + layer].count);
+        break;
+      }
+      case ITSCA_NEIGHBOURS: {
+        if (!layerOk(0, 3)) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_download: bad layer");
+        const Seg g = ctx->edgSeg[item * (kCellLayers - 1) + layer];
+        rows = fetch(ctx->pred.p + g.base, g.count);
+        break;
+      }
+      case ITSCA_ROADS: {
+        const Seg g = ctx->roadSeg[item];
+        auto cells = fetch(ctx->rCells.p + (size_t)g.base * kCellLayers, (size_t)g.count * kCellLayers);
+        auto lab = fetch(ctx->rLabel.p + g.base, g.count), fake = fetch(ctx->rFake.p + g.base, g.count);
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        rows.resize((size_t)g.count * 8);
+        for (int i = 0; i < g.count; ++i) {
+          int size = 0;
+          for (int k = 0; k < kCellLayers; ++k) {
+            rows[(size_t)i * 8 + k] = cells[(size_t)i * kCellLayers + k];
+            size += cells[(size_t)i * kCellLayers + k] >= 0 ? 1 : 0;
+          }
+          rows[(size_t)i * 8 + 5] = size;
+          rows[(size_t)i * 8 + 6] = lab[i];
+          rows[(size_t)i * 8 + 7] = fake[i];
+        }
+        break;
+      }
+      default:
+        return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_download: unknown record kind");
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const size_t bytes = rows.size() * 4;
+    if (written) *written = bytes;
+    if (bytes > cap_bytes) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_download: destination too small");
+    if (bytes) std::memcpy(dst, rows.data(), bytes);
+    return ITSCA_OK;
+  });
+}
+
+itsca_status itsca_set_profiling(itsca_ctx* ctx, int enable)
+{
+  if (!ctx) return ITSCA_ERR_ARGUMENT;
+  ctx->profiling = enable != 0;
+  return ITSCA_OK;
+}
+
+itsca_status itsca_stage_times(itsca_ctx* ctx, float ms[5])
+{
+  return guarded(ctx, "itsca_stage_times", [&]() -> itsca_status {
+    if (!ctx->timerReady) return ctx->fail(ITSCA_ERR_STATE, "itsca_stage_times: profiling was not enabled");
+    HIP_TRY(hipEventSynchronize(ctx->timer.ev[kNumTimedStages]));
+    for (int k = 0; k < kNumTimedStages; ++k) HIP_TRY(hipEventElapsedTime(&ms[k], ctx->timer.ev[k], ctx->timer.ev[k + 1]));
+    return ITSCA_OK;
+  });
+}
+
+}  // extern "C"

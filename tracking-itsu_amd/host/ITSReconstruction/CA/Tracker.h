@@ -1,0 +1,58 @@
+// Tracker.h -- the reference tracker entry point, Tracker<IsGPU>
+// (include/ITSReconstruction/CA/Tracker.h:50-79), on the gfx950 back end.
+//
+// Tracker<true> runs the whole plain CA (index tables, tracklets, cells,
+// neighbours, levels, roads and MC labels) on the device through the C ABI of
+// libitsca_hip.so.  Errors are thrown as std::runtime_error carrying the
+// library's message, like the reference's CUDA path (gpu/cuda/Utils.cu:27-38).
+// The reference's CPU back end, Tracker<false>, is not part of this library.
+#pragma once
+
+#include <fstream>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "Event.h"
+#include "PrimaryVertexContext.h"
+#include "Road.h"
+
+struct itsca_ctx;
+
+namespace o2 {
+namespace ITS {
+namespace CA {
+
+template <bool IsGPU>
+class Tracker;
+
+template <>
+class Tracker<true> {
+ public:
+  explicit Tracker(int device = 0);
+  ~Tracker();
+  Tracker(const Tracker&) = delete;
+  Tracker& operator=(const Tracker&) = delete;
+
+  // One vector of roads per primary vertex (Tracker.cxx:244-265).
+  std::vector<std::vector<Road>> clustersToTracks(const Event& event);
+  // Same for a batch of events in one device run: result[e][v].
+  std::vector<std::vector<std::vector<Road>>> clustersToTracks(const std::vector<Event>& events);
+  // Per-vertex device time per stage, tab-separated like the reference's
+  // TimeBenchmark output (Tracker.cxx:368-402): prep, tracklets, cells,
+  // neighbours, levels+roads, total.
+  std::vector<std::vector<Road>> clustersToTracksTimeBenchmark(const Event& event, std::ofstream& out);
+
+  // Host snapshot of the device state of vertex `vertex` of the last run
+  // (the batch's work item order: event by event, vertex by vertex).
+  PrimaryVertexContext getPrimaryVertexContext(int item) const;
+
+ private:
+  void runBatch(const std::vector<const Event*>& events);
+  std::vector<Road> downloadRoads(int item) const;
+  itsca_ctx* mCtx = nullptr;
+};
+
+}  // namespace CA
+}  // namespace ITS
+}  // namespace o2

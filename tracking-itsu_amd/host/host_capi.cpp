@@ -1,0 +1,86 @@
+// host_capi.cpp -- C entry points of libitsca.so for event batches
+// (include/itsca_host.h).
+#include <cstring>
+#include <new>
+
+#include "events.hpp"
+#include "itsca_host.h"
+
+struct itsca_events {
+  itsca::EventBatch batch;
+};
+
+extern "C" {
+
+itsca_events* itsca_events_generate(int preset, uint64_t seed, int num_events)
+{
+  if (num_events < 0 || (preset != 1 && preset != 2 && preset != 3 && preset != 5)) return nullptr;
+  itsca_events* ev = new (std::nothrow) itsca_events();
+  if (!ev) return nullptr;
+  itsca::generateEvents(preset, seed, num_events, ev->batch);
+  return ev;
+}
+
+itsca_events* itsca_events_load(const char* path)
+{
+  if (!path) return nullptr;
+  itsca_events* ev = new (std::nothrow) itsca_events();
+  if (!ev) return nullptr;
+  std::string err;
+  if (!itsca::loadEvents(path, ev->batch, err)) {
+    delete ev;
+    return nullptr;
+  }
+  return ev;
+}
+
+int itsca_events_write(const itsca_events* ev, const char* events_path, const char* labels_path)
+{
+  if (!ev || !events_path) return -1;
+  std::string err;
+  if (!itsca::writeEvents(ev->batch, events_path, err)) return -1;
+  if (labels_path && !itsca::writeLabels(ev->batch, labels_path, err)) return -1;
+  return 0;
+}
+
+void itsca_events_free(itsca_events* ev) { delete ev; }
+
+int itsca_events_count(const itsca_events* ev) { return ev ? (int)ev->batch.events.size() : -1; }
+
+int itsca_events_num_vertices(const itsca_events* ev, int event)
+{
+  if (!ev || event < 0 || event >= (int)ev->batch.events.size()) return -1;
+  return ev->batch.events[event].numVertices();
+}
+
+int itsca_events_layer_offsets(const itsca_events* ev, int event, int32_t offsets[8])
+{
+  if (!ev || !offsets || event < 0 || event >= (int)ev->batch.events.size()) return -1;
+  const itsca::RawEvent& e = ev->batch.events[event];
+  offsets[0] = 0;
+  for (int l = 0; l < itsca::kNumLayers; ++l) offsets[l + 1] = offsets[l] + e.layerSize(l);
+  return 0;
+}
+
+int itsca_events_copy(const itsca_events* ev, int event, float* x, float* y, float* z, int32_t* cluster_id,
+                      int32_t* mc_id, float* vertices)
+{
+  if (!ev || event < 0 || event >= (int)ev->batch.events.size()) return -1;
+  const itsca::RawEvent& e = ev->batch.events[event];
+  size_t o = 0;
+  for (int l = 0; l < itsca::kNumLayers; ++l) {
+    const size_t n = (size_t)e.layerSize(l);
+    if (n) {
+      if (x) std::memcpy(x + o, e.x[l].data(), n * 4);
+      if (y) std::memcpy(y + o, e.y[l].data(), n * 4);
+      if (z) std::memcpy(z + o, e.z[l].data(), n * 4);
+      if (cluster_id) std::memcpy(cluster_id + o, e.id[l].data(), n * 4);
+      if (mc_id) std::memcpy(mc_id + o, e.mc[l].data(), n * 4);
+    }
+    o += n;
+  }
+  if (vertices && !e.vertices.empty()) std::memcpy(vertices, e.vertices.data(), e.vertices.size() * 4);
+  return 0;
+}
+
+}  // extern "C"

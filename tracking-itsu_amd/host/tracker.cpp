@@ -1,0 +1,293 @@
+// tracker.cpp -- Tracker<true> and IOUtils of the C++ API, on top of the C ABI
+// of libitsca_hip.so (include/itsca.h).
+#include <cstdint>
+#include <cstring>
+#include <iostream>
+#include <sstream>
+#include <stdexcept>
+#include <tuple>
+#include <unordered_set>
+
+#include "ITSReconstruction/CA/IOUtils.h"
+#include "ITSReconstruction/CA/Tracker.h"
+#include "itsca.h"
+
+namespace o2 {
+namespace ITS {
+namespace CA {
+
+namespace {
+
+void check(itsca_ctx* ctx, itsca_status st, const char* what)
+{
+  if (st != ITSCA_OK) throw std::runtime_error(std::string(what) + ": " + itsca_last_error(ctx));
+}
+
+template <typename T>
+std::vector<T> download(itsca_ctx* ctx, int item, int what, int layer, size_t rows, size_t fields)
+{
+  std::vector<int32_t> buf(rows * fields + 1);
+  size_t written = 0;
+  check(ctx, itsca_download(ctx, item, what, layer, buf.data(), buf.size() * 4, &written), "itsca_download");
+  buf.resize(written / 4);
+  std::vector<T> out;
+  (void)out;
+  return std::vector<T>(buf.begin(), buf.end());
+}
+
+float asFloat(int32_t v)
+{
+  float f;
+  std::memcpy(&f, &v, 4);
+  return f;
+}
+
+}  // namespace
+
+void Event::printPrimaryVertices() const
+{
+  for (const float3& v : mPrimaryVertices) std::cout << "-1\t" << v.x << "\t" << v.y << "\t" << v.z << std::endl;
+}
+
+Tracker<true>::Tracker(int device)
+{
+  itsca_config cfg{};
+  cfg.device = device;
+  const itsca_status st = itsca_create(&mCtx, &cfg);
+  if (st != ITSCA_OK) {
+    const std::string msg = mCtx ? itsca_last_error(mCtx) : "itsca_create failed";
+    itsca_destroy(mCtx);
+    mCtx = nullptr;
+    throw std::runtime_error(msg);
+  }
+}
+
+Tracker<true>::~Tracker() { itsca_destroy(mCtx); }
+
+void Tracker<true>::runBatch(const std::vector<const Event*>& events)
+{
+  check(mCtx, itsca_clear_events(mCtx), "itsca_clear_events");
+  std::vector<float> x, y, z, v;
+  std::vector<int32_t> id, mc;
+  for (const Event* ev : events) {
+    x.clear(), y.clear(), z.clear(), id.clear(), mc.clear(), v.clear();
+    int32_t offs[8];
+    offs[0] = 0;
+    for (int l = 0; l < Constants::ITS::LayersNumber; ++l) {
+      for (const Cluster& c : ev->getLayer(l).getClusters()) {
+        x.push_back(c.xCoordinate);
+        y.push_back(c.yCoordinate);
+        z.push_back(c.zCoordinate);
+        id.push_back(c.clusterId);
+        mc.push_back(c.monteCarloId);
+      }
+      offs[l + 1] = (int32_t)x.size();
+    }
+    for (int k = 0; k < ev->getPrimaryVerticesNum(); ++k) {
+      const float3& p = ev->getPrimaryVertex(k);
+      v.insert(v.end(), {p.x, p.y, p.z});
+    }
+    check(mCtx,
+          itsca_add_event(mCtx, x.data(), y.data(), z.data(), id.data(), mc.data(), offs, v.data(),
+                          ev->getPrimaryVerticesNum(), nullptr),
+          "itsca_add_event");
+  }
+  check(mCtx, itsca_upload(mCtx), "itsca_upload");
+  check(mCtx, itsca_run(mCtx), "itsca_run");
+}
+
+std::vector<Road> Tracker<true>::downloadRoads(int item) const
+{
+  itsca_counts n;
+  check(mCtx, itsca_get_counts(mCtx, item, &n), "itsca_get_counts");
+  const std::vector<int32_t> rows = download<int32_t>(mCtx, item, ITSCA_ROADS, 0, n.roads, 8);
+  std::vector<Road> roads;
+  roads.reserve(n.roads);
+  for (int i = 0; i < n.roads; ++i) {
+    Road r;
+    for (int k = 0; k < Constants::ITS::CellsPerRoad; ++k)
+      if (rows[(size_t)i * 8 + k] >= 0) r.addCell(k, rows[(size_t)i * 8 + k]);
+    r.setLabel(rows[(size_t)i * 8 + 6]);
+    r.setFakeRoad(rows[(size_t)i * 8 + 7] != 0);
+    roads.push_back(r);
+  }
+  return roads;
+}
+
+std::vector<std::vector<Road>> Tracker<true>::clustersToTracks(const Event& event)
+{
+  runBatch({&event});
+  std::vector<std::vector<Road>> roads;
+  for (int v = 0; v < event.getPrimaryVerticesNum(); ++v) roads.push_back(downloadRoads(v));
+  return roads;
+}
+
+std::vector<std::vector<std::vector<Road>>> Tracker<true>::clustersToTracks(const std::vector<Event>& events)
+{
+  std::vector<const Event*> ptrs;
+  for (const Event& e : events) ptrs.push_back(&e);
+  runBatch(ptrs);
+  std::vector<std::vector<std::vector<Road>>> out(events.size());
+  int item = 0;
+  for (size_t e = 0; e < events.size(); ++e)
+    for (int v = 0; v < events[e].getPrimaryVerticesNum(); ++v) out[e].push_back(downloadRoads(item++));
+  return out;
+}
+
+std::vector<std::vector<Road>> Tracker<true>::clustersToTracksTimeBenchmark(const Event& event, std::ofstream& out)
+{
+  check(mCtx, itsca_set_profiling(mCtx, 1), "itsca_set_profiling");
+  std::vector<std::vector<Road>> roads;
+  for (int v = 0; v < event.getPrimaryVerticesNum(); ++v) {
+    Event single(event.getEventId());
+    single.addPrimaryVertex(event.getPrimaryVertex(v).x, event.getPrimaryVertex(v).y, event.getPrimaryVertex(v).z);
+    for (int l = 0; l < Constants::ITS::LayersNumber; ++l)
+      for (const Cluster& c : event.getLayer(l).getClusters())
+        single.pushClusterToLayer(l, c.clusterId, c.xCoordinate, c.yCoordinate, c.zCoordinate, c.alphaAngle,
+                                  c.monteCarloId);
+    runBatch({&single});
+    float ms[5];
+    check(mCtx, itsca_stage_times(mCtx, ms), "itsca_stage_times");
+    float total = 0;
+    for (float t : ms) {
+      out << t << "\t";
+      total += t;
+    }
+    out << total << std::endl;
+    roads.push_back(downloadRoads(0));
+  }
+  check(mCtx, itsca_set_profiling(mCtx, 0), "itsca_set_profiling");
+  return roads;
+}
+
+PrimaryVertexContext Tracker<true>::getPrimaryVertexContext(int item) const
+{
+  itsca_counts n;
+  check(mCtx, itsca_get_counts(mCtx, item, &n), "itsca_get_counts");
+  PrimaryVertexContext ctx;
+  for (int l = 0; l < Constants::ITS::LayersNumber; ++l) {
+    const auto rows = download<int32_t>(mCtx, item, ITSCA_CLUSTERS, l, n.clusters[l], 8);
+    for (int i = 0; i < n.clusters[l]; ++i) {
+      const int32_t* r = &rows[(size_t)i * 8];
+      Cluster c(r[0], l, asFloat(r[4]), asFloat(r[5]), asFloat(r[6]), 0.0f, r[7]);
+      c.phiCoordinate = asFloat(r[1]);
+      c.rCoordinate = asFloat(r[2]);
+      c.indexTableBinIndex = r[3];
+      ctx.mClusters[l].push_back(c);
+    }
+  }
+  for (int l = 1; l < Constants::ITS::LayersNumber; ++l) {
+    const auto t = download<int32_t>(mCtx, item, ITSCA_INDEX_TABLE, l, PrimaryVertexContext::TableSize, 1);
+    for (int b = 0; b < PrimaryVertexContext::TableSize; ++b) ctx.mIndexTables[l - 1][b] = t[b];
+  }
+  for (int l = 0; l < Constants::ITS::TrackletsPerRoad; ++l) {
+    const auto rows = download<int32_t>(mCtx, item, ITSCA_TRACKLETS, l, n.tracklets[l], 4);
+    for (int i = 0; i < n.tracklets[l]; ++i)
+      ctx.mTracklets[l].push_back(
+          {rows[(size_t)i * 4], rows[(size_t)i * 4 + 1], asFloat(rows[(size_t)i * 4 + 2]), asFloat(rows[(size_t)i * 4 + 3])});
+  }
+  for (int l = 0; l < Constants::ITS::CellsPerRoad; ++l)
+    ctx.mTrackletsLookupTable[l] = download<int>(mCtx, item, ITSCA_TRACKLET_LOOKUP, l, n.clusters[l + 1], 1);
+  for (int l = 0; l < Constants::ITS::CellsPerRoad; ++l) {
+    const auto rows = download<int32_t>(mCtx, item, ITSCA_CELLS, l, n.cells[l], 10);
+    for (int i = 0; i < n.cells[l]; ++i) {
+      const int32_t* r = &rows[(size_t)i * 10];
+      ctx.mCells[l].emplace_back(r[0], r[1], r[2], r[3], r[4], float3{asFloat(r[5]), asFloat(r[6]), asFloat(r[7])},
+                                 asFloat(r[8]), r[9]);
+    }
+  }
+  for (int l = 0; l < Constants::ITS::CellsPerRoad - 1; ++l) {
+    ctx.mCellsLookupTable[l] = download<int>(mCtx, item, ITSCA_CELL_LOOKUP, l, n.tracklets[l + 1], 1);
+    const auto off = download<int>(mCtx, item, ITSCA_NEIGHBOUR_OFFSETS, l, n.cells[l + 1] + 1, 1);
+    const auto lst = download<int>(mCtx, item, ITSCA_NEIGHBOURS, l, n.neighbours[l], 1);
+    ctx.mCellsNeighbours[l].resize(n.cells[l + 1]);
+    for (int c = 0; c < n.cells[l + 1]; ++c)
+      ctx.mCellsNeighbours[l][c].assign(lst.begin() + off[c], lst.begin() + off[c + 1]);
+  }
+  ctx.mRoads = downloadRoads(item);
+  return ctx;
+}
+
+namespace IOUtils {
+
+// IOUtils.cxx:38-78
+std::vector<Event> loadEventData(const std::string& fileName)
+{
+  std::vector<Event> events;
+  std::ifstream in(fileName);
+  std::string line, unused;
+  int clusterId = -1;
+  while (std::getline(in, line)) {
+    std::istringstream ss(line);
+    int layer;
+    float x, y, z, alpha;
+    int mc;
+    if (!(ss >> layer >> x >> y >> z)) continue;
+    if (layer == -1) {
+      if (clusterId != 0) events.emplace_back((int)events.size());
+      events.back().addPrimaryVertex(x, y, z);
+      clusterId = 0;
+    } else if (ss >> unused >> unused >> unused >> alpha >> mc) {
+      if (events.empty() || layer < 0 || layer >= Constants::ITS::LayersNumber) continue;
+      events.back().pushClusterToLayer(layer, clusterId, x, y, z, alpha, mc);
+      ++clusterId;
+    }
+  }
+  return events;
+}
+
+// IOUtils.cxx:80-123 (pions with 7 clusters only)
+std::vector<std::unordered_map<int, Label>> loadLabels(int eventsNum, const std::string& fileName)
+{
+  std::vector<std::unordered_map<int, Label>> labels;
+  labels.reserve(eventsNum);
+  std::unordered_map<int, Label> current;
+  std::ifstream in(fileName);
+  std::string line;
+  std::getline(in, line);
+  while (std::getline(in, line)) {
+    std::istringstream ss(line);
+    int mc, pdg, n;
+    float pt, phi, eta;
+    if (!(ss >> mc)) continue;
+    if (mc == -1) {
+      labels.push_back(current);
+      current.clear();
+    } else if (ss >> pt >> phi >> eta >> pdg >> n) {
+      if (std::abs(pdg) == Constants::PDGCodes::PionCode && n == 7)
+        current.emplace(std::piecewise_construct, std::forward_as_tuple(mc),
+                        std::forward_as_tuple(mc, pt, phi, eta, pdg, n));
+    }
+  }
+  labels.push_back(current);
+  return labels;
+}
+
+// IOUtils.cxx:125-170 (correct / duplicate / fake classification)
+void writeRoadsReport(std::ofstream& correct, std::ofstream& duplicate, std::ofstream& fake,
+                      const std::vector<std::vector<Road>>& roads, const std::unordered_map<int, Label>& labels)
+{
+  std::unordered_set<int> found;
+  correct << -1 << std::endl;
+  fake << -1 << std::endl;
+  for (const std::vector<Road>& vertexRoads : roads) {
+    for (const Road& road : vertexRoads) {
+      const auto it = labels.find(road.getLabel());
+      if (it == labels.end()) continue;
+      if (road.isFakeRoad()) {
+        fake << it->second << std::endl;
+      } else if (found.count(it->second.monteCarloId)) {
+        duplicate << it->second << std::endl;
+      } else {
+        correct << it->second << std::endl;
+        found.insert(it->second.monteCarloId);
+      }
+    }
+  }
+}
+
+}  // namespace IOUtils
+
+}  // namespace CA
+}  // namespace ITS
+}  // namespace o2

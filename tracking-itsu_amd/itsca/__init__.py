@@ -1,0 +1,336 @@
+"""Python bindings (ctypes) of the gfx950 ITS cellular-automaton track finder.
+
+The product is the C ABI of ``lib/libitsca_hip.so`` (``include/itsca.h``) and the
+event API of ``lib/libitsca.so`` (``include/itsca_host.h``).  This module only
+loads them, moves numpy arrays across the boundary and renders results in the
+canonical text form used for bit-exact comparisons.  There is no fallback: if
+the native libraries are missing, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+BIN_DIR = os.path.join(PKG_DIR, "bin")
+
+LAYERS = 7
+TRACKLET_LAYERS = 6
+CELL_LAYERS = 5
+TABLE_SIZE = 401
+
+# Record kinds of itsca_download (include/itsca.h)
+CLUSTERS, INDEX_TABLE, TRACKLETS, TRACKLET_LOOKUP, CELLS, CELL_LOOKUP, NEIGHBOUR_OFFSETS, NEIGHBOURS, ROADS = range(9)
+
+_hip = None
+_host = None
+
+
+class ItscaError(RuntimeError):
+    pass
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("persistent_blocks", ctypes.c_int), ("capacity_scale", ctypes.c_double)]
+
+
+class Counts(ctypes.Structure):
+    _fields_ = [
+        ("clusters", ctypes.c_int * 7),
+        ("tracklets", ctypes.c_int * 6),
+        ("cells", ctypes.c_int * 5),
+        ("neighbours", ctypes.c_int * 4),
+        ("roads", ctypes.c_int),
+    ]
+
+
+HIP_SYMBOLS = [
+    "itsca_create", "itsca_destroy", "itsca_last_error", "itsca_clear_events", "itsca_add_event", "itsca_upload",
+    "itsca_run", "itsca_run_async", "itsca_finish", "itsca_num_items", "itsca_get_counts", "itsca_download",
+    "itsca_set_profiling", "itsca_stage_times",
+]
+HOST_SYMBOLS = [
+    "itsca_events_generate", "itsca_events_load", "itsca_events_write", "itsca_events_free", "itsca_events_count",
+    "itsca_events_num_vertices", "itsca_events_layer_offsets", "itsca_events_copy",
+]
+
+
+def _path(name: str) -> str:
+    p = os.path.join(LIB_DIR, name)
+    if not os.path.exists(p):
+        raise ItscaError(f"{p} is missing: build the package first (__graft_entry__.build())")
+    return p
+
+
+def hip_lib() -> ctypes.CDLL:
+    """Loads libitsca_hip.so (the gfx950 kernels and the C ABI)."""
+    global _hip
+    if _hip is None:
+        lib = ctypes.CDLL(_path("libitsca_hip.so"))
+        P, I, F = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_float)
+        I32 = ctypes.POINTER(ctypes.c_int32)
+        lib.itsca_create.argtypes = [ctypes.POINTER(P), ctypes.POINTER(_Config)]
+        lib.itsca_destroy.argtypes = [P]
+        lib.itsca_destroy.restype = None
+        lib.itsca_last_error.argtypes = [P]
+        lib.itsca_last_error.restype = ctypes.c_char_p
+        lib.itsca_clear_events.argtypes = [P]
+        lib.itsca_add_event.argtypes = [P, F, F, F, I32, I32, I32, F, I, ctypes.POINTER(I)]
+        for f in ("itsca_upload", "itsca_run", "itsca_run_async", "itsca_finish", "itsca_num_items"):
+            getattr(lib, f).argtypes = [P]
+        lib.itsca_get_counts.argtypes = [P, I, ctypes.POINTER(Counts)]
+        lib.itsca_download.argtypes = [P, I, I, I, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+        lib.itsca_set_profiling.argtypes = [P, I]
+        lib.itsca_stage_times.argtypes = [P, F]
+        _hip = lib
+    return _hip
+
+
+def host_lib() -> ctypes.CDLL:
+    """Loads libitsca.so (event generator and reference text formats)."""
+    global _host
+    if _host is None:
+        lib = ctypes.CDLL(_path("libitsca.so"))
+        P, I = ctypes.c_void_p, ctypes.c_int
+        lib.itsca_events_generate.argtypes = [I, ctypes.c_uint64, I]
+        lib.itsca_events_generate.restype = P
+        lib.itsca_events_load.argtypes = [ctypes.c_char_p]
+        lib.itsca_events_load.restype = P
+        lib.itsca_events_write.argtypes = [P, ctypes.c_char_p, ctypes.c_char_p]
+        lib.itsca_events_free.argtypes = [P]
+        lib.itsca_events_free.restype = None
+        lib.itsca_events_count.argtypes = [P]
+        lib.itsca_events_num_vertices.argtypes = [P, I]
+        lib.itsca_events_layer_offsets.argtypes = [P, I, ctypes.POINTER(ctypes.c_int32)]
+        F, I32 = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32)
+        lib.itsca_events_copy.argtypes = [P, I, F, F, F, I32, I32, F]
+        _host = lib
+    return _host
+
+
+@dataclass
+class Event:
+    """One event: clusters of all layers (layer after layer, file order)."""
+
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    cluster_id: np.ndarray
+    mc: np.ndarray
+    layer_offsets: np.ndarray  # 8 x int32
+    vertices: np.ndarray  # (nv, 3) float32
+
+    def layer(self, l: int) -> slice:
+        return slice(int(self.layer_offsets[l]), int(self.layer_offsets[l + 1]))
+
+    def drop_layers(self, layers: Sequence[int]) -> "Event":
+        """Copy with the clusters of `layers` removed (ids keep their values)."""
+        keep = np.ones(len(self.x), dtype=bool)
+        for l in layers:
+            keep[self.layer(l)] = False
+        counts = [int(self.layer_offsets[l + 1] - self.layer_offsets[l]) for l in range(LAYERS)]
+        for l in layers:
+            counts[l] = 0
+        offs = np.zeros(8, dtype=np.int32)
+        offs[1:] = np.cumsum(counts)
+        return Event(self.x[keep], self.y[keep], self.z[keep], self.cluster_id[keep], self.mc[keep], offs,
+                     self.vertices.copy())
+
+
+class _EventsHandle:
+    def __init__(self, handle: int):
+        if not handle:
+            raise ItscaError("event batch could not be created")
+        self.h = handle
+
+    def __del__(self):
+        if getattr(self, "h", None) and _host is not None:
+            _host.itsca_events_free(self.h)
+            self.h = None
+
+
+def _read_events(handle: _EventsHandle) -> List[Event]:
+    lib = host_lib()
+    out = []
+    for e in range(lib.itsca_events_count(handle.h)):
+        offs = np.zeros(8, dtype=np.int32)
+        lib.itsca_events_layer_offsets(handle.h, e, offs.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+        n = int(offs[7])
+        nv = lib.itsca_events_num_vertices(handle.h, e)
+        x, y, z = (np.zeros(n, dtype=np.float32) for _ in range(3))
+        cid, mc = np.zeros(n, dtype=np.int32), np.zeros(n, dtype=np.int32)
+        v = np.zeros((max(nv, 1), 3), dtype=np.float32)
+        fp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))  # noqa: E731
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))  # noqa: E731
+        if lib.itsca_events_copy(handle.h, e, fp(x), fp(y), fp(z), ip(cid), ip(mc), fp(v)) != 0:
+            raise ItscaError("itsca_events_copy failed")
+        out.append(Event(x, y, z, cid, mc, offs, v[:nv].copy()))
+    return out
+
+
+def generate_events(preset: int, seed: int, num_events: int = 1) -> List[Event]:
+    """Seeded synthetic events (preset 1 small, 2 central, 3 mixed, 5 pile-up)."""
+    h = _EventsHandle(host_lib().itsca_events_generate(int(preset), int(seed), int(num_events)))
+    return _read_events(h)
+
+
+def write_events(preset: int, seed: int, num_events: int, path: str, labels_path: Optional[str] = None) -> None:
+    h = _EventsHandle(host_lib().itsca_events_generate(int(preset), int(seed), int(num_events)))
+    lp = labels_path.encode() if labels_path else None
+    if host_lib().itsca_events_write(h.h, path.encode(), lp) != 0:
+        raise ItscaError(f"cannot write {path}")
+
+
+def load_events(path: str) -> List[Event]:
+    """Reads the reference event text format (src/CA/IOUtils.cxx:38-78)."""
+    return _read_events(_EventsHandle(host_lib().itsca_events_load(path.encode())))
+
+
+class GpuTracker:
+    """A context of the gfx950 tracker (one HIP stream, device-resident batch)."""
+
+    def __init__(self, device: int = 0, persistent_blocks: int = 0, capacity_scale: float = 0.0):
+        lib = hip_lib()
+        self._lib = lib
+        self.ctx = ctypes.c_void_p()
+        cfg = _Config(device, persistent_blocks, capacity_scale)
+        st = lib.itsca_create(ctypes.byref(self.ctx), ctypes.byref(cfg))
+        if st != 0:
+            msg = lib.itsca_last_error(self.ctx).decode() if self.ctx else "itsca_create failed"
+            lib.itsca_destroy(self.ctx)
+            self.ctx = None
+            raise ItscaError(msg)
+        self.events: List[Event] = []
+
+    def close(self):
+        if self.ctx:
+            self._lib.itsca_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, st: int, what: str):
+        if st != 0:
+            raise ItscaError(f"{what}: {self._lib.itsca_last_error(self.ctx).decode()} (status {st})")
+
+    def set_events(self, events: Sequence[Event]) -> None:
+        lib = self._lib
+        self._check(lib.itsca_clear_events(self.ctx), "itsca_clear_events")
+        fp = lambda a: np.ascontiguousarray(a, dtype=np.float32).ctypes.data_as(ctypes.POINTER(ctypes.c_float))  # noqa
+        for ev in events:
+            arrs = [np.ascontiguousarray(a) for a in (ev.x, ev.y, ev.z)]
+            ids = np.ascontiguousarray(ev.cluster_id, dtype=np.int32)
+            mcs = np.ascontiguousarray(ev.mc, dtype=np.int32)
+            offs = np.ascontiguousarray(ev.layer_offsets, dtype=np.int32)
+            vtx = np.ascontiguousarray(ev.vertices, dtype=np.float32).reshape(-1)
+            ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))  # noqa: E731
+            self._check(lib.itsca_add_event(self.ctx, fp(arrs[0]), fp(arrs[1]), fp(arrs[2]), ip(ids), ip(mcs),
+                                            ip(offs), fp(vtx), int(len(ev.vertices)), None), "itsca_add_event")
+        self._check(lib.itsca_upload(self.ctx), "itsca_upload")
+        self.events = list(events)
+
+    def run(self) -> None:
+        self._check(self._lib.itsca_run(self.ctx), "itsca_run")
+
+    def run_async(self) -> None:
+        self._check(self._lib.itsca_run_async(self.ctx), "itsca_run_async")
+
+    def finish(self) -> None:
+        self._check(self._lib.itsca_finish(self.ctx), "itsca_finish")
+
+    def set_profiling(self, on: bool) -> None:
+        self._check(self._lib.itsca_set_profiling(self.ctx, 1 if on else 0), "itsca_set_profiling")
+
+    def stage_times(self) -> np.ndarray:
+        ms = np.zeros(5, dtype=np.float32)
+        self._check(self._lib.itsca_stage_times(self.ctx, ms.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+                    "itsca_stage_times")
+        return ms
+
+    def num_items(self) -> int:
+        return self._lib.itsca_num_items(self.ctx)
+
+    def counts(self, item: int) -> Counts:
+        c = Counts()
+        self._check(self._lib.itsca_get_counts(self.ctx, item, ctypes.byref(c)), "itsca_get_counts")
+        return c
+
+    def download(self, item: int, what: int, layer: int = 0, rows: int = 0, fields: int = 1) -> np.ndarray:
+        buf = np.zeros(max(rows * fields, 1), dtype=np.int32)
+        written = ctypes.c_size_t(0)
+        self._check(self._lib.itsca_download(self.ctx, item, what, layer, buf.ctypes.data, buf.nbytes,
+                                             ctypes.byref(written)), "itsca_download")
+        n = written.value // 4
+        out = buf[:n]
+        return out.reshape(-1, fields) if fields > 1 else out
+
+    def roads(self, item: int) -> np.ndarray:
+        """(R, 8) int32: cell per layer 0..4, size, label, fake."""
+        return self.download(item, ROADS, 0, self.counts(item).roads, 8)
+
+
+def _hex(a: np.ndarray) -> List[str]:
+    return ["%08x" % v for v in np.asarray(a).astype(np.int32).view(np.uint32).tolist()]
+
+
+def _ints_line(a) -> str:
+    return " ".join(str(int(v)) for v in a) + "\n"
+
+
+def dump_item(tracker: GpuTracker, item: int, vertex: Sequence[float]) -> str:
+    """Canonical text form of one work item's full state (same format as the
+    CPU oracle's dump: every record, floats as IEEE bit patterns)."""
+    n = tracker.counts(item)
+    vx = np.asarray(vertex, dtype=np.float32)
+    parts = ["vertex %s\n" % " ".join(_hex(vx.view(np.int32)))]
+    for l in range(LAYERS):
+        c = tracker.download(item, CLUSTERS, l, n.clusters[l], 8)
+        parts.append("clusters %d %d\n" % (l, n.clusters[l]))
+        if len(c):
+            ph, rr = _hex(c[:, 1]), _hex(c[:, 2])
+            parts.append("".join("%d %s %s %d\n" % (i, p, r, b) for i, p, r, b in
+                                 zip(c[:, 0].tolist(), ph, rr, c[:, 3].tolist())))
+    for l in range(1, LAYERS):
+        t = tracker.download(item, INDEX_TABLE, l, TABLE_SIZE, 1)
+        parts.append("table %d %d\n" % (l, TABLE_SIZE))
+        parts.append(_ints_line(t))
+    for l in range(TRACKLET_LAYERS):
+        t = tracker.download(item, TRACKLETS, l, n.tracklets[l], 4)
+        parts.append("tracklets %d %d\n" % (l, n.tracklets[l]))
+        if len(t):
+            parts.append("".join("%d %d %s %s\n" % q for q in
+                                 zip(t[:, 0].tolist(), t[:, 1].tolist(), _hex(t[:, 2]), _hex(t[:, 3]))))
+    for l in range(CELL_LAYERS):
+        lk = tracker.download(item, TRACKLET_LOOKUP, l, n.clusters[l + 1], 1)
+        parts.append("tlookup %d %d\n" % (l, len(lk)))
+        parts.append(_ints_line(lk))
+    for l in range(CELL_LAYERS):
+        c = tracker.download(item, CELLS, l, n.cells[l], 10)
+        parts.append("cells %d %d\n" % (l, n.cells[l]))
+        if len(c):
+            cols = [c[:, k].tolist() for k in range(5)] + [_hex(c[:, k]) for k in range(5, 9)] + [c[:, 9].tolist()]
+            parts.append("".join("%d %d %d %d %d %s %s %s %s %d\n" % q for q in zip(*cols)))
+    for l in range(CELL_LAYERS - 1):
+        lk = tracker.download(item, CELL_LOOKUP, l, n.tracklets[l + 1], 1)
+        parts.append("clookup %d %d\n" % (l, len(lk)))
+        parts.append(_ints_line(lk))
+    for l in range(CELL_LAYERS - 1):
+        cn = n.cells[l + 1]
+        off = tracker.download(item, NEIGHBOUR_OFFSETS, l, cn + 1, 1)
+        lst = tracker.download(item, NEIGHBOURS, l, n.neighbours[l], 1).tolist()
+        parts.append("neighbours %d %d\n" % (l, cn))
+        lines = []
+        for i in range(cn):
+            seg = lst[int(off[i]):int(off[i + 1])]
+            lines.append(" ".join([str(len(seg))] + [str(v) for v in seg]) + "\n")
+        parts.append("".join(lines))
+    r = tracker.roads(item)
+    parts.append("roads %d\n" % len(r))
+    parts.append("".join("%d %d %d %d %d %d %d %d\n" % tuple(row) for row in r.tolist()))
+    return "".join(parts)
