@@ -35,6 +35,30 @@ def bytes_alg(c):
     return b
 
 
+def kernel_bytes(name, c):
+    """Algorithmic HBM bytes of one launch of kernel `name` for one work item,
+    from the same record sizes: each kernel reads its inputs once and writes
+    its outputs once."""
+    N, T, C, E, R = list(c.clusters), list(c.tracklets), list(c.cells), list(c.neighbours), c.roads
+    if name == "kPrep":  # raw x, y, z, id, mc in; sorted cluster + 401-entry table out
+        return sum(n * (20 + 36) for n in N) + 7 * 401 * 4
+    if name == "kTrackletsCount":  # both layers' clusters in; one count per cluster out
+        return sum((N[l] + N[l + 1]) * 36 + N[l] * 4 for l in range(6))
+    if name == "kTrackletsFill":  # tracklet records and per-cluster offsets out
+        return sum(T[l] * 16 + N[l] * 4 for l in range(6))
+    if name == "kCellsCount":  # two tracklet layers and three cluster layers in; one count per tracklet out
+        return sum((T[l] + T[l + 1]) * 16 + (N[l] + N[l + 1] + N[l + 2]) * 36 + T[l] * 4 for l in range(5))
+    if name == "kCellsFill":  # cell records and per-tracklet offsets out
+        return sum(C[l] * 40 + T[l] * 4 for l in range(5))
+    if name in ("kEdgesCount", "kEdgesFill", "kInIndex"):  # two cell layers in; edges and counts out
+        return sum((C[l] + C[l + 1]) * 40 + E[l] * 4 + C[l + 1] * 4 for l in range(4))
+    if name == "kLevels":
+        return sum(C) * 4 + sum(E) * 4
+    if name == "kRoadsFill":
+        return R * 32 + sum(E) * 4
+    return 0
+
+
 def cpu_baseline(events_path, n_events):
     """The reference CPU tracker itself (built from its sources into
     oracle/_ref/), single thread, on a bounded sample of the same workload."""
@@ -90,18 +114,27 @@ def main():
     events_per_s = args.batch * args.steps / elapsed
 
     # algorithmic bytes per step and per-stage device time (separate profiled run)
-    balg = sum(bytes_alg(tracker.counts(i)) for i in range(tracker.num_items()))
+    counts = [tracker.counts(i) for i in range(tracker.num_items())]
+    balg = sum(bytes_alg(c) for c in counts)
     tracker.set_profiling(True)
     prof_runs = 5
     stage_ms = [0.0] * 5
+    kernel_ms = {}
     for _ in range(prof_runs):
         tracker.run()
         stage_ms = [a + b for a, b in zip(stage_ms, tracker.stage_times().tolist())]
+        for name, ms in tracker.kernel_times():
+            kernel_ms[name] = kernel_ms.get(name, 0.0) + ms / prof_runs
     stage_ms = [v / prof_runs for v in stage_ms]
     tracker.set_profiling(False)
     c0 = tracker.counts(0)
     dominant = max(range(5), key=lambda k: stage_ms[k])
-    achieved = balg / (ms_per_step * 1e-3) / 1e9
+    # roofline of the dominant kernel: its algorithmic bytes per launch (one
+    # launch covers the whole batch) over its HIP-event time on the stream
+    top = max(kernel_ms, key=kernel_ms.get)
+    top_bytes = sum(kernel_bytes(top, c) for c in counts)
+    launches = 4 if top == "kLevels" else 1
+    achieved = top_bytes / (kernel_ms[top] / launches * 1e-3) / 1e9
 
     line = {
         "metric": "events/s on 1 MI355X, Pb-Pb-like 8k tracks: plain GPU vs batched+learned filter",
@@ -121,8 +154,11 @@ def main():
                    "events_per_step": args.batch, "form": "plain GPU (batched launch)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": "whole pipeline", "bytes_alg_per_step": balg},
+                     "kernel": top, "kernel_ms": round(kernel_ms[top] / launches, 4), "kernel_bytes_alg": top_bytes,
+                     "pipeline_bytes_alg_per_step": balg,
+                     "pipeline_GBps": round(balg / (ms_per_step * 1e-3) / 1e9, 2)},
         "stage_ms": {STAGES[k]: round(stage_ms[k], 4) for k in range(5)},
+        "kernel_ms": {k: round(v, 4) for k, v in sorted(kernel_ms.items(), key=lambda kv: -kv[1])},
         "dominant_stage": STAGES[dominant],
         "counts_event0": {"tracklets": list(c0.tracklets), "cells": list(c0.cells), "roads": c0.roads},
     }

@@ -112,6 +112,13 @@ itsca_status itsca_download(itsca_ctx* ctx, int item, int what, int layer, void*
 itsca_status itsca_set_profiling(itsca_ctx* ctx, int enable);
 itsca_status itsca_stage_times(itsca_ctx* ctx, float ms[5]);
 
+/* Per-launch device time of the last profiled run (HIP events recorded on the
+ * context's stream between consecutive launches); returns the number of
+ * launches (filling at most cap entries), or -1 if no profiled run exists.
+ * itsca_kernel_name gives the kernel of launch i. */
+int itsca_kernel_times(itsca_ctx* ctx, float* ms, int cap);
+const char* itsca_kernel_name(const itsca_ctx* ctx, int i);
+
 #ifdef __cplusplus
 }
 #endif

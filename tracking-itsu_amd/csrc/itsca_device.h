@@ -32,6 +32,9 @@ struct Seg {
 // elements; chunkStart[s] is the first chunk of segment s.
 constexpr int kChunk = 256;
 constexpr int kBlock = 256;
+// Accepted candidates kept per element between a count and its fill.
+constexpr int kTrkSlots = 32;  // per cluster (next-layer cluster positions)
+constexpr int kCellSlots = 4;  // per tracklet (next-layer tracklet indices)
 
 enum Stage { kStTracklets = 0, kStCells = 1, kStEdges = 2, kStLevels = 3, kStRoads = 7, kNumStages = 8 };
 // kStLevels + (l-1) for cell layers 1..4 (levels of layer 0 are all 1).
@@ -63,12 +66,14 @@ struct DevState {
   int* table;  // W*7*401
   // tracklets
   int *trkCnt, *trkOff;  // per sorted cluster
+  int* trkCand;          // kTrkSlots per sorted cluster
   Seg* trkSeg;           // W*6
   int *tFirst, *tSecond;
   float *tTanL, *tPhi;
   int tCap;
   // cells
   int *cellCnt, *cellOff;  // per tracklet
+  int* cellCand;           // kCellSlots per tracklet
   Seg* celSeg;             // W*5
   int *c1, *c2, *c3, *ct1, *ct2, *clevel;
   float *cnx, *cny, *cnz, *ccurv;
@@ -77,6 +82,7 @@ struct DevState {
   // tracklets of layer l grouped by their second cluster
   int* inOff;   // per sorted cluster (layers 1..4): start in inList of layer l-1
   int* inList;  // aligned with the tracklets (trkSeg of layer l)
+  int* inTmp;   // bucket scratch, aligned with the tracklets
   // predecessor (neighbour) lists
   int *edgeCnt, *edgeOff;  // per cell
   Seg* edgSeg;             // W*4

@@ -9,17 +9,35 @@ namespace itsca {
 
 constexpr int kNumTimedStages = 5;
 
-// Optional per-stage HIP events; mark(k) records event k (0..kNumTimedStages).
-struct StageTimer {
-  hipEvent_t ev[kNumTimedStages + 1];
-  void mark(int k, hipStream_t stream) { (void)hipEventRecord(ev[k], stream); }
+// Optional HIP events around every launch of a run, on the launch stream:
+// event 0 before the first kernel, event k+1 after kernel k.
+struct KernelTimer {
+  static constexpr int kMax = 48;
+  hipEvent_t ev[kMax + 1];
+  const char* name[kMax];
+  int stage[kMax];
+  int n = 0;
+  void start(hipStream_t stream)
+  {
+    n = 0;
+    (void)hipEventRecord(ev[0], stream);
+  }
+  void after(const char* kernel, int stageIndex, hipStream_t stream)
+  {
+    if (n >= kMax) return;
+    name[n] = kernel;
+    stage[n] = stageIndex;
+    (void)hipEventRecord(ev[n + 1], stream);
+    ++n;
+  }
 };
 
 struct LaunchConfig {
   int persistentBlocks;
-  int* planScratch;  // device ints, at least maxSegs
+  int* planScratch;       // device ints, at least maxSegs
+  int maxLayerClusters;   // largest cluster count of any (item, layer)
 };
 
-void launchPipeline(const DevState& s, const LaunchConfig& cfg, hipStream_t stream, StageTimer* timer);
+void launchPipeline(const DevState& s, const LaunchConfig& cfg, hipStream_t stream, KernelTimer* timer);
 
 }  // namespace itsca

@@ -294,6 +294,8 @@ __device__ __forceinline__ void forEachTracklet(const DevState& s, const int* ta
   }
 }
 
+// Count: each cluster's accepted candidates, in emission order, are kept in
+// kTrkSlots scratch slots so that the fill does not repeat the window search.
 __global__ void __launch_bounds__(kBlock) kTrackletsCount(DevState s)
 {
   __shared__ int scratch[kBlock / 64 + 1];
@@ -309,8 +311,12 @@ __global__ void __launch_bounds__(kBlock) kTrackletsCount(DevState s)
     if (i < cur.count) {
       const int* table = s.table + (size_t)(w * kLayers + l + 1) * kTableSize;
       const int p = cur.base + i;
+      int* slots = s.trkCand + (size_t)p * kTrkSlots;
       forEachTracklet(s, table, next.base, next.count, s.cz[p], s.cr[p], s.cphi[p], s.itemVertex[3 * w + 2], l,
-                      [&](int) { ++cnt; });
+                      [&](int j) {
+                        if (cnt < kTrkSlots) slots[cnt] = j;
+                        ++cnt;
+                      });
       s.trkCnt[p] = cnt;
     }
     const int total = blockSum(cnt, scratch);
@@ -318,9 +324,41 @@ __global__ void __launch_bounds__(kBlock) kTrackletsCount(DevState s)
   }
 }
 
+// Tracklet ctor, Tracklet.cxx:32-40 (absolute x, y for the direction).
+__device__ __forceinline__ void writeTracklet(const DevState& s, int k, int i, int j, int curBase, int nextBase)
+{
+  if (k >= s.tCap) return;
+  const int pi = curBase + i, q = nextBase + j;
+  const float zi = s.cz[pi], ri = s.cr[pi];
+  s.tFirst[k] = i;
+  s.tSecond[k] = j;
+  s.tTanL[k] = (zi - s.cz[q]) / (ri - s.cr[q]);
+  s.tPhi[k] = fdlibmAtan2f(s.cy[pi] - s.cy[q], s.cx[pi] - s.cx[q]);
+  s.cellCnt[k] = 0;  // stays 0 where the cell stage does not run
+}
+
+// Position of output slot o among the chunk's elements: the largest c with
+// exs[c] <= o (exs is the block's exclusive scan, exs[kBlock] = total > o).
+__device__ __forceinline__ int ownerOfSlot(const int* exs, int o)
+{
+  int lo = 0, hi = kBlock;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (exs[mid] <= o)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Fill: one output record per thread (coalesced writes, no divergent window
+// search); clusters with more than kTrkSlots tracklets are recomputed by their
+// own thread.
 __global__ void __launch_bounds__(kBlock) kTrackletsFill(DevState s)
 {
   __shared__ int scratch[kBlock / 64 + 1];
+  __shared__ int exs[kBlock + 1];
   const int S = s.W * kTrackletLayers;
   const int* chunkStart = s.chunkStart + kStTracklets * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
@@ -328,31 +366,29 @@ __global__ void __launch_bounds__(kBlock) kTrackletsFill(DevState s)
     const int sg = segmentOfChunk(chunkStart, S, chunk);
     const int w = sg / kTrackletLayers, l = sg % kTrackletLayers;
     const Seg cur = s.clSeg[w * kLayers + l], next = s.clSeg[w * kLayers + l + 1];
-    const int i = (chunk - chunkStart[sg]) * kChunk + threadIdx.x;
+    const int first = (chunk - chunkStart[sg]) * kChunk;
+    const int i = first + threadIdx.x;
     const int p = cur.base + i;
     const int cnt = i < cur.count ? s.trkCnt[p] : 0;
     int total;
     const int ex = blockExclusiveScan(cnt, scratch, total);
-    const int out = s.chunkOff[(size_t)kStTracklets * s.maxChunks + chunk] + ex;  // global position
-    const int segBase = s.trkSeg[sg].base;
-    if (i < cur.count) {
-      s.trkOff[p] = out - segBase;
-      if (cnt > 0) {
-        const int* table = s.table + (size_t)(w * kLayers + l + 1) * kTableSize;
-        const float x = s.cx[p], y = s.cy[p], z = s.cz[p], r = s.cr[p];
-        int k = out;
-        forEachTracklet(s, table, next.base, next.count, z, r, s.cphi[p], s.itemVertex[3 * w + 2], l, [&](int j) {
-          if (k < s.tCap) {
-            const int q = next.base + j;
-            s.tFirst[k] = i;
-            s.tSecond[k] = j;
-            s.tTanL[k] = (z - s.cz[q]) / (r - s.cr[q]);
-            s.tPhi[k] = fdlibmAtan2f(y - s.cy[q], x - s.cx[q]);
-            s.cellCnt[k] = 0;  // stays 0 where the cell stage does not run
-          }
-          ++k;
-        });
-      }
+    exs[threadIdx.x] = ex;
+    if (threadIdx.x == 0) exs[kBlock] = total;
+    const int chunkBase = s.chunkOff[(size_t)kStTracklets * s.maxChunks + chunk];
+    if (i < cur.count) s.trkOff[p] = chunkBase + ex - s.trkSeg[sg].base;
+    __syncthreads();
+    for (int o = threadIdx.x; o < total; o += kBlock) {
+      const int c = ownerOfSlot(exs, o);
+      if (exs[c + 1] - exs[c] > kTrkSlots) continue;
+      const int n = o - exs[c];
+      const int j = s.trkCand[(size_t)(cur.base + first + c) * kTrkSlots + n];
+      writeTracklet(s, chunkBase + o, first + c, j, cur.base, next.base);
+    }
+    if (cnt > kTrkSlots) {
+      const int* table = s.table + (size_t)(w * kLayers + l + 1) * kTableSize;
+      int k = chunkBase + ex;
+      forEachTracklet(s, table, next.base, next.count, s.cz[p], s.cr[p], s.cphi[p], s.itemVertex[3 * w + 2], l,
+                      [&](int j) { writeTracklet(s, k++, i, j, cur.base, next.base); });
     }
   }
 }
@@ -382,67 +418,97 @@ struct CellOut {
   float nx, ny, nz, curvature;
 };
 
-template <typename F>
-__device__ __forceinline__ void forEachCell(const DevState& s, int w, int l, int t /*global tracklet*/, F&& emit)
+// Geometry shared by every candidate of tracklet t (layer l, global index).
+struct CellSeed {
+  int l, tnBase, cl2Base, uFirst, nCand;
+  float x1, y1, z1, r1, r1sq, x2, y2, r2sq, d1x, d1y, d1z, vx, vy, vz, tTanL, tPhi;
+};
+
+__device__ __forceinline__ CellSeed cellSeed(const DevState& s, int w, int l, int t)
 {
-  const Seg tl = s.trkSeg[w * kTrackletLayers + l];
-  const Seg tn = s.trkSeg[w * kTrackletLayers + l + 1];
-  const Seg cl0 = s.clSeg[w * kLayers + l], cl1 = s.clSeg[w * kLayers + l + 1], cl2 = s.clSeg[w * kLayers + l + 2];
+  CellSeed c;
+  c.l = l;
+  c.tnBase = s.trkSeg[w * kTrackletLayers + l + 1].base;
+  const Seg cl0 = s.clSeg[w * kLayers + l], cl1 = s.clSeg[w * kLayers + l + 1];
+  c.cl2Base = s.clSeg[w * kLayers + l + 2].base;
   const int k = s.tSecond[t];
-  const int nCand = s.trkCnt[cl1.base + k];
-  if (nCand == 0) return;  // no tracklet of layer l+1 starts at this cluster
-  const int uFirst = s.trkOff[cl1.base + k];
-  const float vx = s.itemVertex[3 * w], vy = s.itemVertex[3 * w + 1], vz = s.itemVertex[3 * w + 2];
+  c.nCand = s.trkCnt[cl1.base + k];  // tracklets of layer l+1 starting at this cluster
+  c.uFirst = c.nCand > 0 ? s.trkOff[cl1.base + k] : 0;
+  c.vx = s.itemVertex[3 * w];
+  c.vy = s.itemVertex[3 * w + 1];
+  c.vz = s.itemVertex[3 * w + 2];
   const int p1 = cl0.base + s.tFirst[t], p2 = cl1.base + k;
-  const float x1 = s.cx[p1], y1 = s.cy[p1], z1 = s.cz[p1], r1 = s.cr[p1];
-  const float x2 = s.cx[p2], y2 = s.cy[p2], r2 = s.cr[p2];
-  const float r1sq = r1 * r1;
-  const float r2sq = r2 * r2;
-  const float d1x = x2 - x1, d1y = y2 - y1, d1z = r2sq - r1sq;
-  const float tTanL = s.tTanL[t], tPhi = s.tPhi[t];
-  const float maxDeltaZ = cellMaxDeltaZ(l), maxDCA = cellMaxDCA(l);
-  (void)tl;
-  for (int iu = uFirst; iu < uFirst + nCand; ++iu) {
-    const int u = tn.base + iu;
-    const float uTanL = s.tTanL[u], uPhi = s.tPhi[u];
-    const float deltaTanLambda = fabsf(tTanL - uTanL);
-    const float deltaPhi = fabsf(tPhi - uPhi);
-    if (!(deltaTanLambda < kCellMaxDeltaTanLambda &&
-          (deltaPhi < kCellMaxDeltaPhi || fabsf(deltaPhi - kTwoPi) < kCellMaxDeltaPhi)))
-      continue;
-    const float avgTanLambda = 0.5f * (tTanL + uTanL);
-    const float dirZ = -avgTanLambda * r1 + z1;
-    const float deltaZ = fabsf(dirZ - vz);
-    if (!(deltaZ < maxDeltaZ)) continue;
-    const int p3 = cl2.base + s.tSecond[u];
-    const float x3 = s.cx[p3], y3 = s.cy[p3], r3 = s.cr[p3];
-    const float r3sq = r3 * r3;
-    const float d2x = x3 - x1, d2y = y3 - y1, d2z = r3sq - r1sq;
-    const float nx = (d1y * d2z) - (d1z * d2y);
-    const float ny = (d1z * d2x) - (d1x * d2z);
-    const float nz = (d1x * d2y) - (d1y * d2x);
-    const float norm = sqrtf(nx * nx + ny * ny + nz * nz);
-    if (norm < kFloatMin || fabsf(nz) < kFloatMin) continue;
-    const float inv = 1.0f / norm;
-    const float hx = nx * inv, hy = ny * inv, hz = nz * inv;
-    const float planeDistance = -hx * (x2 - vx) - (hy * y2 - vy) - hz * r2sq;
-    const float hz2 = hz * hz;
-    const float radius = sqrtf((1.0f - hz2 - 4.0f * planeDistance * hz) / (4.0f * hz2));
-    const float cx = -0.5f * hx / hz, cy = -0.5f * hy / hz;
-    const float dca = fabsf(radius - sqrtf(cx * cx + cy * cy));
-    if (dca > maxDCA) continue;  // NaN passes, as in the reference CPU path
-    CellOut o;
-    o.nx = hx;
-    o.ny = hy;
-    o.nz = hz;
-    o.curvature = 1.0f / radius;
-    emit(iu, o);
-  }
+  c.x1 = s.cx[p1];
+  c.y1 = s.cy[p1];
+  c.z1 = s.cz[p1];
+  c.r1 = s.cr[p1];
+  c.x2 = s.cx[p2];
+  c.y2 = s.cy[p2];
+  const float r2 = s.cr[p2];
+  c.r1sq = c.r1 * c.r1;
+  c.r2sq = r2 * r2;
+  c.d1x = c.x2 - c.x1;
+  c.d1y = c.y2 - c.y1;
+  c.d1z = c.r2sq - c.r1sq;
+  c.tTanL = s.tTanL[t];
+  c.tPhi = s.tPhi[t];
+  return c;
 }
 
+// Cheap cuts of candidate iu (Tracker.cxx:161-173): tanLambda and phi
+// continuity, then the z of the line through the first cluster at the vertex.
+__device__ __forceinline__ bool cellCheapCuts(const DevState& s, const CellSeed& c, int iu)
+{
+  const int u = c.tnBase + iu;
+  const float uTanL = s.tTanL[u], uPhi = s.tPhi[u];
+  const float deltaTanLambda = fabsf(c.tTanL - uTanL);
+  const float deltaPhi = fabsf(c.tPhi - uPhi);
+  if (!(deltaTanLambda < kCellMaxDeltaTanLambda &&
+        (deltaPhi < kCellMaxDeltaPhi || fabsf(deltaPhi - kTwoPi) < kCellMaxDeltaPhi)))
+    return false;
+  const float avgTanLambda = 0.5f * (c.tTanL + uTanL);
+  const float dirZ = -avgTanLambda * c.r1 + c.z1;
+  const float deltaZ = fabsf(dirZ - c.vz);
+  return deltaZ < cellMaxDeltaZ(c.l);
+}
+
+// Conformal-plane fit and DCA cut (Tracker.cxx:175-218, MathUtils.h:53-60).
+__device__ __forceinline__ bool cellFit(const DevState& s, const CellSeed& c, int iu, CellOut& o)
+{
+  const int u = c.tnBase + iu;
+  const int p3 = c.cl2Base + s.tSecond[u];
+  const float x3 = s.cx[p3], y3 = s.cy[p3], r3 = s.cr[p3];
+  const float r3sq = r3 * r3;
+  const float d2x = x3 - c.x1, d2y = y3 - c.y1, d2z = r3sq - c.r1sq;
+  const float nx = (c.d1y * d2z) - (c.d1z * d2y);
+  const float ny = (c.d1z * d2x) - (c.d1x * d2z);
+  const float nz = (c.d1x * d2y) - (c.d1y * d2x);
+  const float norm = sqrtf(nx * nx + ny * ny + nz * nz);
+  if (norm < kFloatMin || fabsf(nz) < kFloatMin) return false;
+  const float inv = 1.0f / norm;
+  const float hx = nx * inv, hy = ny * inv, hz = nz * inv;
+  const float planeDistance = -hx * (c.x2 - c.vx) - (hy * c.y2 - c.vy) - hz * c.r2sq;
+  const float hz2 = hz * hz;
+  const float radius = sqrtf((1.0f - hz2 - 4.0f * planeDistance * hz) / (4.0f * hz2));
+  const float cx = -0.5f * hx / hz, cy = -0.5f * hy / hz;
+  const float dca = fabsf(radius - sqrtf(cx * cx + cy * cy));
+  if (dca > cellMaxDCA(c.l)) return false;  // NaN passes, as in the reference CPU path
+  o.nx = hx;
+  o.ny = hy;
+  o.nz = hz;
+  o.curvature = 1.0f / radius;
+  return true;
+}
+
+constexpr int kCellList = 8;  // cheap-cut survivors kept per thread before the fit
+
+// Count: the cheap cuts run over every candidate; survivors (usually 0-2) are
+// fitted in a second loop, so the expensive fit does not sit inside the
+// divergent candidate loop.  Accepted candidates are kept in kCellSlots slots.
 __global__ void __launch_bounds__(kBlock) kCellsCount(DevState s)
 {
   __shared__ int scratch[kBlock / 64 + 1];
+  __shared__ int list[kCellList][kBlock];
   const int S = s.W * kCellLayers;
   const int* chunkStart = s.chunkStart + kStCells * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
@@ -453,17 +519,71 @@ __global__ void __launch_bounds__(kBlock) kCellsCount(DevState s)
     const int it = (chunk - chunkStart[sg]) * kChunk + threadIdx.x;
     int cnt = 0;
     if (it < tl.count) {
-      forEachCell(s, w, l, tl.base + it, [&](int, const CellOut&) { ++cnt; });
-      s.cellCnt[tl.base + it] = cnt;
+      const int t = tl.base + it;
+      const CellSeed c = cellSeed(s, w, l, t);
+      int* slots = s.cellCand + (size_t)t * kCellSlots;
+      int m = 0;
+      for (int iu = c.uFirst; iu < c.uFirst + c.nCand; ++iu) {
+        if (!cellCheapCuts(s, c, iu)) continue;
+        if (m < kCellList) {
+          list[m][threadIdx.x] = iu;
+        } else {  // long survivor list: fit inline
+          CellOut o;
+          if (m == kCellList) {
+            for (int q = 0; q < kCellList; ++q)
+              if (cellFit(s, c, list[q][threadIdx.x], o)) {
+                if (cnt < kCellSlots) slots[cnt] = list[q][threadIdx.x];
+                ++cnt;
+              }
+          }
+          if (cellFit(s, c, iu, o)) {
+            if (cnt < kCellSlots) slots[cnt] = iu;
+            ++cnt;
+          }
+        }
+        ++m;
+      }
+      if (m <= kCellList) {
+        for (int q = 0; q < m; ++q) {
+          CellOut o;
+          const int iu = list[q][threadIdx.x];
+          if (cellFit(s, c, iu, o)) {
+            if (cnt < kCellSlots) slots[cnt] = iu;
+            ++cnt;
+          }
+        }
+      }
+      s.cellCnt[t] = cnt;
     }
     const int total = blockSum(cnt, scratch);
     if (threadIdx.x == 0) s.chunkSum[(size_t)kStCells * s.maxChunks + chunk] = total;
   }
 }
 
+__device__ __forceinline__ void writeCell(const DevState& s, const CellSeed& c, int k, int it, int iu,
+                                          const CellOut& o, int first, int second)
+{
+  if (k >= s.cCap) return;
+  s.c1[k] = first;
+  s.c2[k] = second;
+  s.c3[k] = s.tSecond[c.tnBase + iu];
+  s.ct1[k] = it;
+  s.ct2[k] = iu;
+  s.cnx[k] = o.nx;
+  s.cny[k] = o.ny;
+  s.cnz[k] = o.nz;
+  s.ccurv[k] = o.curvature;
+  s.clevel[k] = 1;
+  s.cpaths[k] = 1;
+  s.edgeCnt[k] = 0;
+}
+
+// Fill: one cell per thread, refitted from the kept candidate; tracklets with
+// more than kCellSlots cells are redone by their own thread.
 __global__ void __launch_bounds__(kBlock) kCellsFill(DevState s)
 {
   __shared__ int scratch[kBlock / 64 + 1];
+  __shared__ int exs[kBlock + 1];
   const int S = s.W * kCellLayers;
   const int* chunkStart = s.chunkStart + kStCells * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
@@ -471,36 +591,35 @@ __global__ void __launch_bounds__(kBlock) kCellsFill(DevState s)
     const int sg = segmentOfChunk(chunkStart, S, chunk);
     const int w = sg / kCellLayers, l = sg % kCellLayers;
     const Seg tl = s.trkSeg[w * kTrackletLayers + l];
-    const int it = (chunk - chunkStart[sg]) * kChunk + threadIdx.x;
+    const int firstT = (chunk - chunkStart[sg]) * kChunk;
+    const int it = firstT + threadIdx.x;
     const int t = tl.base + it;
     const int cnt = it < tl.count ? s.cellCnt[t] : 0;
     int total;
     const int ex = blockExclusiveScan(cnt, scratch, total);
-    const int out = s.chunkOff[(size_t)kStCells * s.maxChunks + chunk] + ex;
-    const int segBase = s.celSeg[sg].base;
-    if (it < tl.count) {
-      s.cellOff[t] = out - segBase;
-      if (cnt > 0) {
-        const int first = s.tFirst[t], second = s.tSecond[t];
-        const int tn = s.trkSeg[w * kTrackletLayers + l + 1].base;
-        int k = out;
-        forEachCell(s, w, l, t, [&](int iu, const CellOut& o) {
-          if (k < s.cCap) {
-            s.c1[k] = first;
-            s.c2[k] = second;
-            s.c3[k] = s.tSecond[tn + iu];
-            s.ct1[k] = it;
-            s.ct2[k] = iu;
-            s.cnx[k] = o.nx;
-            s.cny[k] = o.ny;
-            s.cnz[k] = o.nz;
-            s.ccurv[k] = o.curvature;
-            s.clevel[k] = 1;
-            s.cpaths[k] = 1;
-            s.edgeCnt[k] = 0;
-          }
-          ++k;
-        });
+    exs[threadIdx.x] = ex;
+    if (threadIdx.x == 0) exs[kBlock] = total;
+    const int chunkBase = s.chunkOff[(size_t)kStCells * s.maxChunks + chunk];
+    if (it < tl.count) s.cellOff[t] = chunkBase + ex - s.celSeg[sg].base;
+    __syncthreads();
+    for (int o = threadIdx.x; o < total; o += kBlock) {
+      const int c = ownerOfSlot(exs, o);
+      if (exs[c + 1] - exs[c] > kCellSlots) continue;
+      const int n = o - exs[c];
+      const int tt = tl.base + firstT + c;
+      const int iu = s.cellCand[(size_t)tt * kCellSlots + n];
+      const CellSeed cs = cellSeed(s, w, l, tt);
+      CellOut out;
+      cellFit(s, cs, iu, out);  // accepted in the count pass; same inputs, same result
+      writeCell(s, cs, chunkBase + o, firstT + c, iu, out, s.tFirst[tt], s.tSecond[tt]);
+    }
+    if (cnt > kCellSlots) {
+      const CellSeed cs = cellSeed(s, w, l, t);
+      int k = chunkBase + ex;
+      for (int iu = cs.uFirst; iu < cs.uFirst + cs.nCand; ++iu) {
+        CellOut out;
+        if (cellCheapCuts(s, cs, iu) && cellFit(s, cs, iu, out))
+          writeCell(s, cs, k++, it, iu, out, s.tFirst[t], s.tSecond[t]);
       }
     }
   }
@@ -512,58 +631,64 @@ __global__ void __launch_bounds__(kBlock) kCellsFill(DevState s)
 // per (item, layer), layers 0..3.  Bucket order from the LDS atomics is made
 // deterministic by sorting each (small) bucket afterwards.
 // ---------------------------------------------------------------------------
-constexpr int kKeysPerPass = 8192;
+// One workgroup per (item, layer, key range of kInKeys keys).  Each workgroup
+// counts the tracklets whose key lies below its range (its base), builds the
+// histogram of its keys in LDS, places the tracklets into buckets with LDS
+// atomics (arbitrary order inside a bucket) and finally writes each tracklet
+// at bucket start + (number of bucket members with a smaller index): the
+// result is ascending inside each bucket whatever the atomics did.
+constexpr int kInKeys = 2048;
 
-__global__ void __launch_bounds__(1024) kInIndex(DevState s)
+__global__ void __launch_bounds__(1024) kInIndex(DevState s, int rangesPerSeg)
 {
-  __shared__ int hist[kKeysPerPass];
+  __shared__ int hist[kInKeys];
+  __shared__ int start[kInKeys];
   __shared__ int scratch[1024 / 64 + 1];
-  const int w = blockIdx.x / (kCellLayers - 1), l = blockIdx.x % (kCellLayers - 1);
+  const int seg = blockIdx.x / rangesPerSeg, range = blockIdx.x % rangesPerSeg;
+  const int w = seg / (kCellLayers - 1), l = seg % (kCellLayers - 1);
   const Seg tl = s.trkSeg[w * kTrackletLayers + l];
   const Seg keys = s.clSeg[w * kLayers + l + 1];
-  int* inOff = s.inOff + keys.base;
-  int* list = s.inList + tl.base;
-  int carry = 0;
-  for (int k0 = 0; k0 < keys.count; k0 += kKeysPerPass) {
-    const int nk = min(kKeysPerPass, keys.count - k0);
-    for (int k = threadIdx.x; k < nk; k += blockDim.x) hist[k] = 0;
-    __syncthreads();
-    for (int t = threadIdx.x; t < tl.count; t += blockDim.x) {
-      const int key = s.tSecond[tl.base + t] - k0;
-      if (key >= 0 && key < nk) atomicAdd(&hist[key], 1);
+  const int k0 = range * kInKeys;
+  if (k0 >= keys.count) return;
+  const int nk = min(kInKeys, keys.count - k0);
+  const int* key = s.tSecond + tl.base;
+  int below = 0;
+  for (int k = threadIdx.x; k < nk; k += blockDim.x) hist[k] = 0;
+  for (int t = threadIdx.x; t < tl.count; t += blockDim.x) below += key[t] < k0 ? 1 : 0;
+  below = blockSum(below, scratch);
+  for (int t = threadIdx.x; t < tl.count; t += blockDim.x) {
+    const int kk = key[t] - k0;
+    if (kk >= 0 && kk < nk) atomicAdd(&hist[kk], 1);
+  }
+  __syncthreads();
+  int carry = below;
+  for (int k1 = 0; k1 < nk; k1 += blockDim.x) {
+    const int k = k1 + threadIdx.x;
+    const int v = k < nk ? hist[k] : 0;
+    int total;
+    const int ex = blockExclusiveScan(v, scratch, total);
+    if (k < nk) {
+      start[k] = carry + ex;
+      s.inOff[keys.base + k0 + k] = carry + ex;
+      hist[k] = carry + ex;  // insertion cursor
     }
-    __syncthreads();
-    for (int k1 = 0; k1 < nk; k1 += blockDim.x) {
-      const int k = k1 + threadIdx.x;
-      const int v = k < nk ? hist[k] : 0;
-      int total;
-      const int ex = blockExclusiveScan(v, scratch, total);
-      if (k < nk) {
-        hist[k] = carry + ex;
-        inOff[k0 + k] = carry + ex;
-      }
-      carry += total;
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < tl.count; t += blockDim.x) {
-      const int key = s.tSecond[tl.base + t] - k0;
-      if (key >= 0 && key < nk) list[atomicAdd(&hist[key], 1)] = t;
-    }
-    __syncthreads();
+    carry += total;
+  }
+  __syncthreads();
+  int* tmp = s.inTmp + tl.base;
+  for (int t = threadIdx.x; t < tl.count; t += blockDim.x) {
+    const int kk = key[t] - k0;
+    if (kk >= 0 && kk < nk) tmp[atomicAdd(&hist[kk], 1)] = t;
   }
   __threadfence_block();
   __syncthreads();
-  for (int k = threadIdx.x; k < keys.count; k += blockDim.x) {
-    const int lo = inOff[k], hi = k + 1 < keys.count ? inOff[k + 1] : tl.count;
-    for (int a = lo + 1; a < hi; ++a) {  // insertion sort of one small bucket
-      const int v = list[a];
-      int b = a - 1;
-      while (b >= lo && list[b] > v) {
-        list[b + 1] = list[b];
-        --b;
-      }
-      list[b + 1] = v;
-    }
+  for (int t = threadIdx.x; t < tl.count; t += blockDim.x) {
+    const int kk = key[t] - k0;
+    if (kk < 0 || kk >= nk) continue;
+    const int lo = start[kk], hi = hist[kk];  // the cursor ended at the bucket end
+    int rank = 0;
+    for (int a = lo; a < hi; ++a) rank += tmp[a] < t ? 1 : 0;
+    s.inList[tl.base + lo + rank] = t;
   }
 }
 
@@ -826,46 +951,61 @@ __global__ void __launch_bounds__(kBlock) kRoadsFill(DevState s)
 // ---------------------------------------------------------------------------
 // Launch sequence of one plain-CA run over the whole batch.
 // ---------------------------------------------------------------------------
-void launchPipeline(const DevState& s, const LaunchConfig& cfg, hipStream_t stream, StageTimer* timer)
+void launchPipeline(const DevState& s, const LaunchConfig& cfg, hipStream_t stream, KernelTimer* timer)
 {
   const int G = cfg.persistentBlocks;
   int* counts = cfg.planScratch;
-  auto mark = [&](int stage) {
-    if (timer) timer->mark(stage, stream);
+  auto done = [&](const char* name, int stage) {
+    if (timer) timer->after(name, stage, stream);
   };
-  mark(0);
+  if (timer) timer->start(stream);
   hipLaunchKernelGGL(kPrep, dim3(s.W * kLayers), dim3(1024), 0, stream, s);
-  mark(1);
+  done("kPrep", 0);
   hipLaunchKernelGGL(kPlanTracklets, dim3(1), dim3(1024), 0, stream, s, counts);
+  done("kPlanTracklets", 1);
   hipLaunchKernelGGL(kTrackletsCount, dim3(G), dim3(kBlock), 0, stream, s);
+  done("kTrackletsCount", 1);
   hipLaunchKernelGGL(kScanChunks, dim3(1), dim3(1024), 0, stream, s, (int)kStTracklets, s.W * kTrackletLayers,
                      s.trkSeg, s.tCap, (int)kStatusOverflowTracklets);
+  done("kScanChunks", 1);
   hipLaunchKernelGGL(kTrackletsFill, dim3(G), dim3(kBlock), 0, stream, s);
-  mark(2);
+  done("kTrackletsFill", 1);
   hipLaunchKernelGGL(kPlanCells, dim3(1), dim3(1024), 0, stream, s, counts);
+  done("kPlanCells", 2);
   hipLaunchKernelGGL(kCellsCount, dim3(G), dim3(kBlock), 0, stream, s);
+  done("kCellsCount", 2);
   hipLaunchKernelGGL(kScanChunks, dim3(1), dim3(1024), 0, stream, s, (int)kStCells, s.W * kCellLayers, s.celSeg,
                      s.cCap, (int)kStatusOverflowCells);
+  done("kScanChunks", 2);
   hipLaunchKernelGGL(kCellsFill, dim3(G), dim3(kBlock), 0, stream, s);
-  mark(3);
-  hipLaunchKernelGGL(kInIndex, dim3(s.W * (kCellLayers - 1)), dim3(1024), 0, stream, s);
+  done("kCellsFill", 2);
+  const int ranges = (cfg.maxLayerClusters + kInKeys - 1) / kInKeys;
+  hipLaunchKernelGGL(kInIndex, dim3(s.W * (kCellLayers - 1) * (ranges > 0 ? ranges : 1)), dim3(1024), 0, stream, s,
+                     ranges > 0 ? ranges : 1);
+  done("kInIndex", 3);
   hipLaunchKernelGGL(kPlanEdges, dim3(1), dim3(1024), 0, stream, s, counts);
+  done("kPlanEdges", 3);
   hipLaunchKernelGGL(kEdgesCount, dim3(G), dim3(kBlock), 0, stream, s);
+  done("kEdgesCount", 3);
   hipLaunchKernelGGL(kScanChunks, dim3(1), dim3(1024), 0, stream, s, (int)kStEdges, s.W * (kCellLayers - 1),
                      s.edgSeg, s.eCap, (int)kStatusOverflowEdges);
+  done("kScanChunks", 3);
   hipLaunchKernelGGL(kEdgesFill, dim3(G), dim3(kBlock), 0, stream, s);
-  mark(4);
+  done("kEdgesFill", 3);
   for (int layer = 1; layer < kCellLayers; ++layer) {
     // The last level launch shares its chunk plan with the roads stage: it also
     // writes each chunk's number of roads.
     const int stage = layer == kCellLayers - 1 ? (int)kStRoads : kStLevels + layer - 1;
     hipLaunchKernelGGL(kPlanLayer, dim3(1), dim3(1024), 0, stream, s, counts, layer, stage);
+    done("kPlanLayer", 4);
     hipLaunchKernelGGL(kLevels, dim3(G), dim3(kBlock), 0, stream, s, layer, stage);
+    done("kLevels", 4);
   }
   hipLaunchKernelGGL(kScanChunks, dim3(1), dim3(1024), 0, stream, s, (int)kStRoads, s.W, s.roadSeg, s.rCap,
                      (int)kStatusOverflowRoads);
+  done("kScanChunks", 4);
   hipLaunchKernelGGL(kRoadsFill, dim3(G), dim3(kBlock), 0, stream, s);
-  mark(5);
+  done("kRoadsFill", 4);
 }
 
 }  // namespace itsca

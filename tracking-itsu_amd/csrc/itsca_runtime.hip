@@ -68,7 +68,7 @@ struct itsca_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   bool profiling = false;
-  StageTimer timer{};
+  KernelTimer timer{};
   bool timerReady = false;
   bool ran = false;
 
@@ -94,7 +94,7 @@ struct itsca_ctx {
   DBuf<Seg> dClSeg, dTrkSeg, dCelSeg, dEdgSeg, dRoadSeg;
   DBuf<float> cx, cy, cz, cphi, cr, sPhi, sR;
   DBuf<int> cid, cmc, cbin, sBin, sTmp, trkCnt, trkOff, inOff, table;
-  DBuf<int> tFirst, tSecond, cellCnt, cellOff, inList;
+  DBuf<int> tFirst, tSecond, cellCnt, cellOff, inList, inTmp, trkCand, cellCand;
   DBuf<float> tTanL, tPhi;
   DBuf<int> c1, c2, c3, ct1, ct2, clevel, edgeCnt, edgeOff, roadOff;
   DBuf<float> cnx, cny, cnz, ccurv;
@@ -155,6 +155,7 @@ struct itsca_ctx {
     HIP_TRY(trkCnt.reserve(SC));
     HIP_TRY(trkOff.reserve(SC));
     HIP_TRY(inOff.reserve(SC));
+    HIP_TRY(trkCand.reserve(SC * kTrkSlots));
     HIP_TRY(table.reserve((size_t)W * kLayers * kTableSize));
     HIP_TRY(tFirst.reserve(tCap));
     HIP_TRY(tSecond.reserve(tCap));
@@ -163,6 +164,8 @@ struct itsca_ctx {
     HIP_TRY(cellCnt.reserve(tCap));
     HIP_TRY(cellOff.reserve(tCap));
     HIP_TRY(inList.reserve(tCap));
+    HIP_TRY(inTmp.reserve(tCap));
+    HIP_TRY(cellCand.reserve((size_t)tCap * kCellSlots));
     HIP_TRY(c1.reserve(cCap));
     HIP_TRY(c2.reserve(cCap));
     HIP_TRY(c3.reserve(cCap));
@@ -221,6 +224,7 @@ struct itsca_ctx {
     s.table = table.p;
     s.trkCnt = trkCnt.p;
     s.trkOff = trkOff.p;
+    s.trkCand = trkCand.p;
     s.trkSeg = dTrkSeg.p;
     s.tFirst = tFirst.p;
     s.tSecond = tSecond.p;
@@ -229,6 +233,7 @@ struct itsca_ctx {
     s.tCap = tCap;
     s.cellCnt = cellCnt.p;
     s.cellOff = cellOff.p;
+    s.cellCand = cellCand.p;
     s.celSeg = dCelSeg.p;
     s.c1 = c1.p;
     s.c2 = c2.p;
@@ -244,6 +249,7 @@ struct itsca_ctx {
     s.cCap = cCap;
     s.inOff = inOff.p;
     s.inList = inList.p;
+    s.inTmp = inTmp.p;
     s.edgeCnt = edgeCnt.p;
     s.edgeOff = edgeOff.p;
     s.edgSeg = dEdgSeg.p;
@@ -265,7 +271,9 @@ struct itsca_ctx {
 
   void enqueue()
   {
-    LaunchConfig cfg{persistentBlocks, planScratch.p};
+    int maxN = 0;
+    for (const Seg& g : clSeg) maxN = std::max(maxN, g.count);
+    LaunchConfig cfg{persistentBlocks, planScratch.p, maxN};
     if (profiling && !timerReady) {
       for (auto& e : timer.ev) HIP_TRY(hipEventCreate(&e));
       timerReady = true;
@@ -662,11 +670,33 @@ itsca_status itsca_set_profiling(itsca_ctx* ctx, int enable)
 itsca_status itsca_stage_times(itsca_ctx* ctx, float ms[5])
 {
   return guarded(ctx, "itsca_stage_times", [&]() -> itsca_status {
-    if (!ctx->timerReady) return ctx->fail(ITSCA_ERR_STATE, "itsca_stage_times: profiling was not enabled");
-    HIP_TRY(hipEventSynchronize(ctx->timer.ev[kNumTimedStages]));
-    for (int k = 0; k < kNumTimedStages; ++k) HIP_TRY(hipEventElapsedTime(&ms[k], ctx->timer.ev[k], ctx->timer.ev[k + 1]));
+    if (!ctx->timerReady || ctx->timer.n == 0)
+      return ctx->fail(ITSCA_ERR_STATE, "itsca_stage_times: no profiled run");
+    HIP_TRY(hipEventSynchronize(ctx->timer.ev[ctx->timer.n]));
+    for (int k = 0; k < kNumTimedStages; ++k) ms[k] = 0.0f;
+    for (int i = 0; i < ctx->timer.n; ++i) {
+      float t = 0.0f;
+      HIP_TRY(hipEventElapsedTime(&t, ctx->timer.ev[i], ctx->timer.ev[i + 1]));
+      ms[ctx->timer.stage[i]] += t;
+    }
     return ITSCA_OK;
   });
+}
+
+int itsca_kernel_times(itsca_ctx* ctx, float* ms, int cap)
+{
+  if (!ctx || !ctx->timerReady || ctx->timer.n == 0) return -1;
+  if (hipEventSynchronize(ctx->timer.ev[ctx->timer.n]) != hipSuccess) return -1;
+  const int n = std::min(cap, ctx->timer.n);
+  for (int i = 0; i < n; ++i)
+    if (hipEventElapsedTime(&ms[i], ctx->timer.ev[i], ctx->timer.ev[i + 1]) != hipSuccess) return -1;
+  return ctx->timer.n;
+}
+
+const char* itsca_kernel_name(const itsca_ctx* ctx, int i)
+{
+  if (!ctx || i < 0 || i >= ctx->timer.n) return "";
+  return ctx->timer.name[i];
 }
 
 }  // extern "C"

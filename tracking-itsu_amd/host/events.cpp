@@ -142,7 +142,7 @@ void addTrackHits(Rng& rng, const Track& t, LayerHits* hits, std::vector<McLabel
 {
   int n = 0;
   for (int l = 0; l < kNumLayers; ++l) {
-    double px, py, pz;
+    double px = 0, py = 0, pz = 0;
     if (!crossCylinder(t, kGenR[l], px, py, pz)) continue;
     if (std::fabs(pz) > kGenHalfZ[l]) continue;
     const double dphi = rng.normal() * kSmear / kGenR[l];

@@ -52,7 +52,7 @@ class Counts(ctypes.Structure):
 HIP_SYMBOLS = [
     "itsca_create", "itsca_destroy", "itsca_last_error", "itsca_clear_events", "itsca_add_event", "itsca_upload",
     "itsca_run", "itsca_run_async", "itsca_finish", "itsca_num_items", "itsca_get_counts", "itsca_download",
-    "itsca_set_profiling", "itsca_stage_times",
+    "itsca_set_profiling", "itsca_stage_times", "itsca_kernel_times", "itsca_kernel_name",
 ]
 HOST_SYMBOLS = [
     "itsca_events_generate", "itsca_events_load", "itsca_events_write", "itsca_events_free", "itsca_events_count",
@@ -87,6 +87,9 @@ def hip_lib() -> ctypes.CDLL:
         lib.itsca_download.argtypes = [P, I, I, I, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         lib.itsca_set_profiling.argtypes = [P, I]
         lib.itsca_stage_times.argtypes = [P, F]
+        lib.itsca_kernel_times.argtypes = [P, F, I]
+        lib.itsca_kernel_name.argtypes = [P, I]
+        lib.itsca_kernel_name.restype = ctypes.c_char_p
         _hip = lib
     return _hip
 
@@ -252,6 +255,14 @@ class GpuTracker:
         self._check(self._lib.itsca_stage_times(self.ctx, ms.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
                     "itsca_stage_times")
         return ms
+
+    def kernel_times(self):
+        """[(kernel name, ms)] of every launch of the last profiled run."""
+        ms = np.zeros(64, dtype=np.float32)
+        n = self._lib.itsca_kernel_times(self.ctx, ms.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), 64)
+        if n < 0:
+            raise ItscaError("itsca_kernel_times: no profiled run")
+        return [(self._lib.itsca_kernel_name(self.ctx, i).decode(), float(ms[i])) for i in range(min(n, 64))]
 
     def num_items(self) -> int:
         return self._lib.itsca_num_items(self.ctx)
