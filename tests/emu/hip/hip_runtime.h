@@ -1,0 +1,98 @@
+// hip_runtime.h -- CPU emulation shim for the kernel and runtime sources of
+// libitsca_hip.so (TEST INFRASTRUCTURE ONLY).
+//
+// Compiling the product's .hip files with g++ against this header runs every
+// kernel on the CPU: one std::thread per GPU thread, a std::barrier per
+// workgroup for __syncthreads, a per-wavefront exchange for __shfl_up, and
+// plain host memory for device buffers.  Built with AddressSanitizer, an
+// out-of-bounds access in a kernel becomes an ASan report.  Used to check
+// kernel logic on machines without a GPU; never a product path.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__ static
+
+struct dim3 {
+  unsigned x = 1, y = 1, z = 1;
+  dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+struct uint3e {
+  unsigned x = 0, y = 0, z = 0;
+};
+
+namespace emu {
+extern thread_local uint3e tIdx, bIdx;
+extern thread_local dim3 bDim, gDim;
+void syncthreads();
+int shflUp(int v, unsigned d, int width);
+void launch(dim3 grid, dim3 block, const std::function<void()>& body);
+}  // namespace emu
+
+#define threadIdx (emu::tIdx)
+#define blockIdx (emu::bIdx)
+#define blockDim (emu::bDim)
+#define gridDim (emu::gDim)
+
+inline void __syncthreads() { emu::syncthreads(); }
+inline void __threadfence_block() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
+inline int __shfl_up(int v, unsigned d, int width = 64) { return emu::shflUp(v, d, width); }
+inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+
+inline int min(int a, int b) { return a < b ? a : b; }
+inline int max(int a, int b) { return a > b ? a : b; }
+inline unsigned min(unsigned a, unsigned b) { return a < b ? a : b; }
+
+struct float2 {
+  float x, y;
+};
+struct float4 {
+  float x, y, z, w;
+};
+inline float2 make_float2(float a, float b) { return float2{a, b}; }
+inline float4 make_float4(float a, float b, float c, float d) { return float4{a, b, c, d}; }
+
+// --- runtime API subset --------------------------------------------------
+typedef int hipError_t;
+typedef void* hipStream_t;
+typedef void* hipEvent_t;
+enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorInvalidDevice = 101 };
+enum hipMemcpyKind { hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2, hipMemcpyDeviceToDevice = 3 };
+#define hipStreamNonBlocking 1
+struct hipDeviceProp_t {
+  int multiProcessorCount;
+};
+hipError_t hipMalloc(void** p, size_t bytes);
+template <typename T>
+hipError_t hipMalloc(T** p, size_t bytes)
+{
+  return hipMalloc(reinterpret_cast<void**>(p), bytes);
+}
+hipError_t hipFree(void* p);
+hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKind, hipStream_t);
+hipError_t hipStreamSynchronize(hipStream_t);
+hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned);
+hipError_t hipStreamDestroy(hipStream_t);
+hipError_t hipGetDeviceCount(int* n);
+hipError_t hipSetDevice(int);
+hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int);
+hipError_t hipGetLastError();
+const char* hipGetErrorString(hipError_t);
+hipError_t hipEventCreate(hipEvent_t* e);
+hipError_t hipEventDestroy(hipEvent_t e);
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t);
+hipError_t hipEventSynchronize(hipEvent_t e);
+hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b);
+
+#define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...) \
+  emu::launch(dim3(grid), dim3(block), [&]() { kernel(__VA_ARGS__); })

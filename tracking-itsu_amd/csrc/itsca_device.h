@@ -35,6 +35,7 @@ constexpr int kBlock = 256;
 // Accepted candidates kept per element between a count and its fill.
 constexpr int kTrkSlots = 32;  // per cluster (next-layer cluster positions)
 constexpr int kCellSlots = 4;  // per tracklet (next-layer tracklet indices)
+constexpr int kEdgeSlots = 8;  // per cell (next-layer cell indices)
 
 enum Stage { kStTracklets = 0, kStCells = 1, kStEdges = 2, kStLevels = 3, kStRoads = 7, kNumStages = 8 };
 // kStLevels + (l-1) for cell layers 1..4 (levels of layer 0 are all 1).
@@ -60,6 +61,7 @@ struct DevState {
   // sorted clusters
   const Seg* clSeg;  // W*7
   float *cx, *cy, *cz, *cphi, *cr;
+  float4* crzp;  // (r, z, phi, 0) per sorted cluster: one load per tracklet candidate
   int *cid, *cmc, *cbin;
   float *sPhi, *sR;  // prep scratch, original order
   int *sBin, *sTmp;
@@ -69,7 +71,7 @@ struct DevState {
   int* trkCand;          // kTrkSlots per sorted cluster
   Seg* trkSeg;           // W*6
   int *tFirst, *tSecond;
-  float *tTanL, *tPhi;
+  float2* tTP;  // (tanLambda, phi): one load per cell candidate
   int tCap;
   // cells
   int *cellCnt, *cellOff;  // per tracklet
@@ -79,13 +81,12 @@ struct DevState {
   float *cnx, *cny, *cnz, *ccurv;
   uint32_t* cpaths;  // number of level-descending paths from layer 0
   int cCap;
-  // tracklets of layer l grouped by their second cluster
-  int* inOff;   // per sorted cluster (layers 1..4): start in inList of layer l-1
-  int* inList;  // aligned with the tracklets (trkSeg of layer l)
-  int* inTmp;   // bucket scratch, aligned with the tracklets
-  // predecessor (neighbour) lists
-  int *edgeCnt, *edgeOff;  // per cell
-  Seg* edgSeg;             // W*4
+  // neighbour edges (c of layer l -> n of layer l+1), first in (c, n) order,
+  // then transposed into predecessor lists
+  int *succCnt, *succCand;  // per cell: accepted successors, kEdgeSlots slots
+  int *eFrom, *eTo, *eTmp;  // compact, edgSeg order
+  int *edgeCnt, *edgeOff;   // per cell of layer l+1: its predecessor list
+  Seg* edgSeg;              // W*4
   int* pred;
   int eCap;
   // roads

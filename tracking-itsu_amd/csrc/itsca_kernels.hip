@@ -149,6 +149,7 @@ __global__ void __launch_bounds__(1024) kPrep(DevState s)
     s.cz[p] = s.rz[rawBase + i];
     s.cphi[p] = s.sPhi[base + i];
     s.cr[p] = s.sR[base + i];
+    s.crzp[p] = make_float4(s.sR[base + i], s.rz[rawBase + i], s.sPhi[base + i], 0.0f);
     s.cid[p] = s.rid[rawBase + i];
     s.cmc[p] = s.rmc[rawBase + i];
     s.cbin[p] = b;
@@ -281,14 +282,23 @@ __device__ __forceinline__ void forEachTracklet(const DevState& s, const int* ta
     const int jFirst = table[firstBin];
     const int jLast = table[maxBin];
     // Inclusive upper bound as in the reference, guarded by the layer size.
-    for (int j = jFirst; j <= jLast && j < nNext; ++j) {
-      const float rn = s.cr[baseNext + j];
-      const float zn = s.cz[baseNext + j];
-      const float phin = s.cphi[baseNext + j];
-      const float deltaZ = fabsf(tanLambda * (rn - r) + z - zn);
-      const float deltaPhi = fabsf(phi - phin);
+    const int jEnd = min(jLast + 1, nNext);
+    auto test = [&](const float4& q, int j) {
+      const float deltaZ = fabsf(tanLambda * (q.x - r) + z - q.y);
+      const float deltaPhi = fabsf(phi - q.z);
       if (deltaZ < maxDeltaZ && (deltaPhi < kPhiCut || fabsf(deltaPhi - kTwoPi) < kPhiCut)) emit(j);
+    };
+    int j = jFirst;
+    // four loads in flight before the first test; emission order unchanged
+    for (; j + 4 <= jEnd; j += 4) {
+      const float4 q0 = s.crzp[baseNext + j], q1 = s.crzp[baseNext + j + 1], q2 = s.crzp[baseNext + j + 2],
+                   q3 = s.crzp[baseNext + j + 3];
+      test(q0, j);
+      test(q1, j + 1);
+      test(q2, j + 2);
+      test(q3, j + 3);
     }
+    for (; j < jEnd; ++j) test(s.crzp[baseNext + j], j);
     ++row;
     if (row == kPhiBins) row = 0;
   }
@@ -332,8 +342,7 @@ __device__ __forceinline__ void writeTracklet(const DevState& s, int k, int i, i
   const float zi = s.cz[pi], ri = s.cr[pi];
   s.tFirst[k] = i;
   s.tSecond[k] = j;
-  s.tTanL[k] = (zi - s.cz[q]) / (ri - s.cr[q]);
-  s.tPhi[k] = fdlibmAtan2f(s.cy[pi] - s.cy[q], s.cx[pi] - s.cx[q]);
+  s.tTP[k] = make_float2((zi - s.cz[q]) / (ri - s.cr[q]), fdlibmAtan2f(s.cy[pi] - s.cy[q], s.cx[pi] - s.cx[q]));
   s.cellCnt[k] = 0;  // stays 0 where the cell stage does not run
 }
 
@@ -352,9 +361,9 @@ __device__ __forceinline__ int ownerOfSlot(const int* exs, int o)
   return lo;
 }
 
-// Fill: one output record per thread (coalesced writes, no divergent window
-// search); clusters with more than kTrkSlots tracklets are recomputed by their
-// own thread.
+// Fill: one output record per thread of the chunk (coalesced writes, no
+// divergent window search); clusters with more than kTrkSlots tracklets are
+// recomputed by their own thread.
 __global__ void __launch_bounds__(kBlock) kTrackletsFill(DevState s)
 {
   __shared__ int scratch[kBlock / 64 + 1];
@@ -450,17 +459,17 @@ __device__ __forceinline__ CellSeed cellSeed(const DevState& s, int w, int l, in
   c.d1x = c.x2 - c.x1;
   c.d1y = c.y2 - c.y1;
   c.d1z = c.r2sq - c.r1sq;
-  c.tTanL = s.tTanL[t];
-  c.tPhi = s.tPhi[t];
+  const float2 tp = s.tTP[t];
+  c.tTanL = tp.x;
+  c.tPhi = tp.y;
   return c;
 }
 
 // Cheap cuts of candidate iu (Tracker.cxx:161-173): tanLambda and phi
 // continuity, then the z of the line through the first cluster at the vertex.
-__device__ __forceinline__ bool cellCheapCuts(const DevState& s, const CellSeed& c, int iu)
+__device__ __forceinline__ bool cellCheapCuts(const CellSeed& c, float2 up)
 {
-  const int u = c.tnBase + iu;
-  const float uTanL = s.tTanL[u], uPhi = s.tPhi[u];
+  const float uTanL = up.x, uPhi = up.y;
   const float deltaTanLambda = fabsf(c.tTanL - uTanL);
   const float deltaPhi = fabsf(c.tPhi - uPhi);
   if (!(deltaTanLambda < kCellMaxDeltaTanLambda &&
@@ -523,8 +532,8 @@ __global__ void __launch_bounds__(kBlock) kCellsCount(DevState s)
       const CellSeed c = cellSeed(s, w, l, t);
       int* slots = s.cellCand + (size_t)t * kCellSlots;
       int m = 0;
-      for (int iu = c.uFirst; iu < c.uFirst + c.nCand; ++iu) {
-        if (!cellCheapCuts(s, c, iu)) continue;
+      const int uEnd = c.uFirst + c.nCand;
+      auto survivor = [&](int iu) {
         if (m < kCellList) {
           list[m][threadIdx.x] = iu;
         } else {  // long survivor list: fit inline
@@ -542,7 +551,19 @@ __global__ void __launch_bounds__(kBlock) kCellsCount(DevState s)
           }
         }
         ++m;
+      };
+      int iu = c.uFirst;
+      // four loads in flight before the first test; survivor order unchanged
+      for (; iu + 4 <= uEnd; iu += 4) {
+        const float2 u0 = s.tTP[c.tnBase + iu], u1 = s.tTP[c.tnBase + iu + 1], u2 = s.tTP[c.tnBase + iu + 2],
+                     u3 = s.tTP[c.tnBase + iu + 3];
+        if (cellCheapCuts(c, u0)) survivor(iu);
+        if (cellCheapCuts(c, u1)) survivor(iu + 1);
+        if (cellCheapCuts(c, u2)) survivor(iu + 2);
+        if (cellCheapCuts(c, u3)) survivor(iu + 3);
       }
+      for (; iu < uEnd; ++iu)
+        if (cellCheapCuts(c, s.tTP[c.tnBase + iu])) survivor(iu);
       if (m <= kCellList) {
         for (int q = 0; q < m; ++q) {
           CellOut o;
@@ -578,8 +599,8 @@ __device__ __forceinline__ void writeCell(const DevState& s, const CellSeed& c, 
   s.edgeCnt[k] = 0;
 }
 
-// Fill: one cell per thread, refitted from the kept candidate; tracklets with
-// more than kCellSlots cells are redone by their own thread.
+// Fill: one cell per thread of the chunk, refitted from the kept candidate;
+// tracklets with more than kCellSlots cells are redone by their own thread.
 __global__ void __launch_bounds__(kBlock) kCellsFill(DevState s)
 {
   __shared__ int scratch[kBlock / 64 + 1];
@@ -618,7 +639,7 @@ __global__ void __launch_bounds__(kBlock) kCellsFill(DevState s)
       int k = chunkBase + ex;
       for (int iu = cs.uFirst; iu < cs.uFirst + cs.nCand; ++iu) {
         CellOut out;
-        if (cellCheapCuts(s, cs, iu) && cellFit(s, cs, iu, out))
+        if (cellCheapCuts(cs, s.tTP[cs.tnBase + iu]) && cellFit(s, cs, iu, out))
           writeCell(s, cs, k++, it, iu, out, s.tFirst[t], s.tSecond[t]);
       }
     }
@@ -626,77 +647,11 @@ __global__ void __launch_bounds__(kBlock) kCellsFill(DevState s)
 }
 
 // ---------------------------------------------------------------------------
-// Tracklets of layer l grouped by second cluster, ascending within a group:
-// a counting sort in LDS, key range in passes of kKeysPerPass.  One workgroup
-// per (item, layer), layers 0..3.  Bucket order from the LDS atomics is made
-// deterministic by sorting each (small) bucket afterwards.
-// ---------------------------------------------------------------------------
-// One workgroup per (item, layer, key range of kInKeys keys).  Each workgroup
-// counts the tracklets whose key lies below its range (its base), builds the
-// histogram of its keys in LDS, places the tracklets into buckets with LDS
-// atomics (arbitrary order inside a bucket) and finally writes each tracklet
-// at bucket start + (number of bucket members with a smaller index): the
-// result is ascending inside each bucket whatever the atomics did.
-constexpr int kInKeys = 2048;
-
-__global__ void __launch_bounds__(1024) kInIndex(DevState s, int rangesPerSeg)
-{
-  __shared__ int hist[kInKeys];
-  __shared__ int start[kInKeys];
-  __shared__ int scratch[1024 / 64 + 1];
-  const int seg = blockIdx.x / rangesPerSeg, range = blockIdx.x % rangesPerSeg;
-  const int w = seg / (kCellLayers - 1), l = seg % (kCellLayers - 1);
-  const Seg tl = s.trkSeg[w * kTrackletLayers + l];
-  const Seg keys = s.clSeg[w * kLayers + l + 1];
-  const int k0 = range * kInKeys;
-  if (k0 >= keys.count) return;
-  const int nk = min(kInKeys, keys.count - k0);
-  const int* key = s.tSecond + tl.base;
-  int below = 0;
-  for (int k = threadIdx.x; k < nk; k += blockDim.x) hist[k] = 0;
-  for (int t = threadIdx.x; t < tl.count; t += blockDim.x) below += key[t] < k0 ? 1 : 0;
-  below = blockSum(below, scratch);
-  for (int t = threadIdx.x; t < tl.count; t += blockDim.x) {
-    const int kk = key[t] - k0;
-    if (kk >= 0 && kk < nk) atomicAdd(&hist[kk], 1);
-  }
-  __syncthreads();
-  int carry = below;
-  for (int k1 = 0; k1 < nk; k1 += blockDim.x) {
-    const int k = k1 + threadIdx.x;
-    const int v = k < nk ? hist[k] : 0;
-    int total;
-    const int ex = blockExclusiveScan(v, scratch, total);
-    if (k < nk) {
-      start[k] = carry + ex;
-      s.inOff[keys.base + k0 + k] = carry + ex;
-      hist[k] = carry + ex;  // insertion cursor
-    }
-    carry += total;
-  }
-  __syncthreads();
-  int* tmp = s.inTmp + tl.base;
-  for (int t = threadIdx.x; t < tl.count; t += blockDim.x) {
-    const int kk = key[t] - k0;
-    if (kk >= 0 && kk < nk) tmp[atomicAdd(&hist[kk], 1)] = t;
-  }
-  __threadfence_block();
-  __syncthreads();
-  for (int t = threadIdx.x; t < tl.count; t += blockDim.x) {
-    const int kk = key[t] - k0;
-    if (kk < 0 || kk >= nk) continue;
-    const int lo = start[kk], hi = hist[kk];  // the cursor ended at the bucket end
-    int rank = 0;
-    for (int a = lo; a < hi; ++a) rank += tmp[a] < t ? 1 : 0;
-    s.inList[tl.base + lo + rank] = t;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Stage (c): neighbours.  Tracker.cxx:416-474.  One thread per cell n of layer
-// l+1 gathers its predecessors: cells (t, u) of layer l with u = n's first
-// tracklet, for t in the tracklets of layer l ending at u's first cluster, in
-// ascending order -- the order the reference pushes them.
+// Stage (c): neighbours.  Tracker.cxx:416-474.  The successors of cell c of
+// layer l are the cells of layer l+1 whose first tracklet is c's second
+// tracklet: a contiguous range.  Edges are produced in (c, n) order by
+// count -> scan -> fill, then transposed into each n's predecessor list in
+// ascending c -- the order in which the reference pushes them.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(1024) kPlanEdges(DevState s, int* counts)
 {
@@ -704,46 +659,31 @@ __global__ void __launch_bounds__(1024) kPlanEdges(DevState s, int* counts)
   const int S = s.W * (kCellLayers - 1);
   for (int i = threadIdx.x; i < S; i += blockDim.x) {
     const int w = i / (kCellLayers - 1), l = i % (kCellLayers - 1);
-    // Every cell of layer l+1 gets its (possibly empty) predecessor list.
-    counts[i] = s.celSeg[w * kCellLayers + l + 1].count;
+    const int cNext = s.celSeg[w * kCellLayers + l + 1].count;
+    counts[i] = cNext > 0 ? s.celSeg[w * kCellLayers + l].count : 0;
   }
   __syncthreads();
   planChunks(counts, S, s.chunkStart + kStEdges * (s.maxSegs + 1), scratch);
 }
 
 template <typename F>
-__device__ __forceinline__ void forEachPredecessor(const DevState& s, int w, int l, int n /*global cell l+1*/,
-                                                   F&& emit)
+__device__ __forceinline__ void forEachSuccessor(const DevState& s, int w, int l, int c /*global cell of l*/,
+                                                 F&& emit)
 {
-  const Seg tl = s.trkSeg[w * kTrackletLayers + l];
-  const Seg tn = s.trkSeg[w * kTrackletLayers + l + 1];
-  const Seg cl = s.celSeg[w * kCellLayers + l];
-  const int keyBase = s.clSeg[w * kLayers + l + 1].base, keyCount = s.clSeg[w * kLayers + l + 1].count;
-  const int u = s.ct1[n];
-  const int k = s.tFirst[tn.base + u];
-  const int lo = s.inOff[keyBase + k];
-  const int hi = k + 1 < keyCount ? s.inOff[keyBase + k + 1] : tl.count;
-  const float nx = s.cnx[n], ny = s.cny[n], nz = s.cnz[n], curv = s.ccurv[n];
+  const int tnBase = s.trkSeg[w * kTrackletLayers + l + 1].base;
+  const int cnBase = s.celSeg[w * kCellLayers + l + 1].base;
+  const int u = tnBase + s.ct2[c];
+  const int cnt = s.cellCnt[u];
+  if (cnt == 0) return;
+  const int first = s.cellOff[u];
+  const float nx = s.cnx[c], ny = s.cny[c], nz = s.cnz[c], curv = s.ccurv[c];
   const float maxNormal = neighbourMaxNormalDelta(l), maxCurv = neighbourMaxCurvatureDelta(l);
-  for (int a = lo; a < hi; ++a) {
-    const int t = s.inList[tl.base + a];
-    const int cnt = s.cellCnt[tl.base + t];
-    if (cnt == 0) continue;
-    // Cells of tracklet t are ordered by their second tracklet; find u.
-    int b0 = s.cellOff[tl.base + t], b1 = b0 + cnt;
-    while (b0 < b1) {
-      const int mid = (b0 + b1) >> 1;
-      if (s.ct2[cl.base + mid] < u)
-        b0 = mid + 1;
-      else
-        b1 = mid;
-    }
-    if (b0 >= s.cellOff[tl.base + t] + cnt || s.ct2[cl.base + b0] != u) continue;
-    const int c = cl.base + b0;
-    const float dx = s.cnx[c] - nx, dy = s.cny[c] - ny, dz = s.cnz[c] - nz;
+  for (int in = first; in < first + cnt; ++in) {
+    const int n = cnBase + in;
+    const float dx = nx - s.cnx[n], dy = ny - s.cny[n], dz = nz - s.cnz[n];
     const float deltaNormal = (dx * dx) + (dy * dy) + (dz * dz);
-    const float deltaCurvature = fabsf(s.ccurv[c] - curv);
-    if (deltaNormal < maxNormal && deltaCurvature < maxCurv) emit(b0);
+    const float deltaCurvature = fabsf(curv - s.ccurv[n]);
+    if (deltaNormal < maxNormal && deltaCurvature < maxCurv) emit(in);
   }
 }
 
@@ -756,12 +696,17 @@ __global__ void __launch_bounds__(kBlock) kEdgesCount(DevState s)
   for (int chunk = xcdBlockIndex(); chunk < nChunks; chunk += gridDim.x) {
     const int sg = segmentOfChunk(chunkStart, S, chunk);
     const int w = sg / (kCellLayers - 1), l = sg % (kCellLayers - 1);
-    const Seg cn = s.celSeg[w * kCellLayers + l + 1];
-    const int in = (chunk - chunkStart[sg]) * kChunk + threadIdx.x;
+    const Seg cl = s.celSeg[w * kCellLayers + l];
+    const int ic = (chunk - chunkStart[sg]) * kChunk + threadIdx.x;
     int cnt = 0;
-    if (in < cn.count) {
-      forEachPredecessor(s, w, l, cn.base + in, [&](int) { ++cnt; });
-      s.edgeCnt[cn.base + in] = cnt;
+    if (ic < cl.count) {
+      const int c = cl.base + ic;
+      int* slots = s.succCand + (size_t)c * kEdgeSlots;
+      forEachSuccessor(s, w, l, c, [&](int in) {
+        if (cnt < kEdgeSlots) slots[cnt] = in;
+        ++cnt;
+      });
+      s.succCnt[c] = cnt;
     }
     const int total = blockSum(cnt, scratch);
     if (threadIdx.x == 0) s.chunkSum[(size_t)kStEdges * s.maxChunks + chunk] = total;
@@ -771,29 +716,104 @@ __global__ void __launch_bounds__(kBlock) kEdgesCount(DevState s)
 __global__ void __launch_bounds__(kBlock) kEdgesFill(DevState s)
 {
   __shared__ int scratch[kBlock / 64 + 1];
+  __shared__ int exs[kBlock + 1];
   const int S = s.W * (kCellLayers - 1);
   const int* chunkStart = s.chunkStart + kStEdges * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
   for (int chunk = xcdBlockIndex(); chunk < nChunks; chunk += gridDim.x) {
     const int sg = segmentOfChunk(chunkStart, S, chunk);
     const int w = sg / (kCellLayers - 1), l = sg % (kCellLayers - 1);
-    const Seg cn = s.celSeg[w * kCellLayers + l + 1];
-    const int in = (chunk - chunkStart[sg]) * kChunk + threadIdx.x;
-    const int n = cn.base + in;
-    const int cnt = in < cn.count ? s.edgeCnt[n] : 0;
+    const Seg cl = s.celSeg[w * kCellLayers + l];
+    const int firstC = (chunk - chunkStart[sg]) * kChunk;
+    const int ic = firstC + threadIdx.x;
+    const int c = cl.base + ic;
+    const int cnt = ic < cl.count ? s.succCnt[c] : 0;
     int total;
     const int ex = blockExclusiveScan(cnt, scratch, total);
-    const int out = s.chunkOff[(size_t)kStEdges * s.maxChunks + chunk] + ex;
-    if (in < cn.count) {
-      s.edgeOff[n] = out - s.edgSeg[sg].base;
-      if (cnt > 0) {
-        int k = out;
-        forEachPredecessor(s, w, l, n, [&](int c) {
-          if (k < s.eCap) s.pred[k] = c;
-          ++k;
-        });
+    exs[threadIdx.x] = ex;
+    if (threadIdx.x == 0) exs[kBlock] = total;
+    const int chunkBase = s.chunkOff[(size_t)kStEdges * s.maxChunks + chunk];
+    __syncthreads();
+    for (int o = threadIdx.x; o < total; o += kBlock) {
+      const int owner = ownerOfSlot(exs, o);
+      if (exs[owner + 1] - exs[owner] > kEdgeSlots) continue;
+      const int k = chunkBase + o;
+      if (k < s.eCap) {
+        s.eFrom[k] = firstC + owner;
+        s.eTo[k] = s.succCand[(size_t)(cl.base + firstC + owner) * kEdgeSlots + (o - exs[owner])];
       }
     }
+    if (cnt > kEdgeSlots) {
+      int k = chunkBase + ex;
+      forEachSuccessor(s, w, l, c, [&](int in) {
+        if (k < s.eCap) {
+          s.eFrom[k] = ic;
+          s.eTo[k] = in;
+        }
+        ++k;
+      });
+    }
+  }
+}
+
+// Predecessor lists: a stable counting sort of the (c, n) edges of one
+// (item, layer) by n.  One workgroup per (item, layer); keys in passes of
+// kTransposeKeys.  Bucket placement by LDS atomics, then each edge is written
+// at bucket start + (number of bucket members with a smaller c), which keeps
+// the predecessors ascending whatever order the atomics took.
+constexpr int kTransposeKeys = 8192;
+
+__global__ void __launch_bounds__(1024) kEdgesTranspose(DevState s)
+{
+  __shared__ int hist[kTransposeKeys];
+  __shared__ int start[kTransposeKeys];
+  __shared__ int scratch[1024 / 64 + 1];
+  const int w = blockIdx.x / (kCellLayers - 1), l = blockIdx.x % (kCellLayers - 1);
+  const Seg cn = s.celSeg[w * kCellLayers + l + 1];
+  const Seg e = s.edgSeg[w * (kCellLayers - 1) + l];
+  const int* to = s.eTo + e.base;
+  const int* from = s.eFrom + e.base;
+  int* tmp = s.eTmp + e.base;
+  int carry = 0;
+  for (int k0 = 0; k0 < cn.count; k0 += kTransposeKeys) {
+    const int nk = min(kTransposeKeys, cn.count - k0);
+    for (int k = threadIdx.x; k < nk; k += blockDim.x) hist[k] = 0;
+    __syncthreads();
+    for (int a = threadIdx.x; a < e.count; a += blockDim.x) {
+      const int kk = to[a] - k0;
+      if (kk >= 0 && kk < nk) atomicAdd(&hist[kk], 1);
+    }
+    __syncthreads();
+    for (int k1 = 0; k1 < nk; k1 += blockDim.x) {
+      const int k = k1 + threadIdx.x;
+      const int v = k < nk ? hist[k] : 0;
+      int total;
+      const int ex = blockExclusiveScan(v, scratch, total);
+      if (k < nk) {
+        start[k] = carry + ex;
+        s.edgeOff[cn.base + k0 + k] = carry + ex;
+        s.edgeCnt[cn.base + k0 + k] = v;
+        hist[k] = carry + ex;  // insertion cursor
+      }
+      carry += total;
+    }
+    __syncthreads();
+    for (int a = threadIdx.x; a < e.count; a += blockDim.x) {
+      const int kk = to[a] - k0;
+      if (kk >= 0 && kk < nk) tmp[atomicAdd(&hist[kk], 1)] = a;
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (int a = threadIdx.x; a < e.count; a += blockDim.x) {
+      const int kk = to[a] - k0;
+      if (kk < 0 || kk >= nk) continue;
+      const int lo = start[kk], hi = hist[kk];  // the cursor ended at the bucket end
+      const int mine = from[a];
+      int rank = 0;
+      for (int b = lo; b < hi; ++b) rank += from[tmp[b]] < mine ? 1 : 0;
+      s.pred[e.base + lo + rank] = mine;
+    }
+    __syncthreads();
   }
 }
 
@@ -979,10 +999,6 @@ void launchPipeline(const DevState& s, const LaunchConfig& cfg, hipStream_t stre
   done("kScanChunks", 2);
   hipLaunchKernelGGL(kCellsFill, dim3(G), dim3(kBlock), 0, stream, s);
   done("kCellsFill", 2);
-  const int ranges = (cfg.maxLayerClusters + kInKeys - 1) / kInKeys;
-  hipLaunchKernelGGL(kInIndex, dim3(s.W * (kCellLayers - 1) * (ranges > 0 ? ranges : 1)), dim3(1024), 0, stream, s,
-                     ranges > 0 ? ranges : 1);
-  done("kInIndex", 3);
   hipLaunchKernelGGL(kPlanEdges, dim3(1), dim3(1024), 0, stream, s, counts);
   done("kPlanEdges", 3);
   hipLaunchKernelGGL(kEdgesCount, dim3(G), dim3(kBlock), 0, stream, s);
@@ -992,6 +1008,8 @@ void launchPipeline(const DevState& s, const LaunchConfig& cfg, hipStream_t stre
   done("kScanChunks", 3);
   hipLaunchKernelGGL(kEdgesFill, dim3(G), dim3(kBlock), 0, stream, s);
   done("kEdgesFill", 3);
+  hipLaunchKernelGGL(kEdgesTranspose, dim3(s.W * (kCellLayers - 1)), dim3(1024), 0, stream, s);
+  done("kEdgesTranspose", 3);
   for (int layer = 1; layer < kCellLayers; ++layer) {
     // The last level launch shares its chunk plan with the roads stage: it also
     // writes each chunk's number of roads.

@@ -93,9 +93,11 @@ struct itsca_ctx {
   DBuf<int> rid, rmc, evLayerOff, dItemEvent;
   DBuf<Seg> dClSeg, dTrkSeg, dCelSeg, dEdgSeg, dRoadSeg;
   DBuf<float> cx, cy, cz, cphi, cr, sPhi, sR;
-  DBuf<int> cid, cmc, cbin, sBin, sTmp, trkCnt, trkOff, inOff, table;
-  DBuf<int> tFirst, tSecond, cellCnt, cellOff, inList, inTmp, trkCand, cellCand;
-  DBuf<float> tTanL, tPhi;
+  DBuf<int> cid, cmc, cbin, sBin, sTmp, trkCnt, trkOff, table;
+  DBuf<float4> crzp;
+  DBuf<int> tFirst, tSecond, cellCnt, cellOff, trkCand, cellCand;
+  DBuf<float2> tTP;
+  DBuf<int> succCnt, succCand, eFrom, eTo, eTmp;
   DBuf<int> c1, c2, c3, ct1, ct2, clevel, edgeCnt, edgeOff, roadOff;
   DBuf<float> cnx, cny, cnz, ccurv;
   DBuf<uint32_t> cpaths;
@@ -154,17 +156,14 @@ struct itsca_ctx {
     HIP_TRY(sTmp.reserve(SC));
     HIP_TRY(trkCnt.reserve(SC));
     HIP_TRY(trkOff.reserve(SC));
-    HIP_TRY(inOff.reserve(SC));
+    HIP_TRY(crzp.reserve(SC));
     HIP_TRY(trkCand.reserve(SC * kTrkSlots));
     HIP_TRY(table.reserve((size_t)W * kLayers * kTableSize));
     HIP_TRY(tFirst.reserve(tCap));
     HIP_TRY(tSecond.reserve(tCap));
-    HIP_TRY(tTanL.reserve(tCap));
-    HIP_TRY(tPhi.reserve(tCap));
+    HIP_TRY(tTP.reserve(tCap));
     HIP_TRY(cellCnt.reserve(tCap));
     HIP_TRY(cellOff.reserve(tCap));
-    HIP_TRY(inList.reserve(tCap));
-    HIP_TRY(inTmp.reserve(tCap));
     HIP_TRY(cellCand.reserve((size_t)tCap * kCellSlots));
     HIP_TRY(c1.reserve(cCap));
     HIP_TRY(c2.reserve(cCap));
@@ -181,6 +180,11 @@ struct itsca_ctx {
     HIP_TRY(edgeOff.reserve(cCap));
     HIP_TRY(roadOff.reserve(cCap));
     HIP_TRY(pred.reserve(eCap));
+    HIP_TRY(succCnt.reserve(cCap));
+    HIP_TRY(succCand.reserve((size_t)cCap * kEdgeSlots));
+    HIP_TRY(eFrom.reserve(eCap));
+    HIP_TRY(eTo.reserve(eCap));
+    HIP_TRY(eTmp.reserve(eCap));
     HIP_TRY(rCells.reserve((size_t)rCap * kCellLayers));
     HIP_TRY(rLabel.reserve(rCap));
     HIP_TRY(rFake.reserve(rCap));
@@ -214,6 +218,7 @@ struct itsca_ctx {
     s.cz = cz.p;
     s.cphi = cphi.p;
     s.cr = cr.p;
+    s.crzp = crzp.p;
     s.cid = cid.p;
     s.cmc = cmc.p;
     s.cbin = cbin.p;
@@ -228,8 +233,7 @@ struct itsca_ctx {
     s.trkSeg = dTrkSeg.p;
     s.tFirst = tFirst.p;
     s.tSecond = tSecond.p;
-    s.tTanL = tTanL.p;
-    s.tPhi = tPhi.p;
+    s.tTP = tTP.p;
     s.tCap = tCap;
     s.cellCnt = cellCnt.p;
     s.cellOff = cellOff.p;
@@ -247,9 +251,11 @@ struct itsca_ctx {
     s.ccurv = ccurv.p;
     s.cpaths = cpaths.p;
     s.cCap = cCap;
-    s.inOff = inOff.p;
-    s.inList = inList.p;
-    s.inTmp = inTmp.p;
+    s.succCnt = succCnt.p;
+    s.succCand = succCand.p;
+    s.eFrom = eFrom.p;
+    s.eTo = eTo.p;
+    s.eTmp = eTmp.p;
     s.edgeCnt = edgeCnt.p;
     s.edgeOff = edgeOff.p;
     s.edgSeg = dEdgSeg.p;
@@ -574,8 +580,13 @@ itsca_status itsca_download(itsca_ctx* ctx, int item, int what, int layer, void*
         if (!layerOk(0, 5)) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_download: bad layer");
         const Seg g = ctx->trkSeg[item * kTrackletLayers + layer];
         auto a = fetch(ctx->tFirst.p + g.base, g.count), b = fetch(ctx->tSecond.p + g.base, g.count);
-        auto t = asInt(fetch(ctx->tTanL.p + g.base, g.count)), p = asInt(fetch(ctx->tPhi.p + g.base, g.count));
+        auto tp = fetch(ctx->tTP.p + g.base, g.count);
         HIP_TRY(hipStreamSynchronize(ctx->stream));
+        std::vector<int32_t> t(g.count), p(g.count);
+        for (int i = 0; i < g.count; ++i) {
+          std::memcpy(&t[i], &tp[i].x, 4);
+          std::memcpy(&p[i], &tp[i].y, 4);
+        }
         interleave({&a, &b, &t, &p}, g.count);
         break;
       }

@@ -71,7 +71,13 @@ def hip_lib() -> ctypes.CDLL:
     """Loads libitsca_hip.so (the gfx950 kernels and the C ABI)."""
     global _hip
     if _hip is None:
-        lib = ctypes.CDLL(_path("libitsca_hip.so"))
+        _hip = bind_hip(ctypes.CDLL(_path("libitsca_hip.so")))
+    return _hip
+
+
+def bind_hip(lib: ctypes.CDLL) -> ctypes.CDLL:
+    """Declares the C ABI of include/itsca.h on a loaded library."""
+    if True:
         P, I, F = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_float)
         I32 = ctypes.POINTER(ctypes.c_int32)
         lib.itsca_create.argtypes = [ctypes.POINTER(P), ctypes.POINTER(_Config)]
@@ -90,8 +96,7 @@ def hip_lib() -> ctypes.CDLL:
         lib.itsca_kernel_times.argtypes = [P, F, I]
         lib.itsca_kernel_name.argtypes = [P, I]
         lib.itsca_kernel_name.restype = ctypes.c_char_p
-        _hip = lib
-    return _hip
+    return lib
 
 
 def host_lib() -> ctypes.CDLL:
@@ -197,8 +202,9 @@ def load_events(path: str) -> List[Event]:
 class GpuTracker:
     """A context of the gfx950 tracker (one HIP stream, device-resident batch)."""
 
-    def __init__(self, device: int = 0, persistent_blocks: int = 0, capacity_scale: float = 0.0):
-        lib = hip_lib()
+    def __init__(self, device: int = 0, persistent_blocks: int = 0, capacity_scale: float = 0.0,
+                 lib: Optional[ctypes.CDLL] = None):
+        lib = lib if lib is not None else hip_lib()
         self._lib = lib
         self.ctx = ctypes.c_void_p()
         cfg = _Config(device, persistent_blocks, capacity_scale)
