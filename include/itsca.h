@@ -105,6 +105,29 @@ itsca_status itsca_get_counts(itsca_ctx* ctx, int item, itsca_counts* out);
 itsca_status itsca_download(itsca_ctx* ctx, int item, int what, int layer, void* dst, size_t cap_bytes,
                             size_t* written);
 
+/* Launch shape of the stage kernels.  It never changes a result, only the time
+ * (the autotuner, tracking-itsu_amd/itsca/autotune.py, searches it).
+ *   block_size:        workgroup size and chunk length, 64, 128 or 256
+ *   persistent_blocks: workgroups of each persistent stage kernel (0 = 8 per CU) */
+typedef struct itsca_tuning {
+  int block_size;
+  int persistent_blocks;
+} itsca_tuning;
+itsca_status itsca_set_tuning(itsca_ctx* ctx, const itsca_tuning* tuning);
+itsca_status itsca_get_tuning(itsca_ctx* ctx, itsca_tuning* tuning);
+
+/* Autotuner: times every launch shape (block size 64/128/256 x 4/8/16
+ * workgroups per CU) over `repeats` runs of the uploaded batch, keeps the
+ * fastest, and leaves the results of one run with it.  *best_ms gets its time
+ * per run. */
+itsca_status itsca_autotune(itsca_ctx* ctx, int repeats, itsca_tuning* chosen, float* best_ms);
+
+/* Runs the pipeline as one captured HIP graph (captured on first use, replayed
+ * afterwards, captured again after the buffers grow or a new batch is
+ * uploaded).  Results are identical to the launch-by-launch run; profiled
+ * runs are never captured. */
+itsca_status itsca_set_graph(itsca_ctx* ctx, int enable);
+
 /* Per-stage device time of the last run, from HIP events on the context's
  * stream (milliseconds): [0] cluster preparation and index tables,
  * [1] tracklets, [2] cells, [3] neighbours, [4] levels and roads.

@@ -99,3 +99,51 @@ def test_capacity_growth_keeps_results(golden_dir):
     t.run()
     assert itsca.dump_item(t, 0, ev.vertices[0]) == fixture("small_s1", "stable", 0)
     t.close()
+
+
+@pytest.mark.parametrize("block", [64, 128, 256])
+def test_launch_shapes_do_not_change_results(tracker, golden_dir, block):
+    say(f"gpu tuning block={block}")
+    evs = [itsca.load_events(os.path.join(golden_dir, n + ".events.txt"))[0] for n in ("small_s3", "sparse_s6")]
+    t = itsca.GpuTracker(device=0)
+    t.set_tuning(block, 0)
+    t.set_events(evs)
+    t.run()
+    for i, name in enumerate(("small_s3", "sparse_s6")):
+        assert itsca.dump_item(t, i, evs[i].vertices[0]) == fixture(name, "stable", 0)
+    t.close()
+
+
+def test_graph_replays_match_eager():
+    say("gpu graph replays")
+    evs = itsca.generate_events(3, 3100, 4)
+    t = itsca.GpuTracker(device=0)
+    t.set_events(evs)
+    t.run()
+    ref = [itsca.dump_item(t, i, ev.vertices[0]) for i, ev in enumerate(evs)]
+    other = itsca.GpuTracker(device=0)  # eager work of the same program between replays
+    other.set_events(itsca.generate_events(1, 77, 2))
+    t.set_graph(True)
+    for rep in range(4):
+        t.run_async()
+        t.run_async()
+        t.finish()
+        for i, ev in enumerate(evs):
+            assert itsca.dump_item(t, i, ev.vertices[0]) == ref[i], (rep, i)
+        other.run()
+    t.close()
+    other.close()
+
+
+def test_autotune_keeps_results():
+    say("gpu autotune")
+    evs = itsca.generate_events(3, 3200, 3)
+    t = itsca.GpuTracker(device=0)
+    t.set_events(evs)
+    t.run()
+    ref = [t.roads(i).copy() for i in range(3)]
+    tuning, ms = t.autotune(repeats=2)
+    assert tuning.block_size in (64, 128, 256) and ms > 0
+    for i in range(3):
+        assert np.array_equal(t.roads(i), ref[i])
+    t.close()

@@ -28,10 +28,10 @@ struct Seg {
   int count;
 };
 
-// Stage plans: each stage's persistent kernels walk chunks of kChunk
-// elements; chunkStart[s] is the first chunk of segment s.
-constexpr int kChunk = 256;
-constexpr int kBlock = 256;
+// Stage plans: each stage's persistent kernels walk chunks of `chunk`
+// elements (= their workgroup size: 64, 128 or 256, chosen at run time);
+// chunkStart[s] is the first chunk of segment s.
+constexpr int kMinChunk = 64;
 // Accepted candidates kept per element between a count and its fill.
 constexpr int kTrkSlots = 32;  // per cluster (next-layer cluster positions)
 constexpr int kCellSlots = 4;  // per tracklet (next-layer tracklet indices)
@@ -51,6 +51,7 @@ enum StatusSlot {
 struct DevState {
   int W;  // work items
   int maxSegs;
+  int chunk;  // elements per chunk = workgroup size of the stage kernels
   // work items
   const int* itemEvent;
   const float* itemVertex;  // 3 per item

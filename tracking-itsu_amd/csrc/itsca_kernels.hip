@@ -160,12 +160,12 @@ __global__ void __launch_bounds__(1024) kPrep(DevState s)
 // Plans.  A segment is one (item, layer); plan kernels turn per-segment element
 // counts into chunk ranges.  Single workgroup.
 // ---------------------------------------------------------------------------
-__device__ void planChunks(const int* counts, int S, int* chunkStart, int* scratch)
+__device__ void planChunks(const int* counts, int S, int chunk, int* chunkStart, int* scratch)
 {
   int carry = 0;
   for (int s0 = 0; s0 < S; s0 += blockDim.x) {
     const int sIdx = s0 + threadIdx.x;
-    const int c = sIdx < S ? (counts[sIdx] + kChunk - 1) / kChunk : 0;
+    const int c = sIdx < S ? (counts[sIdx] + chunk - 1) / chunk : 0;
     int total;
     const int ex = blockExclusiveScan(c, scratch, total);
     if (sIdx < S) chunkStart[sIdx] = carry + ex;
@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(1024) kPlanTracklets(DevState s, int* counts)
     counts[i] = ok ? s.clSeg[w * kLayers + l].count : 0;
   }
   __syncthreads();
-  planChunks(counts, S, s.chunkStart + kStTracklets * (s.maxSegs + 1), scratch);
+  planChunks(counts, S, s.chunk, s.chunkStart + kStTracklets * (s.maxSegs + 1), scratch);
 }
 
 // Single-workgroup scan of a stage's chunk sums into global output offsets and
@@ -306,9 +306,10 @@ __device__ __forceinline__ void forEachTracklet(const DevState& s, const int* ta
 
 // Count: each cluster's accepted candidates, in emission order, are kept in
 // kTrkSlots scratch slots so that the fill does not repeat the window search.
-__global__ void __launch_bounds__(kBlock) kTrackletsCount(DevState s)
+template <int B>
+__global__ void __launch_bounds__(B) kTrackletsCount(DevState s)
 {
-  __shared__ int scratch[kBlock / 64 + 1];
+  __shared__ int scratch[B / 64 + 1];
   const int S = s.W * kTrackletLayers;
   const int* chunkStart = s.chunkStart + kStTracklets * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
@@ -316,7 +317,7 @@ __global__ void __launch_bounds__(kBlock) kTrackletsCount(DevState s)
     const int sg = segmentOfChunk(chunkStart, S, chunk);
     const int w = sg / kTrackletLayers, l = sg % kTrackletLayers;
     const Seg cur = s.clSeg[w * kLayers + l], next = s.clSeg[w * kLayers + l + 1];
-    const int i = (chunk - chunkStart[sg]) * kChunk + threadIdx.x;
+    const int i = (chunk - chunkStart[sg]) * B + threadIdx.x;
     int cnt = 0;
     if (i < cur.count) {
       const int* table = s.table + (size_t)(w * kLayers + l + 1) * kTableSize;
@@ -347,10 +348,11 @@ __device__ __forceinline__ void writeTracklet(const DevState& s, int k, int i, i
 }
 
 // Position of output slot o among the chunk's elements: the largest c with
-// exs[c] <= o (exs is the block's exclusive scan, exs[kBlock] = total > o).
+// exs[c] <= o (exs is the block's exclusive scan, exs[B] = total > o).
+template <int B>
 __device__ __forceinline__ int ownerOfSlot(const int* exs, int o)
 {
-  int lo = 0, hi = kBlock;
+  int lo = 0, hi = B;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
     if (exs[mid] <= o)
@@ -364,10 +366,11 @@ __device__ __forceinline__ int ownerOfSlot(const int* exs, int o)
 // Fill: one output record per thread of the chunk (coalesced writes, no
 // divergent window search); clusters with more than kTrkSlots tracklets are
 // recomputed by their own thread.
-__global__ void __launch_bounds__(kBlock) kTrackletsFill(DevState s)
+template <int B>
+__global__ void __launch_bounds__(B) kTrackletsFill(DevState s)
 {
-  __shared__ int scratch[kBlock / 64 + 1];
-  __shared__ int exs[kBlock + 1];
+  __shared__ int scratch[B / 64 + 1];
+  __shared__ int exs[B + 1];
   const int S = s.W * kTrackletLayers;
   const int* chunkStart = s.chunkStart + kStTracklets * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
@@ -375,19 +378,19 @@ __global__ void __launch_bounds__(kBlock) kTrackletsFill(DevState s)
     const int sg = segmentOfChunk(chunkStart, S, chunk);
     const int w = sg / kTrackletLayers, l = sg % kTrackletLayers;
     const Seg cur = s.clSeg[w * kLayers + l], next = s.clSeg[w * kLayers + l + 1];
-    const int first = (chunk - chunkStart[sg]) * kChunk;
+    const int first = (chunk - chunkStart[sg]) * B;
     const int i = first + threadIdx.x;
     const int p = cur.base + i;
     const int cnt = i < cur.count ? s.trkCnt[p] : 0;
     int total;
     const int ex = blockExclusiveScan(cnt, scratch, total);
     exs[threadIdx.x] = ex;
-    if (threadIdx.x == 0) exs[kBlock] = total;
+    if (threadIdx.x == 0) exs[B] = total;
     const int chunkBase = s.chunkOff[(size_t)kStTracklets * s.maxChunks + chunk];
     if (i < cur.count) s.trkOff[p] = chunkBase + ex - s.trkSeg[sg].base;
     __syncthreads();
-    for (int o = threadIdx.x; o < total; o += kBlock) {
-      const int c = ownerOfSlot(exs, o);
+    for (int o = threadIdx.x; o < total; o += B) {
+      const int c = ownerOfSlot<B>(exs, o);
       if (exs[c + 1] - exs[c] > kTrkSlots) continue;
       const int n = o - exs[c];
       const int j = s.trkCand[(size_t)(cur.base + first + c) * kTrkSlots + n];
@@ -420,7 +423,7 @@ __global__ void __launch_bounds__(1024) kPlanCells(DevState s, int* counts)
     counts[i] = ok ? s.trkSeg[w * kTrackletLayers + l].count : 0;
   }
   __syncthreads();
-  planChunks(counts, S, s.chunkStart + kStCells * (s.maxSegs + 1), scratch);
+  planChunks(counts, S, s.chunk, s.chunkStart + kStCells * (s.maxSegs + 1), scratch);
 }
 
 struct CellOut {
@@ -514,10 +517,11 @@ constexpr int kCellList = 8;  // cheap-cut survivors kept per thread before the 
 // Count: the cheap cuts run over every candidate; survivors (usually 0-2) are
 // fitted in a second loop, so the expensive fit does not sit inside the
 // divergent candidate loop.  Accepted candidates are kept in kCellSlots slots.
-__global__ void __launch_bounds__(kBlock) kCellsCount(DevState s)
+template <int B>
+__global__ void __launch_bounds__(B) kCellsCount(DevState s)
 {
-  __shared__ int scratch[kBlock / 64 + 1];
-  __shared__ int list[kCellList][kBlock];
+  __shared__ int scratch[B / 64 + 1];
+  __shared__ int list[kCellList][B];
   const int S = s.W * kCellLayers;
   const int* chunkStart = s.chunkStart + kStCells * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
@@ -525,7 +529,7 @@ __global__ void __launch_bounds__(kBlock) kCellsCount(DevState s)
     const int sg = segmentOfChunk(chunkStart, S, chunk);
     const int w = sg / kCellLayers, l = sg % kCellLayers;
     const Seg tl = s.trkSeg[w * kTrackletLayers + l];
-    const int it = (chunk - chunkStart[sg]) * kChunk + threadIdx.x;
+    const int it = (chunk - chunkStart[sg]) * B + threadIdx.x;
     int cnt = 0;
     if (it < tl.count) {
       const int t = tl.base + it;
@@ -601,10 +605,11 @@ __device__ __forceinline__ void writeCell(const DevState& s, const CellSeed& c, 
 
 // Fill: one cell per thread of the chunk, refitted from the kept candidate;
 // tracklets with more than kCellSlots cells are redone by their own thread.
-__global__ void __launch_bounds__(kBlock) kCellsFill(DevState s)
+template <int B>
+__global__ void __launch_bounds__(B) kCellsFill(DevState s)
 {
-  __shared__ int scratch[kBlock / 64 + 1];
-  __shared__ int exs[kBlock + 1];
+  __shared__ int scratch[B / 64 + 1];
+  __shared__ int exs[B + 1];
   const int S = s.W * kCellLayers;
   const int* chunkStart = s.chunkStart + kStCells * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
@@ -612,19 +617,19 @@ __global__ void __launch_bounds__(kBlock) kCellsFill(DevState s)
     const int sg = segmentOfChunk(chunkStart, S, chunk);
     const int w = sg / kCellLayers, l = sg % kCellLayers;
     const Seg tl = s.trkSeg[w * kTrackletLayers + l];
-    const int firstT = (chunk - chunkStart[sg]) * kChunk;
+    const int firstT = (chunk - chunkStart[sg]) * B;
     const int it = firstT + threadIdx.x;
     const int t = tl.base + it;
     const int cnt = it < tl.count ? s.cellCnt[t] : 0;
     int total;
     const int ex = blockExclusiveScan(cnt, scratch, total);
     exs[threadIdx.x] = ex;
-    if (threadIdx.x == 0) exs[kBlock] = total;
+    if (threadIdx.x == 0) exs[B] = total;
     const int chunkBase = s.chunkOff[(size_t)kStCells * s.maxChunks + chunk];
     if (it < tl.count) s.cellOff[t] = chunkBase + ex - s.celSeg[sg].base;
     __syncthreads();
-    for (int o = threadIdx.x; o < total; o += kBlock) {
-      const int c = ownerOfSlot(exs, o);
+    for (int o = threadIdx.x; o < total; o += B) {
+      const int c = ownerOfSlot<B>(exs, o);
       if (exs[c + 1] - exs[c] > kCellSlots) continue;
       const int n = o - exs[c];
       const int tt = tl.base + firstT + c;
@@ -663,7 +668,7 @@ __global__ void __launch_bounds__(1024) kPlanEdges(DevState s, int* counts)
     counts[i] = cNext > 0 ? s.celSeg[w * kCellLayers + l].count : 0;
   }
   __syncthreads();
-  planChunks(counts, S, s.chunkStart + kStEdges * (s.maxSegs + 1), scratch);
+  planChunks(counts, S, s.chunk, s.chunkStart + kStEdges * (s.maxSegs + 1), scratch);
 }
 
 template <typename F>
@@ -687,9 +692,10 @@ __device__ __forceinline__ void forEachSuccessor(const DevState& s, int w, int l
   }
 }
 
-__global__ void __launch_bounds__(kBlock) kEdgesCount(DevState s)
+template <int B>
+__global__ void __launch_bounds__(B) kEdgesCount(DevState s)
 {
-  __shared__ int scratch[kBlock / 64 + 1];
+  __shared__ int scratch[B / 64 + 1];
   const int S = s.W * (kCellLayers - 1);
   const int* chunkStart = s.chunkStart + kStEdges * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
@@ -697,7 +703,7 @@ __global__ void __launch_bounds__(kBlock) kEdgesCount(DevState s)
     const int sg = segmentOfChunk(chunkStart, S, chunk);
     const int w = sg / (kCellLayers - 1), l = sg % (kCellLayers - 1);
     const Seg cl = s.celSeg[w * kCellLayers + l];
-    const int ic = (chunk - chunkStart[sg]) * kChunk + threadIdx.x;
+    const int ic = (chunk - chunkStart[sg]) * B + threadIdx.x;
     int cnt = 0;
     if (ic < cl.count) {
       const int c = cl.base + ic;
@@ -713,10 +719,11 @@ __global__ void __launch_bounds__(kBlock) kEdgesCount(DevState s)
   }
 }
 
-__global__ void __launch_bounds__(kBlock) kEdgesFill(DevState s)
+template <int B>
+__global__ void __launch_bounds__(B) kEdgesFill(DevState s)
 {
-  __shared__ int scratch[kBlock / 64 + 1];
-  __shared__ int exs[kBlock + 1];
+  __shared__ int scratch[B / 64 + 1];
+  __shared__ int exs[B + 1];
   const int S = s.W * (kCellLayers - 1);
   const int* chunkStart = s.chunkStart + kStEdges * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
@@ -724,18 +731,18 @@ __global__ void __launch_bounds__(kBlock) kEdgesFill(DevState s)
     const int sg = segmentOfChunk(chunkStart, S, chunk);
     const int w = sg / (kCellLayers - 1), l = sg % (kCellLayers - 1);
     const Seg cl = s.celSeg[w * kCellLayers + l];
-    const int firstC = (chunk - chunkStart[sg]) * kChunk;
+    const int firstC = (chunk - chunkStart[sg]) * B;
     const int ic = firstC + threadIdx.x;
     const int c = cl.base + ic;
     const int cnt = ic < cl.count ? s.succCnt[c] : 0;
     int total;
     const int ex = blockExclusiveScan(cnt, scratch, total);
     exs[threadIdx.x] = ex;
-    if (threadIdx.x == 0) exs[kBlock] = total;
+    if (threadIdx.x == 0) exs[B] = total;
     const int chunkBase = s.chunkOff[(size_t)kStEdges * s.maxChunks + chunk];
     __syncthreads();
-    for (int o = threadIdx.x; o < total; o += kBlock) {
-      const int owner = ownerOfSlot(exs, o);
+    for (int o = threadIdx.x; o < total; o += B) {
+      const int owner = ownerOfSlot<B>(exs, o);
       if (exs[owner + 1] - exs[owner] > kEdgeSlots) continue;
       const int k = chunkBase + o;
       if (k < s.eCap) {
@@ -831,12 +838,13 @@ __global__ void __launch_bounds__(1024) kPlanLayer(DevState s, int* counts, int 
   const int S = s.W;
   for (int w = threadIdx.x; w < S; w += blockDim.x) counts[w] = s.celSeg[w * kCellLayers + layer].count;
   __syncthreads();
-  planChunks(counts, S, s.chunkStart + stage * (s.maxSegs + 1), scratch);
+  planChunks(counts, S, s.chunk, s.chunkStart + stage * (s.maxSegs + 1), scratch);
 }
 
-__global__ void __launch_bounds__(kBlock) kLevels(DevState s, int layer, int stage)
+template <int B>
+__global__ void __launch_bounds__(B) kLevels(DevState s, int layer, int stage)
 {
-  __shared__ int scratch[kBlock / 64 + 1];
+  __shared__ int scratch[B / 64 + 1];
   const int S = s.W;
   const int* chunkStart = s.chunkStart + stage * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
@@ -844,7 +852,7 @@ __global__ void __launch_bounds__(kBlock) kLevels(DevState s, int layer, int sta
     const int w = segmentOfChunk(chunkStart, S, chunk);
     const Seg cn = s.celSeg[w * kCellLayers + layer];
     const Seg cp = s.celSeg[w * kCellLayers + layer - 1];
-    const int in = (chunk - chunkStart[w]) * kChunk + threadIdx.x;
+    const int in = (chunk - chunkStart[w]) * B + threadIdx.x;
     int roots = 0;
     if (in < cn.count) {
       const int n = cn.base + in;
@@ -879,9 +887,10 @@ __global__ void __launch_bounds__(kBlock) kLevels(DevState s, int layer, int sta
 // Roads: every root (layer-4 cell of level 5) emits its chains in the order of
 // the reference's depth-first traversal -- ascending predecessor at every
 // layer -- with Moore's majority vote of the MC labels (Tracker.cxx:573-652).
-__global__ void __launch_bounds__(kBlock) kRoadsFill(DevState s)
+template <int B>
+__global__ void __launch_bounds__(B) kRoadsFill(DevState s)
 {
-  __shared__ int scratch[kBlock / 64 + 1];
+  __shared__ int scratch[B / 64 + 1];
   constexpr int L4 = kCellLayers - 1;
   const int S = s.W;
   const int* chunkStart = s.chunkStart + kStRoads * (s.maxSegs + 1);
@@ -889,7 +898,7 @@ __global__ void __launch_bounds__(kBlock) kRoadsFill(DevState s)
   for (int chunk = xcdBlockIndex(); chunk < nChunks; chunk += gridDim.x) {
     const int w = segmentOfChunk(chunkStart, S, chunk);
     const Seg c4 = s.celSeg[w * kCellLayers + L4];
-    const int in = (chunk - chunkStart[w]) * kChunk + threadIdx.x;
+    const int in = (chunk - chunkStart[w]) * B + threadIdx.x;
     int cnt = 0;
     if (in < c4.count && s.clevel[c4.base + in] == kCellsMinLevel) cnt = (int)min(s.cpaths[c4.base + in], 0x3fffffffu);
     int total;
@@ -971,7 +980,8 @@ __global__ void __launch_bounds__(kBlock) kRoadsFill(DevState s)
 // ---------------------------------------------------------------------------
 // Launch sequence of one plain-CA run over the whole batch.
 // ---------------------------------------------------------------------------
-void launchPipeline(const DevState& s, const LaunchConfig& cfg, hipStream_t stream, KernelTimer* timer)
+template <int B>
+void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t stream, KernelTimer* timer)
 {
   const int G = cfg.persistentBlocks;
   int* counts = cfg.planScratch;
@@ -983,30 +993,30 @@ void launchPipeline(const DevState& s, const LaunchConfig& cfg, hipStream_t stre
   done("kPrep", 0);
   hipLaunchKernelGGL(kPlanTracklets, dim3(1), dim3(1024), 0, stream, s, counts);
   done("kPlanTracklets", 1);
-  hipLaunchKernelGGL(kTrackletsCount, dim3(G), dim3(kBlock), 0, stream, s);
+  hipLaunchKernelGGL(kTrackletsCount<B>, dim3(G), dim3(B), 0, stream, s);
   done("kTrackletsCount", 1);
   hipLaunchKernelGGL(kScanChunks, dim3(1), dim3(1024), 0, stream, s, (int)kStTracklets, s.W * kTrackletLayers,
                      s.trkSeg, s.tCap, (int)kStatusOverflowTracklets);
   done("kScanChunks", 1);
-  hipLaunchKernelGGL(kTrackletsFill, dim3(G), dim3(kBlock), 0, stream, s);
+  hipLaunchKernelGGL(kTrackletsFill<B>, dim3(G), dim3(B), 0, stream, s);
   done("kTrackletsFill", 1);
   hipLaunchKernelGGL(kPlanCells, dim3(1), dim3(1024), 0, stream, s, counts);
   done("kPlanCells", 2);
-  hipLaunchKernelGGL(kCellsCount, dim3(G), dim3(kBlock), 0, stream, s);
+  hipLaunchKernelGGL(kCellsCount<B>, dim3(G), dim3(B), 0, stream, s);
   done("kCellsCount", 2);
   hipLaunchKernelGGL(kScanChunks, dim3(1), dim3(1024), 0, stream, s, (int)kStCells, s.W * kCellLayers, s.celSeg,
                      s.cCap, (int)kStatusOverflowCells);
   done("kScanChunks", 2);
-  hipLaunchKernelGGL(kCellsFill, dim3(G), dim3(kBlock), 0, stream, s);
+  hipLaunchKernelGGL(kCellsFill<B>, dim3(G), dim3(B), 0, stream, s);
   done("kCellsFill", 2);
   hipLaunchKernelGGL(kPlanEdges, dim3(1), dim3(1024), 0, stream, s, counts);
   done("kPlanEdges", 3);
-  hipLaunchKernelGGL(kEdgesCount, dim3(G), dim3(kBlock), 0, stream, s);
+  hipLaunchKernelGGL(kEdgesCount<B>, dim3(G), dim3(B), 0, stream, s);
   done("kEdgesCount", 3);
   hipLaunchKernelGGL(kScanChunks, dim3(1), dim3(1024), 0, stream, s, (int)kStEdges, s.W * (kCellLayers - 1),
                      s.edgSeg, s.eCap, (int)kStatusOverflowEdges);
   done("kScanChunks", 3);
-  hipLaunchKernelGGL(kEdgesFill, dim3(G), dim3(kBlock), 0, stream, s);
+  hipLaunchKernelGGL(kEdgesFill<B>, dim3(G), dim3(B), 0, stream, s);
   done("kEdgesFill", 3);
   hipLaunchKernelGGL(kEdgesTranspose, dim3(s.W * (kCellLayers - 1)), dim3(1024), 0, stream, s);
   done("kEdgesTranspose", 3);
@@ -1016,14 +1026,23 @@ void launchPipeline(const DevState& s, const LaunchConfig& cfg, hipStream_t stre
     const int stage = layer == kCellLayers - 1 ? (int)kStRoads : kStLevels + layer - 1;
     hipLaunchKernelGGL(kPlanLayer, dim3(1), dim3(1024), 0, stream, s, counts, layer, stage);
     done("kPlanLayer", 4);
-    hipLaunchKernelGGL(kLevels, dim3(G), dim3(kBlock), 0, stream, s, layer, stage);
+    hipLaunchKernelGGL(kLevels<B>, dim3(G), dim3(B), 0, stream, s, layer, stage);
     done("kLevels", 4);
   }
   hipLaunchKernelGGL(kScanChunks, dim3(1), dim3(1024), 0, stream, s, (int)kStRoads, s.W, s.roadSeg, s.rCap,
                      (int)kStatusOverflowRoads);
   done("kScanChunks", 4);
-  hipLaunchKernelGGL(kRoadsFill, dim3(G), dim3(kBlock), 0, stream, s);
+  hipLaunchKernelGGL(kRoadsFill<B>, dim3(G), dim3(B), 0, stream, s);
   done("kRoadsFill", 4);
+}
+
+void launchPipeline(const DevState& s, const LaunchConfig& cfg, hipStream_t stream, KernelTimer* timer)
+{
+  switch (s.chunk) {
+    case 64: launchPipelineB<64>(s, cfg, stream, timer); break;
+    case 128: launchPipelineB<128>(s, cfg, stream, timer); break;
+    default: launchPipelineB<256>(s, cfg, stream, timer); break;
+  }
 }
 
 }  // namespace itsca

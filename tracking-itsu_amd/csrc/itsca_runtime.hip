@@ -64,6 +64,7 @@ struct itsca_ctx {
   int device = 0;
   int cus = 0;
   int persistentBlocks = 0;
+  int blockSize = 256;  // workgroup (and chunk) size of the stage kernels
   double capacityScale = 1.0;
   hipStream_t stream = nullptr;
   std::string err;
@@ -71,6 +72,11 @@ struct itsca_ctx {
   KernelTimer timer{};
   bool timerReady = false;
   bool ran = false;
+  // whole-pipeline HIP graph, captured once per batch shape and buffer set
+  bool useGraph = false;
+  bool graphValid = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graphExec = nullptr;
 
   // staged host input
   std::vector<float> hx, hy, hz, hvert;
@@ -104,8 +110,18 @@ struct itsca_ctx {
   DBuf<int> pred, rCells, rLabel, rFake;
   DBuf<int> chunkStart, chunkSum, chunkOff, planScratch, dStatus;
 
+  void dropGraph()
+  {
+    if (graphExec) (void)hipGraphExecDestroy(graphExec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graphExec = nullptr;
+    graph = nullptr;
+    graphValid = false;
+  }
+
   ~itsca_ctx()
   {
+    dropGraph();
     if (timerReady)
       for (auto& e : timer.ev) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -139,9 +155,10 @@ struct itsca_ctx {
 
   void allocateOutputs()
   {
+    dropGraph();  // kernel arguments hold the buffer addresses
     const size_t SC = (size_t)std::max<long>(sortedClusters, 1);
     maxSegs = kLayers * std::max(W, 1);
-    maxChunks = (int)((std::max<size_t>({SC, (size_t)tCap, (size_t)cCap}) + kChunk - 1) / kChunk) + 8 * W + 1;
+    maxChunks = (int)((std::max<size_t>({SC, (size_t)tCap, (size_t)cCap}) + kMinChunk - 1) / kMinChunk) + 8 * W + 1;
     HIP_TRY(cx.reserve(SC));
     HIP_TRY(cy.reserve(SC));
     HIP_TRY(cz.reserve(SC));
@@ -204,6 +221,7 @@ struct itsca_ctx {
     DevState s{};
     s.W = W;
     s.maxSegs = maxSegs;
+    s.chunk = blockSize;
     s.itemEvent = dItemEvent.p;
     s.itemVertex = itemVertex.p;
     s.rx = rx.p;
@@ -284,7 +302,21 @@ struct itsca_ctx {
       for (auto& e : timer.ev) HIP_TRY(hipEventCreate(&e));
       timerReady = true;
     }
-    launchPipeline(state(), cfg, stream, profiling ? &timer : nullptr);
+    if (useGraph && !profiling) {
+      if (!graphValid) {
+        HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        launchPipeline(state(), cfg, stream, nullptr);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(stream, &g);
+        HIP_TRY(e);
+        graph = g;
+        HIP_TRY(hipGraphInstantiate(&graphExec, graph, nullptr, nullptr, 0));
+        graphValid = true;
+      }
+      HIP_TRY(hipGraphLaunch(graphExec, stream));
+    } else {
+      launchPipeline(state(), cfg, stream, profiling ? &timer : nullptr);
+    }
     HIP_TRY(hipGetLastError());
   }
 
@@ -669,6 +701,92 @@ itsca_status itsca_download(itsca_ctx* ctx, int item, int what, int layer, void*
     if (bytes) std::memcpy(dst, rows.data(), bytes);
     return ITSCA_OK;
   });
+}
+
+itsca_status itsca_set_tuning(itsca_ctx* ctx, const itsca_tuning* t)
+{
+  if (!ctx || !t) return ITSCA_ERR_ARGUMENT;
+  if (t->block_size != 64 && t->block_size != 128 && t->block_size != 256)
+    return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_set_tuning: block_size must be 64, 128 or 256");
+  if (t->persistent_blocks < 0) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_set_tuning: negative persistent_blocks");
+  ctx->blockSize = t->block_size;
+  ctx->persistentBlocks = t->persistent_blocks > 0 ? t->persistent_blocks : 8 * ctx->cus;
+  ctx->dropGraph();
+  return ITSCA_OK;
+}
+
+itsca_status itsca_get_tuning(itsca_ctx* ctx, itsca_tuning* t)
+{
+  if (!ctx || !t) return ITSCA_ERR_ARGUMENT;
+  t->block_size = ctx->blockSize;
+  t->persistent_blocks = ctx->persistentBlocks;
+  return ITSCA_OK;
+}
+
+// Autotuner: times every launch shape on the uploaded batch and keeps the
+// fastest.  Launch shapes never change results (the chunk plans only decide
+// which workgroup computes which element), so every candidate is valid.
+itsca_status itsca_autotune(itsca_ctx* ctx, int repeats, itsca_tuning* chosen, float* best_ms)
+{
+  return guarded(ctx, "itsca_autotune", [&]() -> itsca_status {
+    if (ctx->W == 0) return ctx->fail(ITSCA_ERR_STATE, "itsca_autotune: nothing uploaded");
+    if (repeats < 1) repeats = 1;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const bool graph = ctx->useGraph, prof = ctx->profiling;
+    ctx->profiling = false;
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    int bestBlock = ctx->blockSize, bestBlocks = ctx->persistentBlocks;
+    float best = -1.0f;
+    for (int block : {64, 128, 256}) {
+      for (int perCu : {4, 8, 16}) {
+        ctx->blockSize = block;
+        ctx->persistentBlocks = perCu * ctx->cus;
+        ctx->dropGraph();
+        const itsca_status st = itsca_run(ctx);  // sizes the buffers, captures the graph if enabled
+        if (st != ITSCA_OK) {
+          (void)hipEventDestroy(a);
+          (void)hipEventDestroy(b);
+          return st;
+        }
+        HIP_TRY(hipEventRecord(a, ctx->stream));
+        for (int r = 0; r < repeats; ++r) ctx->enqueue();
+        HIP_TRY(hipEventRecord(b, ctx->stream));
+        HIP_TRY(hipEventSynchronize(b));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, a, b));
+        ms /= (float)repeats;
+        if (best < 0 || ms < best) {
+          best = ms;
+          bestBlock = block;
+          bestBlocks = ctx->persistentBlocks;
+        }
+      }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    ctx->blockSize = bestBlock;
+    ctx->persistentBlocks = bestBlocks;
+    ctx->profiling = prof;
+    ctx->useGraph = graph;
+    ctx->dropGraph();
+    const itsca_status st = itsca_run(ctx);  // leave valid results of the chosen shape
+    if (chosen) {
+      chosen->block_size = bestBlock;
+      chosen->persistent_blocks = bestBlocks;
+    }
+    if (best_ms) *best_ms = best;
+    return st;
+  });
+}
+
+itsca_status itsca_set_graph(itsca_ctx* ctx, int enable)
+{
+  if (!ctx) return ITSCA_ERR_ARGUMENT;
+  ctx->useGraph = enable != 0;
+  if (!ctx->useGraph) ctx->dropGraph();
+  return ITSCA_OK;
 }
 
 itsca_status itsca_set_profiling(itsca_ctx* ctx, int enable)

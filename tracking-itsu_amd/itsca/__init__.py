@@ -39,6 +39,10 @@ class _Config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("persistent_blocks", ctypes.c_int), ("capacity_scale", ctypes.c_double)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("block_size", ctypes.c_int), ("persistent_blocks", ctypes.c_int)]
+
+
 class Counts(ctypes.Structure):
     _fields_ = [
         ("clusters", ctypes.c_int * 7),
@@ -52,7 +56,8 @@ class Counts(ctypes.Structure):
 HIP_SYMBOLS = [
     "itsca_create", "itsca_destroy", "itsca_last_error", "itsca_clear_events", "itsca_add_event", "itsca_upload",
     "itsca_run", "itsca_run_async", "itsca_finish", "itsca_num_items", "itsca_get_counts", "itsca_download",
-    "itsca_set_profiling", "itsca_stage_times", "itsca_kernel_times", "itsca_kernel_name",
+    "itsca_set_profiling", "itsca_stage_times", "itsca_kernel_times", "itsca_kernel_name", "itsca_set_graph",
+    "itsca_set_tuning", "itsca_get_tuning", "itsca_autotune",
 ]
 HOST_SYMBOLS = [
     "itsca_events_generate", "itsca_events_load", "itsca_events_write", "itsca_events_free", "itsca_events_count",
@@ -92,6 +97,10 @@ def bind_hip(lib: ctypes.CDLL) -> ctypes.CDLL:
         lib.itsca_get_counts.argtypes = [P, I, ctypes.POINTER(Counts)]
         lib.itsca_download.argtypes = [P, I, I, I, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         lib.itsca_set_profiling.argtypes = [P, I]
+        lib.itsca_set_graph.argtypes = [P, I]
+        lib.itsca_set_tuning.argtypes = [P, ctypes.POINTER(Tuning)]
+        lib.itsca_get_tuning.argtypes = [P, ctypes.POINTER(Tuning)]
+        lib.itsca_autotune.argtypes = [P, I, ctypes.POINTER(Tuning), F]
         lib.itsca_stage_times.argtypes = [P, F]
         lib.itsca_kernel_times.argtypes = [P, F, I]
         lib.itsca_kernel_name.argtypes = [P, I]
@@ -252,6 +261,27 @@ class GpuTracker:
 
     def finish(self) -> None:
         self._check(self._lib.itsca_finish(self.ctx), "itsca_finish")
+
+    def set_tuning(self, block_size: int, persistent_blocks: int = 0) -> None:
+        t = Tuning(block_size, persistent_blocks)
+        self._check(self._lib.itsca_set_tuning(self.ctx, ctypes.byref(t)), "itsca_set_tuning")
+
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        self._check(self._lib.itsca_get_tuning(self.ctx, ctypes.byref(t)), "itsca_get_tuning")
+        return t
+
+    def autotune(self, repeats: int = 3):
+        """Picks the fastest launch shape for the uploaded batch; returns
+        (Tuning, ms per run)."""
+        t = Tuning()
+        ms = ctypes.c_float(0)
+        self._check(self._lib.itsca_autotune(self.ctx, int(repeats), ctypes.byref(t), ctypes.byref(ms)),
+                    "itsca_autotune")
+        return t, ms.value
+
+    def set_graph(self, on: bool) -> None:
+        self._check(self._lib.itsca_set_graph(self.ctx, 1 if on else 0), "itsca_set_graph")
 
     def set_profiling(self, on: bool) -> None:
         self._check(self._lib.itsca_set_profiling(self.ctx, 1 if on else 0), "itsca_set_profiling")
