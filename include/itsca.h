@@ -128,6 +128,44 @@ itsca_status itsca_autotune(itsca_ctx* ctx, int repeats, itsca_tuning* chosen, f
  * runs are never captured. */
 itsca_status itsca_set_graph(itsca_ctx* ctx, int enable);
 
+/* ---------------------------------------------------------------------------
+ * Learned tracklet filter (no counterpart in the reference; an optional stage
+ * between the tracklet search and cell building).  A 10 -> 32 -> 32 -> 1 ReLU
+ * network with BF16 weights and activations and FP32 accumulation scores every
+ * tracklet that passed the exact tracklet cuts; tracklets below the threshold
+ * are dropped before cell building, and every cut after that is the exact CA
+ * one, so the roads found are always a subset of the plain roads.
+ * Parameters: 1441 floats, W1[32][10] b1[32] W2[32][32] b2[32] w3[32] b3.
+ * --------------------------------------------------------------------------- */
+#define ITSCA_MLP_PARAMS 1441
+#define ITSCA_MLP_FEATURES 16 /* BF16 per training row: 10 features, 6 zeros */
+itsca_status itsca_mlp_init(itsca_ctx* ctx, uint64_t seed); /* He initialisation */
+itsca_status itsca_mlp_set_params(itsca_ctx* ctx, const float* params);
+itsca_status itsca_mlp_get_params(itsca_ctx* ctx, float* params);
+/* Turns the filter on for later runs; tracklets with logit < threshold are dropped. */
+itsca_status itsca_set_filter(itsca_ctx* ctx, int enable, float threshold);
+
+/* Training set on the device: rows of ITSCA_MLP_FEATURES BF16 values and a
+ * 0/1 label (the tracklet belongs to a cell of a final road of the plain CA). */
+itsca_status itsca_train_clear(itsca_ctx* ctx, int capacity);
+/* After a plain itsca_run: labels every tracklet of the batch and appends every
+ * stride-th one (starting at `phase`) to the training set. */
+itsca_status itsca_train_collect(itsca_ctx* ctx, int stride, int phase, int* added);
+itsca_status itsca_train_set(itsca_ctx* ctx, const uint16_t* features, const uint8_t* labels, int n);
+int itsca_train_size(const itsca_ctx* ctx);
+itsca_status itsca_train_get(itsca_ctx* ctx, uint16_t* features, uint8_t* labels, int n);
+/* One step on rows (mult*k + offset) mod n, k in [first, first+count): writes the
+ * gradient of the weighted binary cross-entropy (mean over the sample weights;
+ * positives weigh pos_weight) and, when lr > 0, applies AdamW (beta1 0.9,
+ * beta2 0.999, eps 1e-8, decoupled weight decay on the weight matrices). */
+itsca_status itsca_train_step(itsca_ctx* ctx, int first, int count, long mult, long offset, float pos_weight, float lr,
+                              float weight_decay, float* loss, float* grad);
+/* One epoch of AdamW steps of `batch` rows over a pseudo-shuffled order. */
+itsca_status itsca_train_epoch(itsca_ctx* ctx, int batch, float lr, float weight_decay, float pos_weight,
+                               uint64_t seed, float* mean_loss);
+/* Logits of the first n training rows (the MFMA inference used by the filter). */
+itsca_status itsca_train_scores(itsca_ctx* ctx, float* logits, int n);
+
 /* Per-stage device time of the last run, from HIP events on the context's
  * stream (milliseconds): [0] cluster preparation and index tables,
  * [1] tracklets, [2] cells, [3] neighbours, [4] levels and roads.

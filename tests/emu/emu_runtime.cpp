@@ -18,6 +18,9 @@ struct Block {
   std::unique_ptr<std::barrier<>> all;
   std::vector<std::unique_ptr<std::barrier<>>> waves;
   std::vector<int> shfl;
+  std::vector<float> shflF;
+  std::vector<emu_bf16x4> ma, mb;
+  std::vector<emu_f32x4> mc;
 };
 thread_local Block* cur = nullptr;
 }  // namespace
@@ -36,6 +39,48 @@ int shflUp(int v, unsigned d, int width)
   return r;
 }
 
+float shflXor(float v, int mask, int width)
+{
+  const int lane = (int)(tIdx.x % 64), wave = (int)(tIdx.x / 64);
+  std::barrier<>& b = *cur->waves[wave];
+  cur->shflF[tIdx.x] = v;
+  b.arrive_and_wait();
+  const int src = lane ^ mask;
+  const float r = (src < 64 && lane / width == src / width) ? cur->shflF[wave * 64 + src] : v;
+  b.arrive_and_wait();
+  return r;
+}
+
+void waveBarrier() { cur->waves[tIdx.x / 64]->arrive_and_wait(); }
+
+emu_f32x4 mfma16x16x16bf16(emu_bf16x4 a, emu_bf16x4 b, emu_f32x4 c)
+{
+  const int lane = (int)(tIdx.x % 64), wave = (int)(tIdx.x / 64);
+  std::barrier<>& bar = *cur->waves[wave];
+  cur->ma[tIdx.x] = a;
+  cur->mb[tIdx.x] = b;
+  cur->mc[tIdx.x] = c;
+  bar.arrive_and_wait();
+  auto bf = [](short h) {
+    const uint32_t u = (uint32_t)(uint16_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+  };
+  // A[m][k]: lane m%16 + 16*(k/4), element k%4; B[k][n]: lane n + 16*(k/4), element k%4
+  emu_f32x4 d;
+  const int col = lane % 16, grp = lane / 16;
+  for (int r = 0; r < 4; ++r) {
+    const int row = grp * 4 + r;
+    float acc = c[r];
+    for (int k = 0; k < 16; ++k)
+      acc += bf(cur->ma[wave * 64 + row + 16 * (k / 4)][k % 4]) * bf(cur->mb[wave * 64 + col + 16 * (k / 4)][k % 4]);
+    d[r] = acc;
+  }
+  bar.arrive_and_wait();
+  return d;
+}
+
 void launch(dim3 grid, dim3 block, const std::function<void()>& body)
 {
   const unsigned nt = block.x;
@@ -45,6 +90,10 @@ void launch(dim3 grid, dim3 block, const std::function<void()>& body)
     const unsigned nw = (nt + 63) / 64;
     for (unsigned w = 0; w < nw; ++w) blk.waves.push_back(std::make_unique<std::barrier<>>(std::min(64u, nt - 64 * w)));
     blk.shfl.assign(nt, 0);
+    blk.shflF.assign(nt, 0.0f);
+    blk.ma.assign(nt, emu_bf16x4{0, 0, 0, 0});
+    blk.mb.assign(nt, emu_bf16x4{0, 0, 0, 0});
+    blk.mc.assign(nt, emu_f32x4{0, 0, 0, 0});
     std::vector<std::thread> th;
     th.reserve(nt);
     for (unsigned t = 0; t < nt; ++t)
@@ -69,6 +118,11 @@ hipError_t hipMalloc(void** p, size_t bytes)
 hipError_t hipFree(void* p)
 {
   std::free(p);
+  return hipSuccess;
+}
+hipError_t hipMemcpy(void* dst, const void* src, size_t bytes, hipMemcpyKind)
+{
+  std::memcpy(dst, src, bytes);
   return hipSuccess;
 }
 hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKind, hipStream_t)

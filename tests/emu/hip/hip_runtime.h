@@ -31,11 +31,19 @@ struct uint3e {
   unsigned x = 0, y = 0, z = 0;
 };
 
+typedef short emu_bf16x4 __attribute__((ext_vector_type(4)));
+typedef float emu_f32x4 __attribute__((ext_vector_type(4)));
+
 namespace emu {
 extern thread_local uint3e tIdx, bIdx;
 extern thread_local dim3 bDim, gDim;
 void syncthreads();
 int shflUp(int v, unsigned d, int width);
+float shflXor(float v, int mask, int width);
+void waveBarrier();
+// v_mfma_f32_16x16x16_bf16 as documented for CDNA wave64: lane i holds
+// A[i%16][4*(i/16)+r], B[4*(i/16)+r][i%16] and C/D[4*(i/16)+r][i%16].
+emu_f32x4 mfma16x16x16bf16(emu_bf16x4 a, emu_bf16x4 b, emu_f32x4 c);
 void launch(dim3 grid, dim3 block, const std::function<void()>& body);
 }  // namespace emu
 
@@ -47,6 +55,13 @@ void launch(dim3 grid, dim3 block, const std::function<void()>& body);
 inline void __syncthreads() { emu::syncthreads(); }
 inline void __threadfence_block() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 inline int __shfl_up(int v, unsigned d, int width = 64) { return emu::shflUp(v, d, width); }
+inline float __shfl_xor(float v, int mask, int width = 64) { return emu::shflXor(v, mask, width); }
+
+// Wave-level builtins used by the MFMA code.
+#define __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, cbsz, abid, blgp) emu::mfma16x16x16bf16(a, b, c)
+#define __builtin_amdgcn_fence(order, scope) __atomic_thread_fence(__ATOMIC_SEQ_CST)
+#define __builtin_amdgcn_wave_barrier() emu::waveBarrier()
+inline float log1pf(float x) { return std::log1p(x); }
 inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 
 inline int min(int a, int b) { return a < b ? a : b; }
@@ -80,6 +95,7 @@ hipError_t hipMalloc(T** p, size_t bytes)
 }
 hipError_t hipFree(void* p);
 hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKind, hipStream_t);
+hipError_t hipMemcpy(void* dst, const void* src, size_t bytes, hipMemcpyKind);
 hipError_t hipStreamSynchronize(hipStream_t);
 hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned);
 hipError_t hipStreamDestroy(hipStream_t);

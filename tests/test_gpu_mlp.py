@@ -1,0 +1,223 @@
+"""The learned tracklet filter on the GPU: the BF16 MFMA forward pass, the
+backward pass and AdamW against the float64 reference (tests/mlp_ref.py,
+itself pinned by test_mlp_ref.py); the training rows (features bit-exact with
+their float32 restatement, labels from the plain roads); and the pipeline
+invariants (a filter that keeps everything reproduces the plain CA bit for
+bit; any filter finds a sub-multiset of the plain roads)."""
+import sys
+from collections import Counter
+
+import numpy as np
+import pytest
+
+import itsca
+import mlp_ref as M
+from golden_cases import fixture, small_cases
+
+pytestmark = pytest.mark.gpu
+CASES = small_cases()
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def tracker():
+    t = itsca.GpuTracker(device=0)
+    yield t
+    t.close()
+
+
+def _random_params(seed):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(M.PARAMS, np.float32)
+    p[M.OFF_W1:M.OFF_B1] = rng.normal(size=320) * np.sqrt(2 / 10)
+    p[M.OFF_B1:M.OFF_W2] = rng.normal(size=32) * 0.2
+    p[M.OFF_W2:M.OFF_B2] = rng.normal(size=1024) * np.sqrt(2 / 32)
+    p[M.OFF_B2:M.OFF_W3] = rng.normal(size=32) * 0.2
+    p[M.OFF_W3:M.OFF_B3] = rng.normal(size=32) * np.sqrt(1 / 32)
+    p[M.OFF_B3] = 0.3
+    return p
+
+
+def _random_rows(seed, n):
+    rng = np.random.default_rng(seed)
+    f = rng.normal(size=(n, 10)).astype(np.float32) * np.float32(1.5)
+    f[: n // 10] *= 20  # some large inputs
+    return M.pack_rows(f), (rng.random(n) < 0.25).astype(np.uint8)
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 1000, 4099])
+def test_mfma_forward_matches_float64(tracker, n):
+    say(f"gpu mlp forward n={n}")
+    p = _random_params(n)
+    X, y = _random_rows(n + 1, n)
+    tracker.set_mlp_params(p)
+    tracker.train_set(X, y)
+    z = tracker.train_scores(n)
+    zr = M.forward(p, X)[3]
+    err = np.abs(z - zr)
+    # FP32 accumulation order differs from float64; a BF16 rounding of a hidden
+    # activation may then flip by one unit, hence the bound on the worst row
+    assert np.max(err / np.maximum(1.0, np.abs(zr))) < 3e-2
+    assert np.quantile(err, 0.99) < 1e-4 * max(1.0, float(np.max(np.abs(zr))))
+
+
+@pytest.mark.parametrize("mult,offset,first,count", [(1, 0, 0, 3000), (7, 3, 10, 513), (1, 0, 0, 1)])
+def test_gradient_matches_float64(tracker, mult, offset, first, count):
+    say(f"gpu mlp gradient {mult} {offset} {first} {count}")
+    n = 3000
+    p = _random_params(11)
+    X, y = _random_rows(12, n)
+    tracker.set_mlp_params(p)
+    tracker.train_set(X, y)
+    loss, g = tracker.train_step(first, count, mult, offset, pos_weight=4.0)
+    rows = (mult * np.arange(first, first + count) + offset) % n
+    lr_, gr = M.loss_and_grad(p, X[rows], y[rows], 4.0)
+    assert abs(loss - lr_) < 1e-5 * max(1.0, abs(lr_))
+    assert np.linalg.norm(g - gr) < 2e-3 * np.linalg.norm(gr)
+    # every block of the parameter vector, not only the dominant one
+    for a, b in [(M.OFF_W1, M.OFF_B1), (M.OFF_B1, M.OFF_W2), (M.OFF_W2, M.OFF_B2), (M.OFF_B2, M.OFF_W3),
+                 (M.OFF_W3, M.OFF_B3), (M.OFF_B3, M.PARAMS)]:
+        assert np.linalg.norm(g[a:b] - gr[a:b]) < 5e-3 * np.linalg.norm(gr[a:b]) + 1e-7, (a, b)
+    assert np.array_equal(tracker.mlp_params(), p)  # lr 0: no update
+
+
+def test_adamw_matches_closed_form(tracker):
+    say("gpu adamw")
+    p0 = _random_params(21)
+    X, y = _random_rows(22, 2000)
+    tracker.set_mlp_params(p0)
+    tracker.train_set(X, y)
+    lr, wd = 2e-3, 0.1
+    grads = []
+    for first in (0, 1000, 500):
+        _, g = tracker.train_step(first, 1000, 1, 0, pos_weight=2.0, lr=lr, weight_decay=wd)
+        grads.append(g.astype(np.float64))
+    ref = M.adamw(p0, grads, lr, wd)
+    np.testing.assert_allclose(tracker.mlp_params(), ref, rtol=1e-5, atol=2e-7)
+    # the gradient of each step is taken at the weights of the step before
+    _, g_ref = M.loss_and_grad(M.adamw(p0, grads[:1], lr, wd), X[1000:2000], y[1000:2000], 2.0)
+    assert np.linalg.norm(grads[1] - g_ref) < 2e-3 * np.linalg.norm(g_ref)
+
+
+def _reference_rows(tracker, item, vertex):
+    """Features and labels of every tracklet of one item, restated on the host
+    from the downloaded clusters, tracklets, cells and roads."""
+    n = tracker.counts(item)
+    cl = [tracker.download(item, itsca.CLUSTERS, l, n.clusters[l], 8) for l in range(itsca.LAYERS)]
+    cols = []
+    for l in range(itsca.LAYERS):
+        c = cl[l]
+        f = c[:, [4, 5, 6, 2, 1]].copy().view(np.float32)
+        cols.append({"x": f[:, 0], "y": f[:, 1], "z": f[:, 2], "r": f[:, 3], "phi": f[:, 4]})
+    trk = [tracker.download(item, itsca.TRACKLETS, l, n.tracklets[l], 4) for l in range(itsca.TRACKLET_LAYERS)]
+    cnt = [np.bincount(trk[l][:, 0], minlength=n.clusters[l]) if len(trk[l]) else np.zeros(n.clusters[l], int)
+           for l in range(itsca.TRACKLET_LAYERS)]
+    labels = [np.zeros(n.tracklets[l], np.uint8) for l in range(itsca.TRACKLET_LAYERS)]
+    if n.roads:
+        cells = [tracker.download(item, itsca.CELLS, k, n.cells[k], 10) for k in range(itsca.CELL_LAYERS)]
+        r = tracker.roads(item)
+        for k in range(itsca.CELL_LAYERS):
+            labels[k][cells[k][r[:, k], 3]] = 1
+            labels[k + 1][cells[k][r[:, k], 4]] = 1
+    X, Y = [], []
+    for l in range(itsca.TRACKLET_LAYERS):
+        if not n.tracklets[l]:
+            continue
+        a, b = trk[l][:, 0], trk[l][:, 1]
+        ci = {k: v[a] for k, v in cols[l].items()}
+        cj = {k: v[b] for k, v in cols[l + 1].items()}
+        cj_cnt = cnt[l + 1][b] if l + 1 < itsca.TRACKLET_LAYERS else np.zeros(len(b), int)
+        X.append(M.pack_rows(M.tracklet_features(ci, cj, vertex, l, cnt[l][a], cj_cnt)))
+        Y.append(labels[l])
+    if not X:
+        return np.zeros((0, 16), np.uint16), np.zeros(0, np.uint8)
+    return np.concatenate(X), np.concatenate(Y)
+
+
+def test_training_rows_match_restatement(tracker):
+    say("gpu training rows: features and labels")
+    evs = [itsca.load_events(p)[0] for _, p, _ in CASES] + itsca.generate_events(5, 41, 1)
+    tracker.set_filter(False)
+    tracker.set_events(evs)
+    tracker.run()
+    vertices = [v for ev in evs for v in ev.vertices]
+    assert len(vertices) == tracker.num_items()
+    Xr = np.concatenate([_reference_rows(tracker, i, vertices[i])[0] for i in range(len(vertices))])
+    Yr = np.concatenate([_reference_rows(tracker, i, vertices[i])[1] for i in range(len(vertices))])
+    total = sum(sum(tracker.counts(i).tracklets) for i in range(len(vertices)))
+    assert len(Yr) == total and 0 < Yr.sum() < total
+    tracker.train_clear(total + 10)
+    assert tracker.train_collect(1, 0) == total
+    X, Y = tracker.train_get(total)
+    assert np.array_equal(Y, Yr)
+    assert np.array_equal(X, Xr)
+    # a strided sample continues the set; capacity bounds what is kept
+    tracker.train_clear(total)
+    k = tracker.train_collect(5, 2)
+    assert k == len(range(2, total, 5))
+    X5, Y5 = tracker.train_get(k)
+    assert np.array_equal(X5, Xr[2::5]) and np.array_equal(Y5, Yr[2::5])
+    k2 = tracker.train_collect(1, 0)
+    assert k2 == total - k and tracker.train_size() == total
+
+
+def test_filter_keeping_everything_is_the_plain_ca(tracker):
+    say("gpu filter with threshold -inf")
+    tracker.mlp_init(5)
+    tracker.set_filter(True, -3.0e38)
+    evs = [itsca.load_events(p)[0] for _, p, _ in CASES]
+    tracker.set_events(evs)
+    tracker.run()
+    item = 0
+    for (name, _, nv), ev in zip(CASES, evs):
+        for v in range(nv):
+            assert itsca.dump_item(tracker, item, ev.vertices[v]) == fixture(name, "stable", v), name
+            item += 1
+    tracker.set_filter(True, 3.0e38)
+    tracker.run()
+    assert all(sum(tracker.counts(i).tracklets) == 0 for i in range(item))
+    tracker.set_filter(False)
+
+
+def _cell_keys(tracker, item):
+    n = tracker.counts(item)
+    ids = [tracker.download(item, itsca.CLUSTERS, l, n.clusters[l], 8)[:, 0] for l in range(itsca.LAYERS)]
+    keys = []
+    for k in range(itsca.CELL_LAYERS):
+        c = tracker.download(item, itsca.CELLS, k, n.cells[k], 10)
+        keys += [(k, a, b, d) for a, b, d in zip(ids[k][c[:, 0]].tolist(), ids[k + 1][c[:, 1]].tolist(),
+                                                 ids[k + 2][c[:, 2]].tolist())]
+    return Counter(keys)
+
+
+@pytest.mark.parametrize("threshold", [-2.0, 0.0, 1.0])
+def test_filtered_roads_are_plain_roads(tracker, threshold):
+    say(f"gpu filter subset threshold={threshold}")
+    evs = itsca.generate_events(2, 7, 1) + itsca.generate_events(5, 8, 1)
+    tracker.set_filter(False)
+    tracker.set_events(evs)
+    tracker.run()
+    items = tracker.num_items()
+    plain = [Counter(itsca.road_tracks(tracker, i)) for i in range(items)]
+    plain_cells = [_cell_keys(tracker, i) for i in range(items)]
+    tracker.set_mlp_params(_random_params(99))
+    tracker.set_filter(True, threshold)
+    tracker.run()
+    kept = 0
+    for i in range(items):
+        learned = Counter(itsca.road_tracks(tracker, i))
+        assert not (learned - plain[i]), i
+        assert not (_cell_keys(tracker, i) - plain_cells[i]), i
+        kept += sum(learned.values())
+    eager = [itsca.road_tracks(tracker, i) for i in range(items)]
+    tracker.set_graph(True)  # the filter inside the captured pipeline
+    for _ in range(3):
+        tracker.run()
+        assert [itsca.road_tracks(tracker, i) for i in range(items)] == eager
+    tracker.set_graph(False)
+    tracker.set_filter(False)
+    say(f"  roads kept {kept} of {sum(sum(p.values()) for p in plain)}")

@@ -69,6 +69,8 @@ struct DevState {
   int* table;  // W*7*401
   // tracklets
   int *trkCnt, *trkOff;  // per sorted cluster
+  int* trkKept;          // per sorted cluster: candidates kept by the learned filter
+  const uint16_t* mlpW;  // learned filter parameters, BF16 bits
   int* trkCand;          // kTrkSlots per sorted cluster
   Seg* trkSeg;           // W*6
   int *tFirst, *tSecond;

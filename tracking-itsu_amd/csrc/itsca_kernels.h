@@ -36,6 +36,8 @@ struct LaunchConfig {
   int persistentBlocks;
   int* planScratch;       // device ints, at least maxSegs
   int maxLayerClusters;   // largest cluster count of any (item, layer)
+  bool filter;            // learned tracklet filter after the tracklet count
+  float threshold;        // its logit threshold
 };
 
 void launchPipeline(const DevState& s, const LaunchConfig& cfg, hipStream_t stream, KernelTimer* timer);

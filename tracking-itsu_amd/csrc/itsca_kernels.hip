@@ -12,6 +12,7 @@
 
 #include "itsca_device.h"
 #include "itsca_kernels.h"
+#include "itsca_mlp_kernels.h"
 
 namespace itsca {
 namespace {
@@ -995,6 +996,10 @@ void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t str
   done("kPlanTracklets", 1);
   hipLaunchKernelGGL(kTrackletsCount<B>, dim3(G), dim3(B), 0, stream, s);
   done("kTrackletsCount", 1);
+  if (cfg.filter) {
+    launchTrackletFilter(s, B, G, cfg.threshold, stream);
+    done("kTrackletsScore", 1);
+  }
   hipLaunchKernelGGL(kScanChunks, dim3(1), dim3(1024), 0, stream, s, (int)kStTracklets, s.W * kTrackletLayers,
                      s.trkSeg, s.tCap, (int)kStatusOverflowTracklets);
   done("kScanChunks", 1);

@@ -1,0 +1,524 @@
+// itsca_mlp.hip -- gfx950 kernels of the learned tracklet filter: features,
+// labels, BF16 MFMA inference (v_mfma_f32_16x16x16_bf16, FP32 accumulation),
+// the FP32 backward pass and AdamW.  All reductions are per-workgroup partials
+// summed in a fixed order, so training is deterministic.
+#include <hip/hip_runtime.h>
+
+#include "itsca_device.h"
+#include "itsca_mlp.h"
+#include "itsca_mlp_kernels.h"
+
+namespace itsca {
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// Orders LDS traffic among the lanes of one wavefront.
+__device__ __forceinline__ void waveSync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ float waveSumXor(float v, int mask) { return v + __shfl_xor(v, mask, 64); }
+
+// Weight operands of one wavefront, loaded once per kernel.  For
+// v_mfma_f32_16x16x16_bf16 (wave64) lane i holds A[i%16][4*(i/16)+r],
+// B[4*(i/16)+r][i%16] and D[4*(i/16)+r][i%16], r = 0..3.  Layer 1 computes
+// H1^T = W1 * X^T, so its D tile is exactly the B operand of layer 2.
+struct MlpWave {
+  bf16x4 a1[2];     // W1 rows 16*ob + i%16, columns 4*(i/16)+r (zero past 10)
+  bf16x4 a2[2][2];  // W2 rows 16*ob + i%16, columns 16*kb + 4*(i/16)+r
+  float b1[2][4], b2[2][4], w3[2][4];  // for the D rows 16*ob + 4*(i/16)+r
+  float b3;
+};
+
+__device__ __forceinline__ MlpWave loadMlp(const uint16_t* w)
+{
+  MlpWave m;
+  const int lane = threadIdx.x & 63, col = lane & 15, grp = lane >> 4;
+  for (int ob = 0; ob < 2; ++ob) {
+    const int row = ob * 16 + col;
+    for (int r = 0; r < 4; ++r) {
+      const int k = grp * 4 + r;
+      m.a1[ob][r] = (short)(k < kMlpIn ? w[kOffW1 + row * kMlpIn + k] : 0);
+      for (int kb = 0; kb < 2; ++kb) m.a2[ob][kb][r] = (short)w[kOffW2 + row * kMlpHidden + kb * 16 + k];
+      const int drow = ob * 16 + grp * 4 + r;
+      m.b1[ob][r] = fromBf16(w[kOffB1 + drow]);
+      m.b2[ob][r] = fromBf16(w[kOffB2 + drow]);
+      m.w3[ob][r] = fromBf16(w[kOffW3 + drow]);
+    }
+  }
+  m.b3 = fromBf16(w[kOffB3]);
+  return m;
+}
+
+// Logit of 16 samples; x is this lane's B operand (features 4*(i/16)+r of
+// sample i%16, BF16).  Every lane returns the logit of sample i%16.
+__device__ __forceinline__ float mlpForward16(const MlpWave& m, bf16x4 x)
+{
+  const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+  bf16x4 h1[2];
+  for (int ob = 0; ob < 2; ++ob) {
+    const f32x4 d = mfma16(m.a1[ob], x, zero);
+    for (int r = 0; r < 4; ++r) {
+      const float v = d[r] + m.b1[ob][r];
+      h1[ob][r] = (short)toBf16(v > 0.0f ? v : 0.0f);
+    }
+  }
+  float part = 0.0f;
+  for (int ob = 0; ob < 2; ++ob) {
+    f32x4 d = mfma16(m.a2[ob][0], h1[0], zero);
+    d = mfma16(m.a2[ob][1], h1[1], d);
+    for (int r = 0; r < 4; ++r) {
+      const float v = d[r] + m.b2[ob][r];
+      part += m.w3[ob][r] * roundBf16(v > 0.0f ? v : 0.0f);
+    }
+  }
+  part = waveSumXor(part, 16);
+  part = waveSumXor(part, 32);
+  return part + m.b3;
+}
+
+// Logits of 64 samples whose BF16 features (16 per sample) sit in
+// `tile` (LDS, [64][16]); lane i gets the logit of sample i.
+__device__ __forceinline__ float mlpForward64(const MlpWave& m, const uint16_t* tile, float* zbuf)
+{
+  const int lane = threadIdx.x & 63, col = lane & 15, grp = lane >> 4;
+  for (int g = 0; g < 4; ++g) {
+    bf16x4 x;
+    for (int r = 0; r < 4; ++r) x[r] = (short)tile[(g * 16 + col) * kMlpInPad + grp * 4 + r];
+    const float z = mlpForward16(m, x);
+    if (grp == 0) zbuf[g * 16 + col] = z;
+  }
+  waveSync();
+  return zbuf[lane];
+}
+
+__device__ __forceinline__ void packFeatures(const float f[kMlpIn], uint16_t* row)
+{
+  for (int k = 0; k < kMlpInPad; ++k) row[k] = k < kMlpIn ? toBf16(f[k]) : 0;
+}
+
+__device__ __forceinline__ void candidateFeatures(const DevState& s, int w, int l, int pi, int pj, float f[kMlpIn])
+{
+  TrackletFeatureInput in;
+  in.xi = s.cx[pi];
+  in.yi = s.cy[pi];
+  in.zi = s.cz[pi];
+  in.ri = s.cr[pi];
+  in.phii = s.cphi[pi];
+  in.xj = s.cx[pj];
+  in.yj = s.cy[pj];
+  in.zj = s.cz[pj];
+  in.rj = s.cr[pj];
+  in.phij = s.cphi[pj];
+  in.vx = s.itemVertex[3 * w];
+  in.vy = s.itemVertex[3 * w + 1];
+  in.vz = s.itemVertex[3 * w + 2];
+  in.layer = l;
+  in.countI = s.trkCnt[pi];
+  in.countJ = l + 1 < kTrackletLayers ? s.trkCnt[pj] : 0;
+  trackletFeatures(in, f);
+}
+
+template <int B>
+__device__ __forceinline__ int ownerOf(const int* exs, int o)
+{
+  int lo = 0, hi = B;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (exs[mid] <= o)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int waveBlockScan(int v, int* scratch, int& total)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nWaves = blockDim.x >> 6;
+  int inc = v;
+  for (int d = 1; d < 64; d <<= 1) {
+    const int t = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += t;
+  }
+  if (lane == 63) scratch[wave] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int k = 0; k < nWaves; ++k) {
+      const int t = scratch[k];
+      scratch[k] = acc;
+      acc += t;
+    }
+    scratch[nWaves] = acc;
+  }
+  __syncthreads();
+  const int ex = inc - v + scratch[wave];
+  total = scratch[nWaves];
+  __syncthreads();
+  return ex;
+}
+
+__device__ __forceinline__ int segmentOf(const int* chunkStart, int S, int chunk)
+{
+  int lo = 0, hi = S;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (chunkStart[mid] <= chunk)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------------------
+// Pipeline filter.  Runs after the tracklet count: every kept candidate slot
+// of a cluster is scored; candidates below the threshold are dropped and the
+// slots compacted in order.  Clusters whose candidates overflowed their slots
+// are kept whole.  Kept counts go to trkKept (trkCnt stays the raw count that
+// the features read) and the chunk sums are rewritten.
+// ---------------------------------------------------------------------------
+template <int B>
+__global__ void __launch_bounds__(B) kTrackletsScore(DevState s, float threshold)
+{
+  __shared__ int scratch[B / 64 + 1];
+  __shared__ int exs[B + 1];
+  __shared__ uint16_t tiles[B / 64][64 * kMlpInPad];
+  __shared__ float zbuf[B / 64][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const MlpWave m = loadMlp(s.mlpW);
+  const int S = s.W * kTrackletLayers;
+  const int* chunkStart = s.chunkStart + kStTracklets * (s.maxSegs + 1);
+  const int nChunks = chunkStart[S];
+  for (int chunk = blockIdx.x; chunk < nChunks; chunk += gridDim.x) {
+    const int sg = segmentOf(chunkStart, S, chunk);
+    const int w = sg / kTrackletLayers, l = sg % kTrackletLayers;
+    const Seg cur = s.clSeg[w * kLayers + l], next = s.clSeg[w * kLayers + l + 1];
+    const int first = (chunk - chunkStart[sg]) * B;
+    const int i = first + threadIdx.x;
+    const int p = cur.base + i;
+    const int raw = i < cur.count ? s.trkCnt[p] : 0;
+    const int scored = raw <= kTrkSlots ? raw : 0;
+    int total;
+    const int ex = waveBlockScan(scored, scratch, total);
+    exs[threadIdx.x] = ex;
+    if (threadIdx.x == 0) exs[B] = total;
+    __syncthreads();
+    for (int tile = wave; tile * 64 < total; tile += B / 64) {
+      const int o = tile * 64 + lane;
+      float f[kMlpIn];
+      int slot = -1;
+      if (o < total) {
+        const int c = ownerOf<B>(exs, o);
+        const int pc = cur.base + first + c;
+        slot = pc * kTrkSlots + (o - exs[c]);
+        candidateFeatures(s, w, l, pc, next.base + s.trkCand[slot], f);
+      } else {
+        for (int k = 0; k < kMlpIn; ++k) f[k] = 0.0f;
+      }
+      packFeatures(f, &tiles[wave][lane * kMlpInPad]);
+      waveSync();
+      const float z = mlpForward64(m, tiles[wave], zbuf[wave]);
+      if (slot >= 0 && !(z >= threshold)) s.trkCand[slot] = -1 - s.trkCand[slot];  // dropped
+      waveSync();
+    }
+    __syncthreads();
+    int kept = raw;
+    if (i < cur.count && raw <= kTrkSlots) {
+      int* slots = s.trkCand + (size_t)p * kTrkSlots;
+      kept = 0;
+      for (int n = 0; n < raw; ++n)
+        if (slots[n] >= 0) slots[kept++] = slots[n];
+    }
+    if (i < cur.count) s.trkKept[p] = kept;
+    int dummy;
+    waveBlockScan(i < cur.count ? kept : 0, scratch, dummy);
+    if (threadIdx.x == 0) s.chunkSum[(size_t)kStTracklets * s.maxChunks + chunk] = dummy;
+  }
+}
+
+// After scoring every layer: the kept counts become the counts the rest of the
+// pipeline reads.  Only the clusters of layers 0..5 (the ones scored) change.
+__global__ void kCommitKept(DevState s)
+{
+  const int S = s.W * kTrackletLayers;
+  for (int sg = blockIdx.y; sg < S; sg += gridDim.y) {
+    const Seg g = s.clSeg[(sg / kTrackletLayers) * kLayers + sg % kTrackletLayers];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.count; i += gridDim.x * blockDim.x)
+      s.trkCnt[g.base + i] = s.trkKept[g.base + i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Training data.  Labels: a tracklet is positive when it is the first or the
+// second tracklet of a cell of some final road.
+// ---------------------------------------------------------------------------
+__global__ void kLabelClear(uint8_t* label, int n)
+{
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) label[t] = 0;
+}
+
+__global__ void kLabelRoads(DevState s, uint8_t* label)
+{
+  for (int w = blockIdx.x; w < s.W; w += gridDim.x) {
+    const Seg roads = s.roadSeg[w];
+    for (int r = threadIdx.x; r < roads.count; r += blockDim.x) {
+      for (int k = 0; k < kCellLayers; ++k) {
+        const int cell = s.rCells[(size_t)(roads.base + r) * kCellLayers + k];
+        if (cell < 0) continue;
+        const int c = s.celSeg[w * kCellLayers + k].base + cell;
+        label[s.trkSeg[w * kTrackletLayers + k].base + s.ct1[c]] = 1;      // same value from every writer
+        label[s.trkSeg[w * kTrackletLayers + k + 1].base + s.ct2[c]] = 1;
+      }
+    }
+  }
+}
+
+// Every stride-th tracklet (global index t = phase + stride * row) becomes row
+// `base + row` of the training set.
+__global__ void kCollect(DevState s, const uint8_t* label, int total, int stride, int phase, uint16_t* outX,
+                         uint8_t* outY, int base, int cap)
+{
+  const int S = s.W * kTrackletLayers;
+  for (int row = blockIdx.x * blockDim.x + threadIdx.x; phase + (long)row * stride < total;
+       row += gridDim.x * blockDim.x) {
+    const int t = phase + row * stride;
+    if (base + row >= cap) break;
+    int lo = 0, hi = S;  // segment of tracklet t
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (s.trkSeg[mid].base <= t)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const int w = lo / kTrackletLayers, l = lo % kTrackletLayers;
+    const int pi = s.clSeg[w * kLayers + l].base + s.tFirst[t];
+    const int pj = s.clSeg[w * kLayers + l + 1].base + s.tSecond[t];
+    float f[kMlpIn];
+    candidateFeatures(s, w, l, pi, pj, f);
+    uint16_t row16[kMlpInPad];
+    packFeatures(f, row16);
+    for (int k = 0; k < kMlpInPad; ++k) outX[(size_t)(base + row) * kMlpInPad + k] = row16[k];
+    outY[base + row] = label[t];
+  }
+}
+
+// Logits of n samples of a training set (MFMA inference).
+__global__ void __launch_bounds__(256) kScoreRows(const uint16_t* w, const uint16_t* X, int n, float* out)
+{
+  __shared__ uint16_t tiles[4][64 * kMlpInPad];
+  __shared__ float zbuf[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const MlpWave m = loadMlp(w);
+  const int waves = gridDim.x * 4;
+  for (int tile = blockIdx.x * 4 + wave; tile * 64 < n; tile += waves) {
+    const int o = tile * 64 + lane;
+    for (int k = 0; k < kMlpInPad; ++k) tiles[wave][lane * kMlpInPad + k] = o < n ? X[(size_t)o * kMlpInPad + k] : 0;
+    waveSync();
+    const float z = mlpForward64(m, tiles[wave], zbuf[wave]);
+    if (o < n) out[o] = z;
+    waveSync();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Training step: FP32 forward (BF16-rounded weights, inputs and activations,
+// as in inference) and backward (the rounding passed straight through), with
+// the weighted binary cross-entropy loss
+//   L = sum_s c_s [ softplus(z_s) - y_s z_s ] / sum_s c_s,  c_s = posWeight if y_s else 1.
+// Samples of a step are rows (a*k + b) mod n for k in [k0, k0+count).  Each
+// workgroup writes its partial gradient (kMlpParams floats) and partial loss
+// and weight sum; kAdamW sums them in workgroup order.
+// ---------------------------------------------------------------------------
+constexpr int kTile = 64;
+constexpr int kGradSlots = kMlpParams + 2;  // gradients, loss, weight
+
+__global__ void __launch_bounds__(256) kTrainGrad(const float* wq, const uint16_t* X, const uint8_t* Y, int n,
+                                                  long a, long b, int k0, int count, float posWeight, float* partial)
+{
+  __shared__ float W[kMlpParams];
+  __shared__ float xs[kTile][kMlpIn];
+  __shared__ float h1[kTile][kMlpHidden], h2[kTile][kMlpHidden], g1[kTile][kMlpHidden], g2[kTile][kMlpHidden];
+  __shared__ float dz[kTile];
+  __shared__ float lossPart[kTile], weightPart[kTile];
+  for (int p = threadIdx.x; p < kMlpParams; p += blockDim.x) W[p] = wq[p];
+  __syncthreads();
+  float acc[(kMlpParams + 255) / 256];
+  for (int q = 0; q < (kMlpParams + 255) / 256; ++q) acc[q] = 0.0f;
+  float lossAcc = 0.0f, weightAcc = 0.0f;
+  const int per = (count + gridDim.x - 1) / gridDim.x;
+  const int begin = blockIdx.x * per, end = min(count, begin + per);
+  for (int t0 = begin; t0 < end; t0 += kTile) {
+    const int sIdx = threadIdx.x;
+    if (sIdx < kTile) {
+      const int k = t0 + sIdx;
+      float loss = 0.0f, weight = 0.0f, d = 0.0f;
+      for (int o = 0; o < kMlpHidden; ++o) h1[sIdx][o] = h2[sIdx][o] = g1[sIdx][o] = g2[sIdx][o] = 0.0f;
+      for (int f = 0; f < kMlpIn; ++f) xs[sIdx][f] = 0.0f;
+      if (k < end) {
+        const long row = (a * (long)(k0 + k) + b) % n;
+        for (int f = 0; f < kMlpIn; ++f) xs[sIdx][f] = fromBf16(X[row * kMlpInPad + f]);
+        for (int o = 0; o < kMlpHidden; ++o) {
+          float v = W[kOffB1 + o];
+          for (int f = 0; f < kMlpIn; ++f) v += W[kOffW1 + o * kMlpIn + f] * xs[sIdx][f];
+          h1[sIdx][o] = roundBf16(v > 0.0f ? v : 0.0f);
+        }
+        float z = W[kOffB3];
+        for (int o = 0; o < kMlpHidden; ++o) {
+          float v = W[kOffB2 + o];
+          for (int q = 0; q < kMlpHidden; ++q) v += W[kOffW2 + o * kMlpHidden + q] * h1[sIdx][q];
+          h2[sIdx][o] = roundBf16(v > 0.0f ? v : 0.0f);
+          z += W[kOffW3 + o] * h2[sIdx][o];
+        }
+        const float y = Y[row] ? 1.0f : 0.0f;
+        weight = Y[row] ? posWeight : 1.0f;
+        const float sp = z > 0.0f ? z + log1pf(expf(-z)) : log1pf(expf(z));  // softplus
+        loss = weight * (sp - y * z);
+        const float prob = 1.0f / (1.0f + expf(-z));
+        d = weight * (prob - y);
+        for (int o = 0; o < kMlpHidden; ++o) g2[sIdx][o] = h2[sIdx][o] > 0.0f ? d * W[kOffW3 + o] : 0.0f;
+        for (int q = 0; q < kMlpHidden; ++q) {
+          float v = 0.0f;
+          for (int o = 0; o < kMlpHidden; ++o) v += g2[sIdx][o] * W[kOffW2 + o * kMlpHidden + q];
+          g1[sIdx][q] = h1[sIdx][q] > 0.0f ? v : 0.0f;
+        }
+      }
+      dz[sIdx] = d;
+      lossPart[sIdx] = loss;
+      weightPart[sIdx] = weight;
+    }
+    __syncthreads();
+    for (int q = 0; q < (kMlpParams + 255) / 256; ++q) {
+      const int p = threadIdx.x + 256 * q;
+      if (p >= kMlpParams) break;
+      float v = 0.0f;
+      if (p < kOffB1) {
+        const int o = p / kMlpIn, f = p % kMlpIn;
+        for (int t = 0; t < kTile; ++t) v += g1[t][o] * xs[t][f];
+      } else if (p < kOffW2) {
+        for (int t = 0; t < kTile; ++t) v += g1[t][p - kOffB1];
+      } else if (p < kOffB2) {
+        const int o = (p - kOffW2) / kMlpHidden, q2 = (p - kOffW2) % kMlpHidden;
+        for (int t = 0; t < kTile; ++t) v += g2[t][o] * h1[t][q2];
+      } else if (p < kOffW3) {
+        for (int t = 0; t < kTile; ++t) v += g2[t][p - kOffB2];
+      } else if (p < kOffB3) {
+        for (int t = 0; t < kTile; ++t) v += dz[t] * h2[t][p - kOffW3];
+      } else {
+        for (int t = 0; t < kTile; ++t) v += dz[t];
+      }
+      acc[q] += v;
+    }
+    if (threadIdx.x == 0)
+      for (int t = 0; t < kTile; ++t) {
+        lossAcc += lossPart[t];
+        weightAcc += weightPart[t];
+      }
+    __syncthreads();
+  }
+  float* out = partial + (size_t)blockIdx.x * kGradSlots;
+  for (int q = 0; q < (kMlpParams + 255) / 256; ++q) {
+    const int p = threadIdx.x + 256 * q;
+    if (p < kMlpParams) out[p] = acc[q];
+  }
+  if (threadIdx.x == 0) {
+    out[kMlpParams] = lossAcc;
+    out[kMlpParams + 1] = weightAcc;
+  }
+}
+
+// Sums the partials in workgroup order, normalises by the total sample
+// weight, and (when lr > 0) applies one AdamW step (decoupled weight decay on
+// the weight matrices, none on biases).  Writes the gradient and the loss.
+__global__ void __launch_bounds__(256) kAdamW(float* master, float* mom1, float* mom2, float* wq, uint16_t* wbf,
+                                              const float* partial, int blocks, int step, float lr, float wd,
+                                              float* gradOut, float* lossOut)
+{
+  __shared__ float total[2];
+  if (threadIdx.x == 0) {
+    float l = 0.0f, wsum = 0.0f;
+    for (int b = 0; b < blocks; ++b) {
+      l += partial[(size_t)b * kGradSlots + kMlpParams];
+      wsum += partial[(size_t)b * kGradSlots + kMlpParams + 1];
+    }
+    total[0] = l;
+    total[1] = wsum > 0.0f ? wsum : 1.0f;
+    if (lossOut) *lossOut = l / total[1];
+  }
+  __syncthreads();
+  const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+  const float c1 = 1.0f - powf(beta1, (float)step), c2 = 1.0f - powf(beta2, (float)step);
+  for (int p = threadIdx.x; p < kMlpParams; p += blockDim.x) {
+    float g = 0.0f;
+    for (int b = 0; b < blocks; ++b) g += partial[(size_t)b * kGradSlots + p];
+    g /= total[1];
+    if (gradOut) gradOut[p] = g;
+    if (lr > 0.0f) {
+      const float m = beta1 * mom1[p] + (1.0f - beta1) * g;
+      const float v = beta2 * mom2[p] + (1.0f - beta2) * g * g;
+      mom1[p] = m;
+      mom2[p] = v;
+      const bool decay = p < kOffB1 || (p >= kOffW2 && p < kOffB2) || (p >= kOffW3 && p < kOffB3);
+      float w = master[p];
+      if (decay) w -= lr * wd * w;
+      w -= lr * (m / c1) / (sqrtf(v / c2) + eps);
+      master[p] = w;
+      wq[p] = roundBf16(w);
+      wbf[p] = toBf16(w);
+    }
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host-side launchers
+// ---------------------------------------------------------------------------
+void launchTrackletFilter(const DevState& s, int block, int blocks, float threshold, hipStream_t stream)
+{
+  switch (block) {
+    case 64: hipLaunchKernelGGL(kTrackletsScore<64>, dim3(blocks), dim3(64), 0, stream, s, threshold); break;
+    case 128: hipLaunchKernelGGL(kTrackletsScore<128>, dim3(blocks), dim3(128), 0, stream, s, threshold); break;
+    default: hipLaunchKernelGGL(kTrackletsScore<256>, dim3(blocks), dim3(256), 0, stream, s, threshold); break;
+  }
+  const int segs = s.W * kTrackletLayers;
+  hipLaunchKernelGGL(kCommitKept, dim3(8, segs < 1 ? 1 : (segs < 4096 ? segs : 4096)), dim3(256), 0, stream, s);
+}
+
+void launchLabelAndCollect(const DevState& s, int totalTracklets, uint8_t* label, int stride, int phase,
+                           uint16_t* outX, uint8_t* outY, int base, int cap, hipStream_t stream)
+{
+  hipLaunchKernelGGL(kLabelClear, dim3(256), dim3(256), 0, stream, label, totalTracklets);
+  hipLaunchKernelGGL(kLabelRoads, dim3(s.W < 1024 ? (s.W > 0 ? s.W : 1) : 1024), dim3(256), 0, stream, s, label);
+  hipLaunchKernelGGL(kCollect, dim3(512), dim3(256), 0, stream, s, label, totalTracklets, stride, phase, outX, outY,
+                     base, cap);
+}
+
+void launchScoreRows(const uint16_t* wbf, const uint16_t* X, int n, float* out, hipStream_t stream)
+{
+  hipLaunchKernelGGL(kScoreRows, dim3(512), dim3(256), 0, stream, wbf, X, n, out);
+}
+
+void launchTrainStep(const MlpBuffers& b, const uint16_t* X, const uint8_t* Y, int n, long a, long off, int k0,
+                     int count, float posWeight, int step, float lr, float wd, float* gradOut, float* lossOut,
+                     hipStream_t stream)
+{
+  hipLaunchKernelGGL(kTrainGrad, dim3(b.gradBlocks), dim3(256), 0, stream, b.wq, X, Y, n, a, off, k0, count,
+                     posWeight, b.partial);
+  hipLaunchKernelGGL(kAdamW, dim3(1), dim3(256), 0, stream, b.master, b.mom1, b.mom2, b.wq, b.wbf, b.partial,
+                     b.gradBlocks, step, lr, wd, gradOut, lossOut);
+}
+
+}  // namespace itsca

@@ -17,6 +17,8 @@
 #include "../../include/itsca.h"
 #include "itsca_device.h"
 #include "itsca_kernels.h"
+#include "itsca_mlp.h"
+#include "itsca_mlp_kernels.h"
 
 using namespace itsca;
 
@@ -72,6 +74,14 @@ struct itsca_ctx {
   KernelTimer timer{};
   bool timerReady = false;
   bool ran = false;
+  // learned tracklet filter
+  bool filterOn = false;
+  float threshold = 0.0f;
+  int mlpStep = 0;
+  int trainN = 0, trainCap = 0;
+  DBuf<float> mlpMaster, mlpMom1, mlpMom2, mlpWq, mlpPartial, mlpGrad, mlpLoss, trainScores;
+  DBuf<uint16_t> mlpWbf, trainX;
+  DBuf<uint8_t> trainY, trkLabel;
   // whole-pipeline HIP graph, captured once per batch shape and buffer set
   bool useGraph = false;
   bool graphValid = false;
@@ -99,7 +109,7 @@ struct itsca_ctx {
   DBuf<int> rid, rmc, evLayerOff, dItemEvent;
   DBuf<Seg> dClSeg, dTrkSeg, dCelSeg, dEdgSeg, dRoadSeg;
   DBuf<float> cx, cy, cz, cphi, cr, sPhi, sR;
-  DBuf<int> cid, cmc, cbin, sBin, sTmp, trkCnt, trkOff, table;
+  DBuf<int> cid, cmc, cbin, sBin, sTmp, trkCnt, trkOff, trkKept, table;
   DBuf<float4> crzp;
   DBuf<int> tFirst, tSecond, cellCnt, cellOff, trkCand, cellCand;
   DBuf<float2> tTP;
@@ -173,6 +183,7 @@ struct itsca_ctx {
     HIP_TRY(sTmp.reserve(SC));
     HIP_TRY(trkCnt.reserve(SC));
     HIP_TRY(trkOff.reserve(SC));
+    HIP_TRY(trkKept.reserve(SC));
     HIP_TRY(crzp.reserve(SC));
     HIP_TRY(trkCand.reserve(SC * kTrkSlots));
     HIP_TRY(table.reserve((size_t)W * kLayers * kTableSize));
@@ -216,6 +227,52 @@ struct itsca_ctx {
     HIP_TRY(dStatus.reserve(kNumStatus));
   }
 
+  void ensureMlp()
+  {
+    if (mlpMaster.p) return;
+    HIP_TRY(mlpMaster.reserve(kMlpParams));
+    HIP_TRY(mlpMom1.reserve(kMlpParams));
+    HIP_TRY(mlpMom2.reserve(kMlpParams));
+    HIP_TRY(mlpWq.reserve(kMlpParams));
+    HIP_TRY(mlpWbf.reserve(kMlpParams));
+    HIP_TRY(mlpGrad.reserve(kMlpParams));
+    HIP_TRY(mlpLoss.reserve(1));
+    HIP_TRY(mlpPartial.reserve((size_t)256 * (kMlpParams + 2)));
+    std::vector<float> zero(kMlpParams, 0.0f);
+    std::vector<uint16_t> zero16(kMlpParams, 0);
+    HIP_TRY(hipMemcpyAsync(mlpMaster.p, zero.data(), kMlpParams * 4, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(mlpMom1.p, zero.data(), kMlpParams * 4, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(mlpMom2.p, zero.data(), kMlpParams * 4, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(mlpWq.p, zero.data(), kMlpParams * 4, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(mlpWbf.p, zero16.data(), kMlpParams * 2, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+  }
+
+  MlpBuffers mlpBuffers() const
+  {
+    return MlpBuffers{mlpMaster.p, mlpMom1.p, mlpMom2.p, mlpWq.p, mlpWbf.p, mlpPartial.p, 256};
+  }
+
+  void setParams(const float* p)
+  {
+    ensureMlp();
+    std::vector<float> q(kMlpParams);
+    std::vector<uint16_t> b(kMlpParams);
+    for (int i = 0; i < kMlpParams; ++i) {
+      q[i] = roundBf16(p[i]);
+      b[i] = toBf16(p[i]);
+    }
+    std::vector<float> zero(kMlpParams, 0.0f);
+    HIP_TRY(hipMemcpyAsync(mlpMaster.p, p, kMlpParams * 4, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(mlpWq.p, q.data(), kMlpParams * 4, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(mlpWbf.p, b.data(), kMlpParams * 2, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(mlpMom1.p, zero.data(), kMlpParams * 4, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(mlpMom2.p, zero.data(), kMlpParams * 4, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    mlpStep = 0;
+    dropGraph();
+  }
+
   DevState state() const
   {
     DevState s{};
@@ -247,6 +304,8 @@ struct itsca_ctx {
     s.table = table.p;
     s.trkCnt = trkCnt.p;
     s.trkOff = trkOff.p;
+    s.trkKept = trkKept.p;
+    s.mlpW = mlpWbf.p;
     s.trkCand = trkCand.p;
     s.trkSeg = dTrkSeg.p;
     s.tFirst = tFirst.p;
@@ -297,7 +356,7 @@ struct itsca_ctx {
   {
     int maxN = 0;
     for (const Seg& g : clSeg) maxN = std::max(maxN, g.count);
-    LaunchConfig cfg{persistentBlocks, planScratch.p, maxN};
+    LaunchConfig cfg{persistentBlocks, planScratch.p, maxN, filterOn, threshold};
     if (profiling && !timerReady) {
       for (auto& e : timer.ev) HIP_TRY(hipEventCreate(&e));
       timerReady = true;
@@ -778,6 +837,209 @@ itsca_status itsca_autotune(itsca_ctx* ctx, int repeats, itsca_tuning* chosen, f
     }
     if (best_ms) *best_ms = best;
     return st;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Learned tracklet filter
+// ---------------------------------------------------------------------------
+itsca_status itsca_mlp_init(itsca_ctx* ctx, uint64_t seed)
+{
+  return guarded(ctx, "itsca_mlp_init", [&]() -> itsca_status {
+    HIP_TRY(hipSetDevice(ctx->device));
+    // He initialisation from a splitmix64 stream (Box-Muller); zero biases.
+    uint64_t st = seed;
+    auto next = [&]() {
+      uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      return z ^ (z >> 31);
+    };
+    auto normal = [&]() {
+      const double u1 = ((next() >> 11) + 1.0) * (1.0 / 9007199254740993.0);
+      const double u2 = (next() >> 11) * (1.0 / 9007199254740992.0);
+      return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    };
+    std::vector<float> p(kMlpParams, 0.0f);
+    for (int i = 0; i < kMlpHidden * kMlpIn; ++i) p[kOffW1 + i] = (float)(normal() * std::sqrt(2.0 / kMlpIn));
+    for (int i = 0; i < kMlpHidden * kMlpHidden; ++i) p[kOffW2 + i] = (float)(normal() * std::sqrt(2.0 / kMlpHidden));
+    for (int i = 0; i < kMlpHidden; ++i) p[kOffW3 + i] = (float)(normal() * std::sqrt(1.0 / kMlpHidden));
+    ctx->setParams(p.data());
+    return ITSCA_OK;
+  });
+}
+
+itsca_status itsca_mlp_set_params(itsca_ctx* ctx, const float* params)
+{
+  if (!ctx || !params) return ITSCA_ERR_ARGUMENT;
+  return guarded(ctx, "itsca_mlp_set_params", [&]() -> itsca_status {
+    HIP_TRY(hipSetDevice(ctx->device));
+    ctx->setParams(params);
+    return ITSCA_OK;
+  });
+}
+
+itsca_status itsca_mlp_get_params(itsca_ctx* ctx, float* params)
+{
+  if (!ctx || !params) return ITSCA_ERR_ARGUMENT;
+  return guarded(ctx, "itsca_mlp_get_params", [&]() -> itsca_status {
+    HIP_TRY(hipSetDevice(ctx->device));
+    ctx->ensureMlp();
+    HIP_TRY(hipMemcpy(params, ctx->mlpMaster.p, kMlpParams * 4, hipMemcpyDeviceToHost));
+    return ITSCA_OK;
+  });
+}
+
+itsca_status itsca_set_filter(itsca_ctx* ctx, int enable, float threshold)
+{
+  if (!ctx) return ITSCA_ERR_ARGUMENT;
+  return guarded(ctx, "itsca_set_filter", [&]() -> itsca_status {
+    if (enable) ctx->ensureMlp();
+    ctx->filterOn = enable != 0;
+    ctx->threshold = threshold;
+    ctx->dropGraph();
+    return ITSCA_OK;
+  });
+}
+
+itsca_status itsca_train_clear(itsca_ctx* ctx, int capacity)
+{
+  if (!ctx || capacity < 0) return ITSCA_ERR_ARGUMENT;
+  return guarded(ctx, "itsca_train_clear", [&]() -> itsca_status {
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(ctx->trainX.reserve((size_t)std::max(capacity, 1) * kMlpInPad));
+    HIP_TRY(ctx->trainY.reserve(std::max(capacity, 1)));
+    ctx->trainCap = capacity;
+    ctx->trainN = 0;
+    return ITSCA_OK;
+  });
+}
+
+itsca_status itsca_train_collect(itsca_ctx* ctx, int stride, int phase, int* added)
+{
+  if (!ctx || stride < 1 || phase < 0) return ITSCA_ERR_ARGUMENT;
+  return guarded(ctx, "itsca_train_collect", [&]() -> itsca_status {
+    if (!ctx->ran) return ctx->fail(ITSCA_ERR_STATE, "itsca_train_collect: no completed run");
+    if (ctx->filterOn) return ctx->fail(ITSCA_ERR_STATE, "itsca_train_collect: labels come from plain runs only");
+    HIP_TRY(hipSetDevice(ctx->device));
+    long total = 0;
+    for (const Seg& g : ctx->trkSeg) total += g.count;
+    HIP_TRY(ctx->trkLabel.reserve(std::max<long>(total, 1)));
+    const long rows = total > phase ? (total - phase + stride - 1) / stride : 0;
+    const int n = (int)std::min<long>(rows, ctx->trainCap - ctx->trainN);
+    if (n > 0)
+      launchLabelAndCollect(ctx->state(), (int)total, ctx->trkLabel.p, stride, phase, ctx->trainX.p, ctx->trainY.p,
+                            ctx->trainN, ctx->trainCap, ctx->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->trainN += std::max(n, 0);
+    if (added) *added = std::max(n, 0);
+    return ITSCA_OK;
+  });
+}
+
+itsca_status itsca_train_set(itsca_ctx* ctx, const uint16_t* features, const uint8_t* labels, int n)
+{
+  if (!ctx || n < 0 || (n > 0 && (!features || !labels))) return ITSCA_ERR_ARGUMENT;
+  return guarded(ctx, "itsca_train_set", [&]() -> itsca_status {
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(ctx->trainX.reserve((size_t)std::max(n, 1) * kMlpInPad));
+    HIP_TRY(ctx->trainY.reserve(std::max(n, 1)));
+    if (n) {
+      HIP_TRY(hipMemcpy(ctx->trainX.p, features, (size_t)n * kMlpInPad * 2, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(ctx->trainY.p, labels, (size_t)n, hipMemcpyHostToDevice));
+    }
+    ctx->trainN = ctx->trainCap = n;
+    return ITSCA_OK;
+  });
+}
+
+int itsca_train_size(const itsca_ctx* ctx) { return ctx ? ctx->trainN : -1; }
+
+itsca_status itsca_train_get(itsca_ctx* ctx, uint16_t* features, uint8_t* labels, int n)
+{
+  if (!ctx || n < 0 || n > ctx->trainN) return ITSCA_ERR_ARGUMENT;
+  return guarded(ctx, "itsca_train_get", [&]() -> itsca_status {
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (n && features) HIP_TRY(hipMemcpy(features, ctx->trainX.p, (size_t)n * kMlpInPad * 2, hipMemcpyDeviceToHost));
+    if (n && labels) HIP_TRY(hipMemcpy(labels, ctx->trainY.p, (size_t)n, hipMemcpyDeviceToHost));
+    return ITSCA_OK;
+  });
+}
+
+itsca_status itsca_train_step(itsca_ctx* ctx, int first, int count, long mult, long offset, float pos_weight, float lr,
+                              float weight_decay, float* loss, float* grad)
+{
+  if (!ctx || first < 0 || count < 1) return ITSCA_ERR_ARGUMENT;
+  return guarded(ctx, "itsca_train_step", [&]() -> itsca_status {
+    if (ctx->trainN < 1) return ctx->fail(ITSCA_ERR_STATE, "itsca_train_step: empty training set");
+    HIP_TRY(hipSetDevice(ctx->device));
+    ctx->ensureMlp();
+    if (lr > 0.0f) ++ctx->mlpStep;
+    launchTrainStep(ctx->mlpBuffers(), ctx->trainX.p, ctx->trainY.p, ctx->trainN, mult, offset, first, count,
+                    pos_weight, std::max(ctx->mlpStep, 1), lr, weight_decay, ctx->mlpGrad.p, ctx->mlpLoss.p,
+                    ctx->stream);
+    HIP_TRY(hipGetLastError());
+    if (loss) HIP_TRY(hipMemcpyAsync(loss, ctx->mlpLoss.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (grad) HIP_TRY(hipMemcpyAsync(grad, ctx->mlpGrad.p, kMlpParams * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (lr > 0.0f) ctx->dropGraph();
+    return ITSCA_OK;
+  });
+}
+
+itsca_status itsca_train_epoch(itsca_ctx* ctx, int batch, float lr, float weight_decay, float pos_weight,
+                               uint64_t seed, float* mean_loss)
+{
+  if (!ctx || batch < 1) return ITSCA_ERR_ARGUMENT;
+  return guarded(ctx, "itsca_train_epoch", [&]() -> itsca_status {
+    const long n = ctx->trainN;
+    if (n < 1) return ctx->fail(ITSCA_ERR_STATE, "itsca_train_epoch: empty training set");
+    HIP_TRY(hipSetDevice(ctx->device));
+    ctx->ensureMlp();
+    // pseudo-shuffle: rows (a*k + b) mod n with a prime a that does not divide n
+    static const long primes[] = {1000003, 999983, 1000033, 1000037, 1000039};
+    long a = 1;
+    for (long pr : primes)
+      if (n % pr != 0) {
+        a = pr % n == 0 ? 1 : pr;
+        break;
+      }
+    const long b = (long)(seed % (uint64_t)n);
+    double sum = 0.0;
+    int steps = 0;
+    std::vector<float> l(1);
+    for (long k0 = 0; k0 < n; k0 += batch) {
+      const int count = (int)std::min<long>(batch, n - k0);
+      ++ctx->mlpStep;
+      launchTrainStep(ctx->mlpBuffers(), ctx->trainX.p, ctx->trainY.p, (int)n, a, b, (int)k0, count, pos_weight,
+                      ctx->mlpStep, lr, weight_decay, nullptr, ctx->mlpLoss.p, ctx->stream);
+      HIP_TRY(hipMemcpyAsync(l.data(), ctx->mlpLoss.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+      HIP_TRY(hipStreamSynchronize(ctx->stream));
+      sum += l[0];
+      ++steps;
+    }
+    HIP_TRY(hipGetLastError());
+    if (mean_loss) *mean_loss = (float)(sum / std::max(steps, 1));
+    ctx->dropGraph();
+    return ITSCA_OK;
+  });
+}
+
+itsca_status itsca_train_scores(itsca_ctx* ctx, float* logits, int n)
+{
+  if (!ctx || !logits || n < 0 || n > ctx->trainN) return ITSCA_ERR_ARGUMENT;
+  return guarded(ctx, "itsca_train_scores", [&]() -> itsca_status {
+    HIP_TRY(hipSetDevice(ctx->device));
+    ctx->ensureMlp();
+    HIP_TRY(ctx->trainScores.reserve(std::max(n, 1)));
+    if (n) {
+      launchScoreRows(ctx->mlpWbf.p, ctx->trainX.p, n, ctx->trainScores.p, ctx->stream);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(logits, ctx->trainScores.p, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return ITSCA_OK;
   });
 }
 

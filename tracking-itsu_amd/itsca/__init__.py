@@ -58,7 +58,12 @@ HIP_SYMBOLS = [
     "itsca_run", "itsca_run_async", "itsca_finish", "itsca_num_items", "itsca_get_counts", "itsca_download",
     "itsca_set_profiling", "itsca_stage_times", "itsca_kernel_times", "itsca_kernel_name", "itsca_set_graph",
     "itsca_set_tuning", "itsca_get_tuning", "itsca_autotune",
+    "itsca_mlp_init", "itsca_mlp_set_params", "itsca_mlp_get_params", "itsca_set_filter", "itsca_train_clear",
+    "itsca_train_collect", "itsca_train_set", "itsca_train_size", "itsca_train_get", "itsca_train_step",
+    "itsca_train_epoch", "itsca_train_scores",
 ]
+MLP_PARAMS = 1441
+MLP_FEATURES = 16
 HOST_SYMBOLS = [
     "itsca_events_generate", "itsca_events_load", "itsca_events_write", "itsca_events_free", "itsca_events_count",
     "itsca_events_num_vertices", "itsca_events_layer_offsets", "itsca_events_copy",
@@ -101,6 +106,20 @@ def bind_hip(lib: ctypes.CDLL) -> ctypes.CDLL:
         lib.itsca_set_tuning.argtypes = [P, ctypes.POINTER(Tuning)]
         lib.itsca_get_tuning.argtypes = [P, ctypes.POINTER(Tuning)]
         lib.itsca_autotune.argtypes = [P, I, ctypes.POINTER(Tuning), F]
+        U16, U8 = ctypes.POINTER(ctypes.c_uint16), ctypes.POINTER(ctypes.c_uint8)
+        lib.itsca_mlp_init.argtypes = [P, ctypes.c_uint64]
+        lib.itsca_mlp_set_params.argtypes = [P, F]
+        lib.itsca_mlp_get_params.argtypes = [P, F]
+        lib.itsca_set_filter.argtypes = [P, I, ctypes.c_float]
+        lib.itsca_train_clear.argtypes = [P, I]
+        lib.itsca_train_collect.argtypes = [P, I, I, ctypes.POINTER(I)]
+        lib.itsca_train_set.argtypes = [P, U16, U8, I]
+        lib.itsca_train_size.argtypes = [P]
+        lib.itsca_train_get.argtypes = [P, U16, U8, I]
+        lib.itsca_train_step.argtypes = [P, I, I, ctypes.c_long, ctypes.c_long, ctypes.c_float, ctypes.c_float,
+                                         ctypes.c_float, F, F]
+        lib.itsca_train_epoch.argtypes = [P, I, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_uint64, F]
+        lib.itsca_train_scores.argtypes = [P, F, I]
         lib.itsca_stage_times.argtypes = [P, F]
         lib.itsca_kernel_times.argtypes = [P, F, I]
         lib.itsca_kernel_name.argtypes = [P, I]
@@ -280,6 +299,70 @@ class GpuTracker:
                     "itsca_autotune")
         return t, ms.value
 
+    # --- learned tracklet filter -------------------------------------------
+    def mlp_init(self, seed: int) -> None:
+        self._check(self._lib.itsca_mlp_init(self.ctx, int(seed)), "itsca_mlp_init")
+
+    def mlp_params(self) -> np.ndarray:
+        p = np.zeros(MLP_PARAMS, dtype=np.float32)
+        self._check(self._lib.itsca_mlp_get_params(self.ctx, _fptr(p)), "itsca_mlp_get_params")
+        return p
+
+    def set_mlp_params(self, params) -> None:
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == MLP_PARAMS
+        self._check(self._lib.itsca_mlp_set_params(self.ctx, _fptr(p)), "itsca_mlp_set_params")
+
+    def set_filter(self, on: bool, threshold: float = 0.0) -> None:
+        self._check(self._lib.itsca_set_filter(self.ctx, 1 if on else 0, float(threshold)), "itsca_set_filter")
+
+    def train_clear(self, capacity: int) -> None:
+        self._check(self._lib.itsca_train_clear(self.ctx, int(capacity)), "itsca_train_clear")
+
+    def train_collect(self, stride: int = 1, phase: int = 0) -> int:
+        added = ctypes.c_int(0)
+        self._check(self._lib.itsca_train_collect(self.ctx, int(stride), int(phase), ctypes.byref(added)),
+                    "itsca_train_collect")
+        return added.value
+
+    def train_set(self, X: np.ndarray, y: np.ndarray) -> None:
+        X = np.ascontiguousarray(X, dtype=np.uint16).reshape(-1, MLP_FEATURES)
+        y = np.ascontiguousarray(y, dtype=np.uint8)
+        self._check(self._lib.itsca_train_set(self.ctx, X.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                                              y.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), len(y)),
+                    "itsca_train_set")
+
+    def train_size(self) -> int:
+        return self._lib.itsca_train_size(self.ctx)
+
+    def train_get(self, n: int):
+        X = np.zeros((max(n, 1), MLP_FEATURES), dtype=np.uint16)
+        y = np.zeros(max(n, 1), dtype=np.uint8)
+        self._check(self._lib.itsca_train_get(self.ctx, X.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                                              y.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), int(n)),
+                    "itsca_train_get")
+        return X[:n], y[:n]
+
+    def train_step(self, first: int, count: int, mult: int = 1, offset: int = 0, pos_weight: float = 1.0,
+                   lr: float = 0.0, weight_decay: float = 0.0):
+        loss = np.zeros(1, dtype=np.float32)
+        grad = np.zeros(MLP_PARAMS, dtype=np.float32)
+        self._check(self._lib.itsca_train_step(self.ctx, int(first), int(count), int(mult), int(offset),
+                                               float(pos_weight), float(lr), float(weight_decay), _fptr(loss),
+                                               _fptr(grad)), "itsca_train_step")
+        return float(loss[0]), grad
+
+    def train_epoch(self, batch: int, lr: float, weight_decay: float, pos_weight: float, seed: int) -> float:
+        loss = np.zeros(1, dtype=np.float32)
+        self._check(self._lib.itsca_train_epoch(self.ctx, int(batch), float(lr), float(weight_decay),
+                                                float(pos_weight), int(seed), _fptr(loss)), "itsca_train_epoch")
+        return float(loss[0])
+
+    def train_scores(self, n: int) -> np.ndarray:
+        z = np.zeros(max(n, 1), dtype=np.float32)
+        self._check(self._lib.itsca_train_scores(self.ctx, _fptr(z), int(n)), "itsca_train_scores")
+        return z[:n]
+
     def set_graph(self, on: bool) -> None:
         self._check(self._lib.itsca_set_graph(self.ctx, 1 if on else 0), "itsca_set_graph")
 
@@ -320,6 +403,24 @@ class GpuTracker:
     def roads(self, item: int) -> np.ndarray:
         """(R, 8) int32: cell per layer 0..4, size, label, fake."""
         return self.download(item, ROADS, 0, self.counts(item).roads, 8)
+
+
+def road_tracks(tracker: GpuTracker, item: int) -> List[tuple]:
+    """Roads of one work item as tuples of the 7 input cluster ids (layer 0..6),
+    in road order: the identity of a track, independent of the cell and
+    tracklet indices of a particular run."""
+    n = tracker.counts(item)
+    if n.roads == 0:
+        return []
+    ids = [tracker.download(item, CLUSTERS, l, n.clusters[l], 8)[:, 0] for l in range(LAYERS)]
+    cells = [tracker.download(item, CELLS, k, n.cells[k], 10) for k in range(CELL_LAYERS)]
+    r = tracker.roads(item)
+    cols = [cells[0][r[:, 0], 0], cells[0][r[:, 0], 1]] + [cells[k][r[:, k], 2] for k in range(CELL_LAYERS)]
+    return list(zip(*[ids[l][cols[l]].tolist() for l in range(LAYERS)]))
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
 def _hex(a: np.ndarray) -> List[str]:

@@ -1,0 +1,196 @@
+"""Training of the learned tracklet filter on the GPU, and its weights file.
+
+The labels come from the plain CA: after a plain run every tracklet is labelled
+1 when it is one of the two tracklets of a cell on a final road, else 0
+(itsca_train_collect).  Training runs the device's own forward/backward and
+AdamW kernels (itsca_train_epoch).  The logit threshold is then tuned on
+separate validation events by running the filtered pipeline and measuring the
+fraction of plain roads it still finds.
+
+    python -m itsca.filter_training --out weights.json
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from collections import Counter
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import MLP_PARAMS, GpuTracker, generate_events, road_tracks
+
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "filter_weights.json")
+
+
+def save_filter(path: str, params: np.ndarray, threshold: float, meta: Dict) -> None:
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    doc = {"format": "itsca tracklet filter 10-32-32-1", "params_f32_bits": ["%08x" % v for v in p.view(np.uint32)],
+           "threshold": float(threshold), "meta": meta}
+    with open(path, "w") as f:
+        json.dump(doc, f, indent=1)
+
+
+def load_filter(path: Optional[str] = None):
+    """(params float32[1441], threshold, meta) of a weights file (default: the
+    file shipped with the package)."""
+    with open(path or DEFAULT_WEIGHTS) as f:
+        doc = json.load(f)
+    p = np.array([int(h, 16) for h in doc["params_f32_bits"]], dtype=np.uint32).view(np.float32)
+    if p.size != MLP_PARAMS:
+        raise ValueError("weights file holds %d parameters, expected %d" % (p.size, MLP_PARAMS))
+    return p, float(doc["threshold"]), doc.get("meta", {})
+
+
+def use_filter(tracker: GpuTracker, path: Optional[str] = None) -> float:
+    """Loads a weights file into the tracker and turns the filter on."""
+    p, thr, _ = load_filter(path)
+    tracker.set_mlp_params(p)
+    tracker.set_filter(True, thr)
+    return thr
+
+
+def _log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def collect(tracker: GpuTracker, preset: int, seeds: Sequence[int], stride: int, capacity: int, batch: int = 25) -> int:
+    """Plain runs over the events of `seeds` (in batches); every stride-th
+    tracklet becomes a training row (the phase rotates between batches)."""
+    tracker.set_filter(False)
+    tracker.train_clear(capacity)
+    seeds = list(seeds)
+    for k in range(0, len(seeds), batch):
+        evs = [e for s in seeds[k:k + batch] for e in generate_events(preset, s, 1)]
+        tracker.set_events(evs)
+        tracker.run()
+        tracker.train_collect(stride, (k // batch) % stride)
+        _log("  collected events %d..%d: %d rows" % (seeds[k], seeds[min(k + batch, len(seeds)) - 1],
+                                                   tracker.train_size()))
+    return tracker.train_size()
+
+
+def train(tracker: GpuTracker, epochs: int, batch: int = 4096, lr: float = 3e-3, lr_min: float = 1e-4,
+          weight_decay: float = 1e-4, pos_weight: float = 4.0, seed: int = 1) -> List[float]:
+    """AdamW epochs over the tracker's training set with a cosine learning rate."""
+    losses = []
+    for e in range(epochs):
+        lr_e = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * e / max(epochs - 1, 1)))
+        losses.append(tracker.train_epoch(batch, lr_e, weight_decay, pos_weight, seed * 1000003 + e))
+        _log("  epoch %d: lr %.2e, mean loss %.5f" % (e, lr_e, losses[-1]))
+    return losses
+
+
+def evaluate(tracker: GpuTracker, events, threshold: Optional[float], plain=None) -> Dict:
+    """Runs `events` plain (unless their plain tracks are given) and with the
+    filter at `threshold`; returns recall of the plain tracks (with
+    multiplicity), tracks outside the plain set, and the tracklet and cell
+    counts of both runs."""
+    tracker.set_events(events)
+    items = None
+    if plain is None:
+        tracker.set_filter(False)
+        tracker.run()
+        items = tracker.num_items()
+        plain = {"tracks": [Counter(road_tracks(tracker, i)) for i in range(items)],
+                 "tracklets": sum(sum(tracker.counts(i).tracklets) for i in range(items)),
+                 "cells": sum(sum(tracker.counts(i).cells) for i in range(items))}
+    tracker.set_filter(True, threshold)
+    tracker.run()
+    items = tracker.num_items()
+    found = extra = total = 0
+    for i in range(items):
+        learned = Counter(road_tracks(tracker, i))
+        found += sum((learned & plain["tracks"][i]).values())
+        extra += sum((learned - plain["tracks"][i]).values())
+        total += sum(plain["tracks"][i].values())
+    tl = sum(sum(tracker.counts(i).tracklets) for i in range(items))
+    ce = sum(sum(tracker.counts(i).cells) for i in range(items))
+    return {"plain": plain, "recall": found / max(total, 1), "plain_tracks": total, "found": found, "extra": extra,
+            "tracklets_kept": tl / max(plain["tracklets"], 1), "cells_kept": ce / max(plain["cells"], 1)}
+
+
+def tune_threshold(tracker: GpuTracker, events, target: float, scores_pos: np.ndarray) -> Dict:
+    """The largest threshold whose road recall on `events` reaches `target`.
+    Candidates are the logits at which 99.9 %, 99.95 %, ... of the positive
+    training rows are kept (from strict to loose)."""
+    qs = [1e-3, 5e-4, 2e-4, 1e-4, 5e-5, 2e-5, 1e-5, 0.0]
+    cands = sorted({float(np.quantile(scores_pos, q)) for q in qs}, reverse=True)
+    plain = None
+    best = None
+    for thr in cands:
+        r = evaluate(tracker, events, thr, plain)
+        plain = r["plain"]
+        _log("  threshold %.4f: recall %.5f (%d/%d), extra %d, tracklets kept %.3f, cells kept %.3f"
+             % (thr, r["recall"], r["found"], r["plain_tracks"], r["extra"], r["tracklets_kept"], r["cells_kept"]))
+        if r["recall"] >= target:
+            best = dict(r, threshold=thr)
+            break
+    if best is None:
+        best = dict(r, threshold=-3.0e38)
+    best.pop("plain")
+    tracker.set_filter(False)
+    return best
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--out", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--preset", type=int, default=2, help="event preset of the training events (2: central)")
+    ap.add_argument("--first-seed", type=int, default=10000)
+    ap.add_argument("--events", type=int, default=500)
+    ap.add_argument("--samples", type=float, default=2e7, help="training rows to keep (sets the stride)")
+    ap.add_argument("--val-first-seed", type=int, default=10500)
+    ap.add_argument("--val-events", type=int, default=32)
+    ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--lr", type=float, default=3e-3)
+    ap.add_argument("--weight-decay", type=float, default=1e-4)
+    ap.add_argument("--pos-weight", type=float, default=4.0)
+    ap.add_argument("--init-seed", type=int, default=7)
+    ap.add_argument("--target-recall", type=float, default=0.999, help="road recall on the validation events")
+    args = ap.parse_args(argv)
+
+    t0 = time.time()
+    tracker = GpuTracker(device=0)
+    probe = generate_events(args.preset, args.first_seed, 1)
+    tracker.set_events(probe)
+    tracker.run()
+    per_event = sum(sum(tracker.counts(i).tracklets) for i in range(tracker.num_items()))
+    stride = max(1, int(round(per_event * args.events / args.samples)))
+    cap = int(per_event * 1.1 * args.events / stride) + 1024
+    n = collect(tracker, args.preset, range(args.first_seed, args.first_seed + args.events), stride, cap)
+    X, Y = tracker.train_get(n)
+    _log("collected %d rows (stride %d, %d tracklets in the first event), %.4f positive, %.1f s"
+         % (n, stride, per_event, Y.mean(), time.time() - t0))
+
+    tracker.mlp_init(args.init_seed)
+    losses = train(tracker, args.epochs, args.batch, args.lr, weight_decay=args.weight_decay,
+                   pos_weight=args.pos_weight, seed=args.init_seed)
+    _log("epoch losses " + " ".join("%.5f" % v for v in losses) + "  (%.1f s)" % (time.time() - t0))
+    z = tracker.train_scores(n)
+    params = tracker.mlp_params()
+
+    val = [e for s in range(args.val_first_seed, args.val_first_seed + args.val_events)
+           for e in generate_events(args.preset, s, 1)]
+    best = tune_threshold(tracker, val, args.target_recall, z[Y > 0])
+    row_recall = float(np.mean(z[Y > 0] >= best["threshold"]))
+    row_kept = float(np.mean(z >= best["threshold"]))
+    meta = {"training_events": "preset %d seeds %d..%d" % (args.preset, args.first_seed,
+                                                              args.first_seed + args.events - 1),
+            "rows": int(n), "stride": stride, "positive_fraction": float(Y.mean()), "epochs": args.epochs,
+            "batch": args.batch, "lr": args.lr, "weight_decay": args.weight_decay, "pos_weight": args.pos_weight,
+            "final_loss": losses[-1], "validation_events": "preset %d seeds %d..%d" % (
+                args.preset, args.val_first_seed, args.val_first_seed + args.val_events - 1),
+            "validation_road_recall": best["recall"], "validation_tracklets_kept": best["tracklets_kept"],
+            "validation_cells_kept": best["cells_kept"], "training_row_recall": row_recall,
+            "training_rows_kept": row_kept}
+    save_filter(args.out, params, best["threshold"], meta)
+    _log(json.dumps(dict(meta, threshold=best["threshold"], seconds=round(time.time() - t0, 1))))
+    tracker.close()
+
+
+if __name__ == "__main__":
+    main()
