@@ -1,11 +1,14 @@
 #!/usr/bin/env python3
-"""Throughput benchmark of the gfx950 plain cellular-automaton track finder.
+"""Throughput benchmark: plain GPU CA vs batched CA with the learned tracklet filter.
 
 Workload: central Pb-Pb-like events (8000 primaries + 2% secondaries + 2% noise,
-about 8.3e3 clusters per layer, one primary vertex), generated from seeds and
-uploaded to HBM before timing.  One step = one plain-CA pass (index tables,
-tracklets, cells, neighbours, levels, roads) over a batch of such events.
-Prints one JSON line.
+about 8.3e3 clusters per layer, one primary vertex), generated from seeds that
+the filter never saw in training, and uploaded to HBM before timing.  One step
+= one pass of the pipeline (index tables, tracklets, [filter], cells,
+neighbours, levels, roads) over a batch of such events, replayed as one HIP
+graph.  Both forms run on the same batch; `value` is the learned form, whose
+roads are checked against the plain ones in the same run (recall, and tracks
+the plain CA does not find).  Prints one JSON line.
 """
 import argparse
 import json
@@ -20,6 +23,7 @@ sys.path.insert(0, os.path.join(ROOT, "tracking-itsu_amd"))
 
 HBM_PEAK_GBS = 8000.0
 STAGES = ["prep+index tables", "tracklets", "cells", "neighbours", "levels+roads"]
+METRIC = "events/s on 1 MI355X, Pb-Pb-like 8k tracks: plain GPU vs batched+learned filter"
 
 
 def bytes_alg(c):
@@ -37,20 +41,22 @@ def bytes_alg(c):
 
 def kernel_bytes(name, c):
     """Algorithmic HBM bytes of one launch of kernel `name` for one work item,
-    from the same record sizes: each kernel reads its inputs once and writes
-    its outputs once."""
+    from the same record sizes (plain-CA counts): each kernel reads its inputs
+    once and writes its outputs once."""
     N, T, C, E, R = list(c.clusters), list(c.tracklets), list(c.cells), list(c.neighbours), c.roads
     if name == "kPrep":  # raw x, y, z, id, mc in; sorted cluster + 401-entry table out
         return sum(n * (20 + 36) for n in N) + 7 * 401 * 4
     if name == "kTrackletsCount":  # both layers' clusters in; one count per cluster out
         return sum((N[l] + N[l + 1]) * 36 + N[l] * 4 for l in range(6))
+    if name == "kTrackletsScore":  # per candidate: its slot and both clusters in; kept count per cluster out
+        return sum(T[l] * (4 + 2 * 36) + N[l] * 4 for l in range(6))
     if name == "kTrackletsFill":  # tracklet records and per-cluster offsets out
         return sum(T[l] * 16 + N[l] * 4 for l in range(6))
     if name == "kCellsCount":  # two tracklet layers and three cluster layers in; one count per tracklet out
         return sum((T[l] + T[l + 1]) * 16 + (N[l] + N[l + 1] + N[l + 2]) * 36 + T[l] * 4 for l in range(5))
     if name == "kCellsFill":  # cell records and per-tracklet offsets out
         return sum(C[l] * 40 + T[l] * 4 for l in range(5))
-    if name in ("kEdgesCount", "kEdgesFill", "kInIndex"):  # two cell layers in; edges and counts out
+    if name in ("kEdgesCount", "kEdgesFill"):  # two cell layers in; edges and counts out
         return sum((C[l] + C[l + 1]) * 40 + E[l] * 4 + C[l + 1] * 4 for l in range(4))
     if name == "kLevels":
         return sum(C) * 4 + sum(E) * 4
@@ -66,15 +72,52 @@ def cpu_baseline(events_path, n_events):
     if os.path.exists(exe):
         out = subprocess.run([exe, "time", events_path, "1"], check=True, capture_output=True, text=True).stdout
         r = json.loads(out.strip().splitlines()[-1])
-        return {"value": r["events"] / r["seconds"], "unit": "events/s", "cores": 1, "kind": "reference",
-                "sample": "%d central events (seeds 2..%d), single pass after one warm-up event" % (n_events, n_events + 1)}
+        return {"value": round(r["events"] / r["seconds"], 3), "unit": "events/s", "cores": 1, "kind": "reference",
+                "sample": "%d central events (seeds 2..%d), single pass after one warm-up event, %.1f s"
+                          % (n_events, n_events + 1, r["seconds"])}
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle  # noqa: E402
     import itsca  # noqa: E402
     evs = itsca.load_events(events_path)
     s = pyoracle.time_events(evs, 1, 1)
-    return {"value": len(evs) / s, "unit": "events/s", "cores": 1, "kind": "port",
+    return {"value": round(len(evs) / s, 3), "unit": "events/s", "cores": 1, "kind": "port",
             "sample": "%d central events, single pass" % len(evs)}
+
+
+def time_form(tracker, steps, warmup):
+    """Events are already resident; the pipeline is replayed as a HIP graph."""
+    tracker.set_graph(False)
+    tracker.run()  # sizes the buffers for this form
+    tracker.set_graph(True)
+    for _ in range(max(warmup, 1)):
+        tracker.run_async()
+    tracker.finish()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tracker.run_async()
+    tracker.finish()
+    elapsed = time.perf_counter() - t0
+    tracker.set_graph(False)
+    return elapsed
+
+
+def kernel_profile(tracker, runs=5):
+    """Mean per-kernel HIP-event times (ms) over `runs` profiled runs, recorded
+    on the library's stream (the stream every kernel is launched on); kernels
+    launched several times per run are summed."""
+    tracker.set_profiling(True)
+    kernel_ms, launches = {}, {}
+    stage_ms = [0.0] * 5
+    for _ in range(runs):
+        tracker.run()
+        stage_ms = [a + b / runs for a, b in zip(stage_ms, tracker.stage_times().tolist())]
+        seen = {}
+        for name, ms in tracker.kernel_times():
+            kernel_ms[name] = kernel_ms.get(name, 0.0) + ms / runs
+            seen[name] = seen.get(name, 0) + 1
+        launches = seen
+    tracker.set_profiling(False)
+    return kernel_ms, launches, stage_ms
 
 
 def main():
@@ -82,9 +125,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=32, help="central events per step")
-    ap.add_argument("--cpu-events", type=int, default=40, help="events in the CPU-baseline sample")
+    ap.add_argument("--batch", type=int, default=64, help="central events per step")
+    ap.add_argument("--cpu-events", type=int, default=200, help="events in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--weights", default=None, help="learned filter weights (default: the shipped file)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every (event, vertex) is independent, so N GPUs would each run "
@@ -92,52 +136,45 @@ def main():
         sys.exit(2)
 
     import itsca
+    from itsca import filter_training as ft
 
     t_gen = time.time()
     events = itsca.generate_events(2, 2, args.batch)
     tracker = itsca.GpuTracker(device=0)
     tracker.set_events(events)
+    params, threshold, meta = ft.load_filter(args.weights)
+    tracker.set_mlp_params(params)
     print("# generated and uploaded %d events in %.1f s" % (len(events), time.time() - t_gen), file=sys.stderr,
           flush=True)
 
-    tracker.run()  # sizes the buffers
-    for _ in range(args.warmup):
-        tracker.run_async()
-    tracker.finish()
+    tracker.set_filter(False)
+    plain_s = time_form(tracker, args.steps, args.warmup)
+    counts = [tracker.counts(i) for i in range(tracker.num_items())]  # plain-CA counts
+    plain_kernels, _, plain_stages = kernel_profile(tracker)
+    print("# plain: %.1f events/s" % (args.batch * args.steps / plain_s), file=sys.stderr, flush=True)
 
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        tracker.run_async()
-    tracker.finish()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = 1e3 * elapsed / args.steps
-    events_per_s = args.batch * args.steps / elapsed
+    tracker.set_filter(True, threshold)
+    learned_s = time_form(tracker, args.steps, args.warmup)
+    kernel_ms, launches, stage_ms = kernel_profile(tracker)
+    print("# learned: %.1f events/s" % (args.batch * args.steps / learned_s), file=sys.stderr, flush=True)
 
-    # algorithmic bytes per step and per-stage device time (separate profiled run)
-    counts = [tracker.counts(i) for i in range(tracker.num_items())]
+    quality = ft.evaluate(tracker, events, threshold)  # plain and filtered roads of the same batch
+    quality.pop("plain")
+    tracker.set_filter(False)
+
+    ms_per_step = 1e3 * learned_s / args.steps
+    events_per_s = args.batch * args.steps / learned_s
+    plain_eps = args.batch * args.steps / plain_s
     balg = sum(bytes_alg(c) for c in counts)
-    tracker.set_profiling(True)
-    prof_runs = 5
-    stage_ms = [0.0] * 5
-    kernel_ms = {}
-    for _ in range(prof_runs):
-        tracker.run()
-        stage_ms = [a + b for a, b in zip(stage_ms, tracker.stage_times().tolist())]
-        for name, ms in tracker.kernel_times():
-            kernel_ms[name] = kernel_ms.get(name, 0.0) + ms / prof_runs
-    stage_ms = [v / prof_runs for v in stage_ms]
-    tracker.set_profiling(False)
-    c0 = tracker.counts(0)
-    dominant = max(range(5), key=lambda k: stage_ms[k])
-    # roofline of the dominant kernel: its algorithmic bytes per launch (one
-    # launch covers the whole batch) over its HIP-event time on the stream
+    # roofline of the learned form's dominant kernel: algorithmic bytes of one
+    # launch (one launch covers the whole batch) over its mean HIP-event time
     top = max(kernel_ms, key=kernel_ms.get)
     top_bytes = sum(kernel_bytes(top, c) for c in counts)
-    launches = 4 if top == "kLevels" else 1
-    achieved = top_bytes / (kernel_ms[top] / launches * 1e-3) / 1e9
+    top_ms = kernel_ms[top] / launches.get(top, 1)
+    achieved = top_bytes / (top_ms * 1e-3) / 1e9
 
     line = {
-        "metric": "events/s on 1 MI355X, Pb-Pb-like 8k tracks: plain GPU vs batched+learned filter",
+        "metric": METRIC,
         "value": round(events_per_s, 2),
         "unit": "events/s",
         "n_gpus": 1,
@@ -149,18 +186,30 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "plain CA, central Pb-Pb-like events (8000 tracks, ~8.3e3 clusters/layer), "
-                               "%d events per step, inputs resident in HBM" % args.batch,
-                   "events_per_step": args.batch, "form": "plain GPU (batched launch)"},
+        "config": {"workload": "central Pb-Pb-like events (8000 tracks, ~8.3e3 clusters/layer, seeds 2..%d), "
+                               "%d events per step, inputs resident in HBM, pipeline replayed as one HIP graph"
+                               % (args.batch + 1, args.batch),
+                   "events_per_step": args.batch,
+                   "form": "batched CA + learned tracklet filter (10-32-32-1, BF16 MFMA, FP32 accumulate)",
+                   "filter_threshold": threshold,
+                   "filter_training": meta.get("training_events")},
+        "plain": {"value": round(plain_eps, 2), "unit": "events/s", "ms_per_step": round(1e3 * plain_s / args.steps, 4),
+                  "kernel_ms": {k: round(v, 4) for k, v in sorted(plain_kernels.items(), key=lambda kv: -kv[1])},
+                  "stage_ms": {STAGES[k]: round(plain_stages[k], 4) for k in range(5)}},
+        "learned_vs_plain": {"speedup": round(events_per_s / plain_eps, 4),
+                             "track_recall": round(quality["recall"], 6), "plain_tracks": quality["plain_tracks"],
+                             "tracks_found": quality["found"], "tracks_outside_plain": quality["extra"],
+                             "tracklets_kept": round(quality["tracklets_kept"], 4),
+                             "cells_kept": round(quality["cells_kept"], 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": top, "kernel_ms": round(kernel_ms[top] / launches, 4), "kernel_bytes_alg": top_bytes,
+                     "kernel": top, "kernel_ms": round(top_ms, 4), "kernel_bytes_alg": top_bytes,
                      "pipeline_bytes_alg_per_step": balg,
                      "pipeline_GBps": round(balg / (ms_per_step * 1e-3) / 1e9, 2)},
-        "stage_ms": {STAGES[k]: round(stage_ms[k], 4) for k in range(5)},
         "kernel_ms": {k: round(v, 4) for k, v in sorted(kernel_ms.items(), key=lambda kv: -kv[1])},
-        "dominant_stage": STAGES[dominant],
-        "counts_event0": {"tracklets": list(c0.tracklets), "cells": list(c0.cells), "roads": c0.roads},
+        "stage_ms": {STAGES[k]: round(stage_ms[k], 4) for k in range(5)},
+        "counts_event0": {"tracklets": list(counts[0].tracklets), "cells": list(counts[0].cells),
+                          "roads": counts[0].roads},
     }
     if not args.no_cpu:
         with tempfile.TemporaryDirectory() as tmp:

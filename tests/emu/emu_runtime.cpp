@@ -21,6 +21,7 @@ struct Block {
   std::vector<float> shflF;
   std::vector<emu_bf16x4> ma, mb;
   std::vector<emu_f32x4> mc;
+  std::vector<emu_bf16x8> ma8, mb8;
 };
 thread_local Block* cur = nullptr;
 }  // namespace
@@ -49,6 +50,20 @@ float shflXor(float v, int mask, int width)
   const float r = (src < 64 && lane / width == src / width) ? cur->shflF[wave * 64 + src] : v;
   b.arrive_and_wait();
   return r;
+}
+
+unsigned long long ballot(bool pred)
+{
+  const int wave = (int)(tIdx.x / 64);
+  std::barrier<>& b = *cur->waves[wave];
+  cur->shfl[tIdx.x] = pred ? 1 : 0;
+  b.arrive_and_wait();
+  unsigned long long m = 0;
+  const int lanes = (int)std::min<unsigned>(64u, bDim.x - 64u * wave);
+  for (int l = 0; l < lanes; ++l)
+    if (cur->shfl[wave * 64 + l]) m |= 1ull << l;
+  b.arrive_and_wait();
+  return m;
 }
 
 void waveBarrier() { cur->waves[tIdx.x / 64]->arrive_and_wait(); }
@@ -81,10 +96,31 @@ emu_f32x4 mfma16x16x16bf16(emu_bf16x4 a, emu_bf16x4 b, emu_f32x4 c)
   return d;
 }
 
+emu_f32x4 mfma16x16x32bf16(emu_bf16x8 a, emu_bf16x8 b, emu_f32x4 c)
+{
+  const int lane = (int)(tIdx.x % 64), wave = (int)(tIdx.x / 64);
+  std::barrier<>& bar = *cur->waves[wave];
+  cur->ma8[tIdx.x] = a;
+  cur->mb8[tIdx.x] = b;
+  bar.arrive_and_wait();
+  // A[m][k]: lane m + 16*(k/8), element k%8; B[k][n]: lane n + 16*(k/8), element k%8
+  emu_f32x4 d;
+  const int col = lane % 16, grp = lane / 16;
+  for (int r = 0; r < 4; ++r) {
+    const int row = grp * 4 + r;
+    float acc = c[r];
+    for (int k = 0; k < 32; ++k)
+      acc += (float)cur->ma8[wave * 64 + row + 16 * (k / 8)][k % 8] * (float)cur->mb8[wave * 64 + col + 16 * (k / 8)][k % 8];
+    d[r] = acc;
+  }
+  bar.arrive_and_wait();
+  return d;
+}
+
 void launch(dim3 grid, dim3 block, const std::function<void()>& body)
 {
   const unsigned nt = block.x;
-  for (unsigned b = 0; b < grid.x; ++b) {
+  for (unsigned b = 0; b < grid.x * grid.y * grid.z; ++b) {
     Block blk;
     blk.all = std::make_unique<std::barrier<>>(nt);
     const unsigned nw = (nt + 63) / 64;
@@ -94,12 +130,16 @@ void launch(dim3 grid, dim3 block, const std::function<void()>& body)
     blk.ma.assign(nt, emu_bf16x4{0, 0, 0, 0});
     blk.mb.assign(nt, emu_bf16x4{0, 0, 0, 0});
     blk.mc.assign(nt, emu_f32x4{0, 0, 0, 0});
+    blk.ma8.assign(nt, emu_bf16x8{});
+    blk.mb8.assign(nt, emu_bf16x8{});
     std::vector<std::thread> th;
     th.reserve(nt);
     for (unsigned t = 0; t < nt; ++t)
       th.emplace_back([&, t, b]() {
         tIdx.x = t;
-        bIdx.x = b;
+        bIdx.x = b % grid.x;
+        bIdx.y = (b / grid.x) % grid.y;
+        bIdx.z = b / (grid.x * grid.y);
         bDim = block;
         gDim = grid;
         cur = &blk;

@@ -33,6 +33,7 @@ struct uint3e {
 
 typedef short emu_bf16x4 __attribute__((ext_vector_type(4)));
 typedef float emu_f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 emu_bf16x8 __attribute__((ext_vector_type(8)));
 
 namespace emu {
 extern thread_local uint3e tIdx, bIdx;
@@ -40,10 +41,14 @@ extern thread_local dim3 bDim, gDim;
 void syncthreads();
 int shflUp(int v, unsigned d, int width);
 float shflXor(float v, int mask, int width);
+unsigned long long ballot(bool pred);
 void waveBarrier();
 // v_mfma_f32_16x16x16_bf16 as documented for CDNA wave64: lane i holds
 // A[i%16][4*(i/16)+r], B[4*(i/16)+r][i%16] and C/D[4*(i/16)+r][i%16].
 emu_f32x4 mfma16x16x16bf16(emu_bf16x4 a, emu_bf16x4 b, emu_f32x4 c);
+// v_mfma_f32_16x16x32_bf16 (gfx950): lane i holds A[i%16][8*(i/16)+r] and
+// B[8*(i/16)+r][i%16], r = 0..7; C/D as for 16x16x16.
+emu_f32x4 mfma16x16x32bf16(emu_bf16x8 a, emu_bf16x8 b, emu_f32x4 c);
 void launch(dim3 grid, dim3 block, const std::function<void()>& body);
 }  // namespace emu
 
@@ -56,9 +61,12 @@ inline void __syncthreads() { emu::syncthreads(); }
 inline void __threadfence_block() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 inline int __shfl_up(int v, unsigned d, int width = 64) { return emu::shflUp(v, d, width); }
 inline float __shfl_xor(float v, int mask, int width = 64) { return emu::shflXor(v, mask, width); }
+inline unsigned long long __ballot(int pred) { return emu::ballot(pred != 0); }
+inline int __popc(unsigned v) { return __builtin_popcount(v); }
 
 // Wave-level builtins used by the MFMA code.
 #define __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, cbsz, abid, blgp) emu::mfma16x16x16bf16(a, b, c)
+#define __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, cbsz, abid, blgp) emu::mfma16x16x32bf16(a, b, c)
 #define __builtin_amdgcn_fence(order, scope) __atomic_thread_fence(__ATOMIC_SEQ_CST)
 #define __builtin_amdgcn_wave_barrier() emu::waveBarrier()
 inline float log1pf(float x) { return std::log1p(x); }
@@ -74,7 +82,15 @@ struct float2 {
 struct float4 {
   float x, y, z, w;
 };
+struct uint2 {
+  unsigned x, y;
+};
+struct uint4 {
+  unsigned x, y, z, w;
+};
 inline float2 make_float2(float a, float b) { return float2{a, b}; }
+inline uint2 make_uint2(unsigned a, unsigned b) { return uint2{a, b}; }
+inline uint4 make_uint4(unsigned a, unsigned b, unsigned c, unsigned d) { return uint4{a, b, c, d}; }
 inline float4 make_float4(float a, float b, float c, float d) { return float4{a, b, c, d}; }
 
 // --- runtime API subset --------------------------------------------------

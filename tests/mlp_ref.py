@@ -119,8 +119,7 @@ def tracklet_features(ci, cj, vertex, layer, count_i, count_j):
         ax, ay, bx, by = ci["x"] - vx, ci["y"] - vy, cj["x"] - vx, cj["y"] - vy
         cross = ax * by - ay * bx
         dx, dy = bx - ax, by - ay
-        la, lb, lc = np.sqrt(ax * ax + ay * ay), np.sqrt(bx * bx + by * by), np.sqrt(dx * dx + dy * dy)
-        den = la * lb * lc
+        den = np.sqrt((ax * ax + ay * ay) * (bx * bx + by * by) * (dx * dx + dy * dy))
         curv = np.where(den > 0, f32(2) * cross / np.where(den > 0, den, f32(1)), f32(0))
         pointing = (ci["z"] - vz) / ci["r"]
         n = len(tanL)

@@ -221,3 +221,41 @@ def test_filtered_roads_are_plain_roads(tracker, threshold):
     tracker.set_graph(False)
     tracker.set_filter(False)
     say(f"  roads kept {kept} of {sum(sum(p.values()) for p in plain)}")
+
+
+@pytest.mark.parametrize("quantile", [0.3, 0.9])
+def test_filter_keeps_exactly_the_tracklets_scored_above_threshold(tracker, quantile):
+    """The in-pipeline scoring sees the same features and runs the same MFMA
+    network as the training rows, so the filtered tracklet list is the plain
+    list restricted to rows with logit >= threshold, in the same order
+    (clusters whose candidates overflowed the 32 slots are kept whole)."""
+    say(f"gpu filter keeps exactly the rows above the {quantile} quantile")
+    evs = itsca.generate_events(2, 31, 1) + itsca.generate_events(5, 32, 1)
+    tracker.set_filter(False)
+    tracker.set_events(evs)
+    tracker.run()
+    items = tracker.num_items()
+    plain = [[tracker.download(i, itsca.TRACKLETS, l, tracker.counts(i).tracklets[l], 4)[:, :2]
+              for l in range(itsca.TRACKLET_LAYERS)] for i in range(items)]
+    total = sum(len(t) for it in plain for t in it)
+    tracker.set_mlp_params(_random_params(5))
+    tracker.train_clear(total)
+    assert tracker.train_collect(1, 0) == total
+    z = tracker.train_scores(total)
+    threshold = float(np.quantile(z, quantile))
+    tracker.set_filter(True, threshold)
+    tracker.run()
+    k = 0
+    dropped = 0
+    for i in range(items):
+        n = tracker.counts(i)
+        for l in range(itsca.TRACKLET_LAYERS):
+            t = plain[i][l]
+            raw = np.bincount(t[:, 0], minlength=n.clusters[l]) if len(t) else np.zeros(n.clusters[l], int)
+            keep = (z[k:k + len(t)] >= threshold) | (raw[t[:, 0]] > 32)
+            k += len(t)
+            got = tracker.download(i, itsca.TRACKLETS, l, n.tracklets[l], 4)[:, :2]
+            assert np.array_equal(got, t[keep]), (i, l)
+            dropped += int((~keep).sum())
+    assert k == total and 0 < dropped < total
+    tracker.set_filter(False)

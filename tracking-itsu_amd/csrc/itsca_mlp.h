@@ -54,8 +54,7 @@ ITSCA_HD void trackletFeatures(const TrackletFeatureInput& in, float f[kMlpIn])
   const float ax = in.xi - in.vx, ay = in.yi - in.vy, bx = in.xj - in.vx, by = in.yj - in.vy;
   const float cross = ax * by - ay * bx;
   const float dx = bx - ax, dy = by - ay;
-  const float la = sqrtf(ax * ax + ay * ay), lb = sqrtf(bx * bx + by * by), lc = sqrtf(dx * dx + dy * dy);
-  const float denom = la * lb * lc;
+  const float denom = sqrtf((ax * ax + ay * ay) * (bx * bx + by * by) * (dx * dx + dy * dy));
   const float curvature = denom > 0.0f ? 2.0f * cross / denom : 0.0f;  // circle through vertex, i, j
   const float pointing = (in.zi - in.vz) / in.ri;
   f[0] = (float)in.layer * 0.2f;

@@ -12,11 +12,23 @@ namespace itsca {
 namespace {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// v_mfma_f32_16x16x16_bf16 (wave64): lane i holds A[i%16][4*(i/16)+r],
+// B[4*(i/16)+r][i%16] and C/D[4*(i/16)+r][i%16], r = 0..3.
 __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c)
 {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// v_mfma_f32_16x16x32_bf16 (gfx950, wave64): lane i holds A[i%16][8*(i/16)+r]
+// and B[8*(i/16)+r][i%16], r = 0..7; C/D as above.
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
 // Orders LDS traffic among the lanes of one wavefront.
@@ -27,120 +39,159 @@ __device__ __forceinline__ void waveSync()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ float waveSumXor(float v, int mask) { return v + __shfl_xor(v, mask, 64); }
+// Two FP32 values as packed BF16, round to nearest even (one
+// v_cvt_pk_bf16_f32 on gfx950).
+__device__ __forceinline__ uint32_t packBf16x2(float lo, float hi)
+{
+  const f32x2 v = {lo, hi};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+}
 
-// Weight operands of one wavefront, loaded once per kernel.  For
-// v_mfma_f32_16x16x16_bf16 (wave64) lane i holds A[i%16][4*(i/16)+r],
-// B[4*(i/16)+r][i%16] and D[4*(i/16)+r][i%16], r = 0..3.  Layer 1 computes
-// H1^T = W1 * X^T, so its D tile is exactly the B operand of layer 2.
+// ReLU of an MFMA result tile, packed as four BF16.  Rounding to nearest even
+// keeps the sign, so ReLU is applied after the conversion: a BF16 value read as
+// a 16-bit integer is negative exactly when the value is negative (or -0), so
+// one v_pk_max_i16 against 0 clamps two values.
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t reluPair(uint32_t packed)
+{
+  const i16x2 v = __builtin_bit_cast(i16x2, packed);
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(v, i16x2{0, 0}));
+}
+__device__ __forceinline__ uint2 reluBf16(f32x4 d)
+{
+  return make_uint2(reluPair(packBf16x2(d[0], d[1])), reluPair(packBf16x2(d[2], d[3])));
+}
+
+__device__ __forceinline__ bf16x8 concat8(uint2 lo, uint2 hi)
+{
+  return __builtin_bit_cast(bf16x8, make_uint4(lo.x, lo.y, hi.x, hi.y));
+}
+
+// Hidden unit that K index k of a 32-wide layer input stands for.  Layer 1
+// (two 16x16x16 MFMAs, one per block of 16 units) leaves units 4*(i/16)+r and
+// 16+4*(i/16)+r in lane i; read as the B operand of a 16x16x32 MFMA these are
+// K indices 8*(i/16)+r and 8*(i/16)+4+r.  Weights are loaded in this order, so
+// no data moves between layers.
+__device__ __forceinline__ int hiddenUnit(int k) { return ((k & 4) ? 16 : 0) + 4 * (k >> 3) + (k & 3); }
+
+// Weight operands of one wavefront, loaded once per kernel.  Layer 1 computes
+// H1^T = W1 * X^T (K = 16 features, 10 used); layers 2 and 3 take the previous
+// D tiles as B operands.  Biases are the accumulator inputs.  The last layer
+// (32 -> 1) holds w3 in every row of A, so every row of D is the logit row and
+// lane i receives the logit of sample i%16 without a cross-lane reduction.
 struct MlpWave {
-  bf16x4 a1[2];     // W1 rows 16*ob + i%16, columns 4*(i/16)+r (zero past 10)
-  bf16x4 a2[2][2];  // W2 rows 16*ob + i%16, columns 16*kb + 4*(i/16)+r
-  float b1[2][4], b2[2][4], w3[2][4];  // for the D rows 16*ob + 4*(i/16)+r
-  float b3;
+  bf16x4 a1[2];  // W1 rows 16*ob + i%16, features 4*(i/16)+r (zero past 10)
+  bf16x8 a2[2];  // W2 rows 16*ob + i%16, units hiddenUnit(8*(i/16)+r)
+  bf16x8 a3;     // w3[hiddenUnit(8*(i/16)+r)], the same for every row
+  f32x4 c1[2], c2[2], c3;  // b1, b2 of the D rows 16*ob + 4*(i/16)+r; b3
 };
 
 __device__ __forceinline__ MlpWave loadMlp(const uint16_t* w)
 {
   MlpWave m;
   const int lane = threadIdx.x & 63, col = lane & 15, grp = lane >> 4;
+  uint16_t a2[2][8], a3[8];
   for (int ob = 0; ob < 2; ++ob) {
     const int row = ob * 16 + col;
     for (int r = 0; r < 4; ++r) {
       const int k = grp * 4 + r;
       m.a1[ob][r] = (short)(k < kMlpIn ? w[kOffW1 + row * kMlpIn + k] : 0);
-      for (int kb = 0; kb < 2; ++kb) m.a2[ob][kb][r] = (short)w[kOffW2 + row * kMlpHidden + kb * 16 + k];
       const int drow = ob * 16 + grp * 4 + r;
-      m.b1[ob][r] = fromBf16(w[kOffB1 + drow]);
-      m.b2[ob][r] = fromBf16(w[kOffB2 + drow]);
-      m.w3[ob][r] = fromBf16(w[kOffW3 + drow]);
+      m.c1[ob][r] = fromBf16(w[kOffB1 + drow]);
+      m.c2[ob][r] = fromBf16(w[kOffB2 + drow]);
     }
+    for (int r = 0; r < 8; ++r) a2[ob][r] = w[kOffW2 + row * kMlpHidden + hiddenUnit(grp * 8 + r)];
   }
-  m.b3 = fromBf16(w[kOffB3]);
+  for (int r = 0; r < 8; ++r) a3[r] = w[kOffW3 + hiddenUnit(grp * 8 + r)];
+  for (int ob = 0; ob < 2; ++ob)
+    m.a2[ob] = __builtin_bit_cast(bf16x8, make_uint4(a2[ob][0] | (uint32_t)a2[ob][1] << 16,
+                                                     a2[ob][2] | (uint32_t)a2[ob][3] << 16,
+                                                     a2[ob][4] | (uint32_t)a2[ob][5] << 16,
+                                                     a2[ob][6] | (uint32_t)a2[ob][7] << 16));
+  m.a3 = __builtin_bit_cast(bf16x8, make_uint4(a3[0] | (uint32_t)a3[1] << 16, a3[2] | (uint32_t)a3[3] << 16,
+                                               a3[4] | (uint32_t)a3[5] << 16, a3[6] | (uint32_t)a3[7] << 16));
+  const float b3 = fromBf16(w[kOffB3]);
+  m.c3 = f32x4{b3, b3, b3, b3};
   return m;
 }
 
-// Logit of 16 samples; x is this lane's B operand (features 4*(i/16)+r of
-// sample i%16, BF16).  Every lane returns the logit of sample i%16.
-__device__ __forceinline__ float mlpForward16(const MlpWave& m, bf16x4 x)
-{
-  const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
-  bf16x4 h1[2];
-  for (int ob = 0; ob < 2; ++ob) {
-    const f32x4 d = mfma16(m.a1[ob], x, zero);
-    for (int r = 0; r < 4; ++r) {
-      const float v = d[r] + m.b1[ob][r];
-      h1[ob][r] = (short)toBf16(v > 0.0f ? v : 0.0f);
-    }
-  }
-  float part = 0.0f;
-  for (int ob = 0; ob < 2; ++ob) {
-    f32x4 d = mfma16(m.a2[ob][0], h1[0], zero);
-    d = mfma16(m.a2[ob][1], h1[1], d);
-    for (int r = 0; r < 4; ++r) {
-      const float v = d[r] + m.b2[ob][r];
-      part += m.w3[ob][r] * roundBf16(v > 0.0f ? v : 0.0f);
-    }
-  }
-  part = waveSumXor(part, 16);
-  part = waveSumXor(part, 32);
-  return part + m.b3;
-}
-
-// Logits of 64 samples whose BF16 features (16 per sample) sit in
-// `tile` (LDS, [64][16]); lane i gets the logit of sample i.
-__device__ __forceinline__ float mlpForward64(const MlpWave& m, const uint16_t* tile, float* zbuf)
+// Logits of 64 samples whose BF16 features (16 per sample, 32 bytes) sit in
+// `tile` (LDS, [64][16]); lane i gets the logit of sample i.  The four groups
+// of 16 samples go through the network layer by layer, so that independent
+// MFMAs cover each other's latency: 8 + 8 + 4 MFMAs per 64 samples.
+__device__ __forceinline__ float mlpForward64(const MlpWave& m, const uint16_t* tile)
 {
   const int lane = threadIdx.x & 63, col = lane & 15, grp = lane >> 4;
+  const uint2* rows = reinterpret_cast<const uint2*>(tile);
+  bf16x4 x[4];
+#pragma unroll
+  for (int g = 0; g < 4; ++g) x[g] = __builtin_bit_cast(bf16x4, rows[(g * 16 + col) * 4 + grp]);
+  uint2 h1[4][2], h2[4][2];
+#pragma unroll
+  for (int g = 0; g < 4; ++g)
+#pragma unroll
+    for (int ob = 0; ob < 2; ++ob) h1[g][ob] = reluBf16(mfma16(m.a1[ob], x[g], m.c1[ob]));
+#pragma unroll
+  for (int g = 0; g < 4; ++g)
+#pragma unroll
+    for (int ob = 0; ob < 2; ++ob) h2[g][ob] = reluBf16(mfma32(m.a2[ob], concat8(h1[g][0], h1[g][1]), m.c2[ob]));
+  float mine = 0.0f;
+#pragma unroll
   for (int g = 0; g < 4; ++g) {
-    bf16x4 x;
-    for (int r = 0; r < 4; ++r) x[r] = (short)tile[(g * 16 + col) * kMlpInPad + grp * 4 + r];
-    const float z = mlpForward16(m, x);
-    if (grp == 0) zbuf[g * 16 + col] = z;
+    const f32x4 z = mfma32(m.a3, concat8(h2[g][0], h2[g][1]), m.c3);  // logit of sample 16*g + col
+    if (grp == g) mine = z[0];
   }
-  waveSync();
-  return zbuf[lane];
+  return mine;
 }
 
+// 16 BF16 features of one sample as two 16-byte stores (features 10..15 zero).
 __device__ __forceinline__ void packFeatures(const float f[kMlpIn], uint16_t* row)
 {
-  for (int k = 0; k < kMlpInPad; ++k) row[k] = k < kMlpIn ? toBf16(f[k]) : 0;
+  static_assert(kMlpIn == 10 && kMlpInPad == 16, "feature row layout");
+  uint4* r = reinterpret_cast<uint4*>(row);
+  r[0] = make_uint4(packBf16x2(f[0], f[1]), packBf16x2(f[2], f[3]), packBf16x2(f[4], f[5]), packBf16x2(f[6], f[7]));
+  r[1] = make_uint4(packBf16x2(f[8], f[9]), 0u, 0u, 0u);
 }
 
-__device__ __forceinline__ void candidateFeatures(const DevState& s, int w, int l, int pi, int pj, float f[kMlpIn])
+// Features of the tracklet candidate (pi of layer l, pj of layer l+1) of work
+// item w, whose vertex is (vx, vy, vz).
+__device__ __forceinline__ void candidateFeatures(const DevState& s, float vx, float vy, float vz, int l, int pi, int pj,
+                                                  float f[kMlpIn])
 {
+  const float4 a = s.crzp[pi], b = s.crzp[pj];  // (r, z, phi, 0)
   TrackletFeatureInput in;
   in.xi = s.cx[pi];
   in.yi = s.cy[pi];
-  in.zi = s.cz[pi];
-  in.ri = s.cr[pi];
-  in.phii = s.cphi[pi];
+  in.zi = a.y;
+  in.ri = a.x;
+  in.phii = a.z;
   in.xj = s.cx[pj];
   in.yj = s.cy[pj];
-  in.zj = s.cz[pj];
-  in.rj = s.cr[pj];
-  in.phij = s.cphi[pj];
-  in.vx = s.itemVertex[3 * w];
-  in.vy = s.itemVertex[3 * w + 1];
-  in.vz = s.itemVertex[3 * w + 2];
+  in.zj = b.y;
+  in.rj = b.x;
+  in.phij = b.z;
+  in.vx = vx;
+  in.vy = vy;
+  in.vz = vz;
   in.layer = l;
   in.countI = s.trkCnt[pi];
   in.countJ = l + 1 < kTrackletLayers ? s.trkCnt[pj] : 0;
   trackletFeatures(in, f);
 }
 
-template <int B>
-__device__ __forceinline__ int ownerOf(const int* exs, int o)
+// Set bits of `bits` in [a, b), b - a <= 32.
+__device__ __forceinline__ int popRange(const uint32_t* bits, int a, int b)
 {
-  int lo = 0, hi = B;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (exs[mid] <= o)
-      lo = mid;
-    else
-      hi = mid;
+  int n = 0;
+  while (a < b) {
+    const int word = a >> 5, sh = a & 31;
+    const int len = min(b - a, 32 - sh);
+    uint32_t v = bits[word] >> sh;
+    if (len < 32) v &= (1u << len) - 1u;
+    n += __popc(v);
+    a += len;
   }
-  return lo;
+  return n;
 }
 
 __device__ __forceinline__ int waveBlockScan(int v, int* scratch, int& total)
@@ -183,19 +234,28 @@ __device__ __forceinline__ int segmentOf(const int* chunkStart, int S, int chunk
 }
 
 // ---------------------------------------------------------------------------
-// Pipeline filter.  Runs after the tracklet count: every kept candidate slot
-// of a cluster is scored; candidates below the threshold are dropped and the
-// slots compacted in order.  Clusters whose candidates overflowed their slots
-// are kept whole.  Kept counts go to trkKept (trkCnt stays the raw count that
-// the features read) and the chunk sums are rewritten.
+// Pipeline filter.  Runs after the tracklet count, one chunk of B clusters per
+// iteration: (1) a scan of the candidate counts gives every candidate slot an
+// index o in the chunk, and an LDS map o -> owning cluster; (2) every
+// wavefront scores 64 candidates at a time (features, BF16 MFMA network), keeps
+// the candidate value in LDS and the keep decision as a ballot bit; (3) the
+// kept candidates are written back in place, compacted in order, each by its
+// own thread (its position is the number of kept bits before it in its
+// cluster).  Clusters whose candidates overflowed their slots are kept whole.
+// Kept counts go to trkKept (trkCnt stays the raw count that the features
+// read) and the chunk sums are rewritten.
 // ---------------------------------------------------------------------------
 template <int B>
 __global__ void __launch_bounds__(B) kTrackletsScore(DevState s, float threshold)
 {
+  static_assert(B <= 256, "the owner map stores cluster indices as bytes");
+  constexpr int kCand = B * kTrkSlots;
   __shared__ int scratch[B / 64 + 1];
   __shared__ int exs[B + 1];
-  __shared__ uint16_t tiles[B / 64][64 * kMlpInPad];
-  __shared__ float zbuf[B / 64][64];
+  __shared__ uint8_t owner[kCand];
+  __shared__ int candv[kCand];
+  __shared__ uint32_t keep[kCand / 32];
+  __shared__ __attribute__((aligned(16))) uint16_t tiles[B / 64][64 * kMlpInPad];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const MlpWave m = loadMlp(s.mlpW);
   const int S = s.W * kTrackletLayers;
@@ -214,37 +274,43 @@ __global__ void __launch_bounds__(B) kTrackletsScore(DevState s, float threshold
     const int ex = waveBlockScan(scored, scratch, total);
     exs[threadIdx.x] = ex;
     if (threadIdx.x == 0) exs[B] = total;
-    __syncthreads();
+    for (int k = 0; k < scored; ++k) owner[ex + k] = (uint8_t)threadIdx.x;
+    __syncthreads();  // every keep word read below is written whole by the tile loop
+    const float vx = s.itemVertex[3 * w], vy = s.itemVertex[3 * w + 1], vz = s.itemVertex[3 * w + 2];
+    const int rowBase = cur.base + first;
     for (int tile = wave; tile * 64 < total; tile += B / 64) {
       const int o = tile * 64 + lane;
       float f[kMlpIn];
-      int slot = -1;
       if (o < total) {
-        const int c = ownerOf<B>(exs, o);
-        const int pc = cur.base + first + c;
-        slot = pc * kTrkSlots + (o - exs[c]);
-        candidateFeatures(s, w, l, pc, next.base + s.trkCand[slot], f);
+        const int c = owner[o];
+        const int j = s.trkCand[(size_t)(rowBase + c) * kTrkSlots + (o - exs[c])];
+        candv[o] = j;
+        candidateFeatures(s, vx, vy, vz, l, rowBase + c, next.base + j, f);
       } else {
         for (int k = 0; k < kMlpIn; ++k) f[k] = 0.0f;
       }
       packFeatures(f, &tiles[wave][lane * kMlpInPad]);
       waveSync();
-      const float z = mlpForward64(m, tiles[wave], zbuf[wave]);
-      if (slot >= 0 && !(z >= threshold)) s.trkCand[slot] = -1 - s.trkCand[slot];  // dropped
+      const float z = mlpForward64(m, tiles[wave]);
+      const unsigned long long kept = __ballot(o < total && z >= threshold);
+      if (lane == 0) {
+        keep[2 * tile] = (uint32_t)kept;
+        keep[2 * tile + 1] = (uint32_t)(kept >> 32);
+      }
       waveSync();
     }
     __syncthreads();
-    int kept = raw;
-    if (i < cur.count && raw <= kTrkSlots) {
-      int* slots = s.trkCand + (size_t)p * kTrkSlots;
-      kept = 0;
-      for (int n = 0; n < raw; ++n)
-        if (slots[n] >= 0) slots[kept++] = slots[n];
+    for (int o = threadIdx.x; o < total; o += B) {
+      if ((keep[o >> 5] >> (o & 31)) & 1u) {
+        const int c = owner[o];
+        s.trkCand[(size_t)(rowBase + c) * kTrkSlots + popRange(keep, exs[c], o)] = candv[o];
+      }
     }
+    const int kept = scored == raw ? popRange(keep, ex, ex + scored) : raw;
     if (i < cur.count) s.trkKept[p] = kept;
-    int dummy;
-    waveBlockScan(i < cur.count ? kept : 0, scratch, dummy);
-    if (threadIdx.x == 0) s.chunkSum[(size_t)kStTracklets * s.maxChunks + chunk] = dummy;
+    int chunkKept;
+    waveBlockScan(i < cur.count ? kept : 0, scratch, chunkKept);
+    if (threadIdx.x == 0) s.chunkSum[(size_t)kStTracklets * s.maxChunks + chunk] = chunkKept;
   }
 }
 
@@ -307,10 +373,8 @@ __global__ void kCollect(DevState s, const uint8_t* label, int total, int stride
     const int pi = s.clSeg[w * kLayers + l].base + s.tFirst[t];
     const int pj = s.clSeg[w * kLayers + l + 1].base + s.tSecond[t];
     float f[kMlpIn];
-    candidateFeatures(s, w, l, pi, pj, f);
-    uint16_t row16[kMlpInPad];
-    packFeatures(f, row16);
-    for (int k = 0; k < kMlpInPad; ++k) outX[(size_t)(base + row) * kMlpInPad + k] = row16[k];
+    candidateFeatures(s, s.itemVertex[3 * w], s.itemVertex[3 * w + 1], s.itemVertex[3 * w + 2], l, pi, pj, f);
+    packFeatures(f, outX + (size_t)(base + row) * kMlpInPad);
     outY[base + row] = label[t];
   }
 }
@@ -318,16 +382,23 @@ __global__ void kCollect(DevState s, const uint8_t* label, int total, int stride
 // Logits of n samples of a training set (MFMA inference).
 __global__ void __launch_bounds__(256) kScoreRows(const uint16_t* w, const uint16_t* X, int n, float* out)
 {
-  __shared__ uint16_t tiles[4][64 * kMlpInPad];
-  __shared__ float zbuf[4][64];
+  __shared__ __attribute__((aligned(16))) uint16_t tiles[4][64 * kMlpInPad];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const MlpWave m = loadMlp(w);
   const int waves = gridDim.x * 4;
   for (int tile = blockIdx.x * 4 + wave; tile * 64 < n; tile += waves) {
     const int o = tile * 64 + lane;
-    for (int k = 0; k < kMlpInPad; ++k) tiles[wave][lane * kMlpInPad + k] = o < n ? X[(size_t)o * kMlpInPad + k] : 0;
+    uint4 lo = make_uint4(0u, 0u, 0u, 0u), hi = lo;
+    if (o < n) {
+      const uint4* src = reinterpret_cast<const uint4*>(X + (size_t)o * kMlpInPad);
+      lo = src[0];
+      hi = src[1];
+    }
+    uint4* dst = reinterpret_cast<uint4*>(&tiles[wave][lane * kMlpInPad]);
+    dst[0] = lo;
+    dst[1] = hi;
     waveSync();
-    const float z = mlpForward64(m, tiles[wave], zbuf[wave]);
+    const float z = mlpForward64(m, tiles[wave]);
     if (o < n) out[o] = z;
     waveSync();
   }
