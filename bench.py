@@ -65,6 +65,25 @@ def kernel_bytes(name, c):
     return 0
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ca_central_batch64_hbm_traffic.md")
+
+
+def measured_traffic(kernel, batch):
+    """HBM bytes of one launch of `kernel` (learned form), from the calibrated
+    FETCH_SIZE + WRITE_SIZE counter runs summarised in TRAFFIC_FILE; those runs
+    used this benchmark's default workload (64 central events, seeds 2..65), so
+    the figure is reported only for that workload."""
+    if batch != 64 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    name = kernel.split("<")[0]
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes" and parts[1] == name:
+                return int(parts[2])
+    return None
+
+
 def cpu_baseline(events_path, n_events):
     """The reference CPU tracker itself (built from its sources into
     oracle/_ref/), single thread, on a bounded sample of the same workload."""
@@ -129,6 +148,9 @@ def main():
     ap.add_argument("--cpu-events", type=int, default=200, help="events in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--weights", default=None, help="learned filter weights (default: the shipped file)")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="first run one device-to-device copy kernel of this many bytes (a reference dispatch of "
+                         "known traffic for hardware-counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every (event, vertex) is independent, so N GPUs would each run "
@@ -137,6 +159,17 @@ def main():
 
     import itsca
     from itsca import filter_training as ft
+
+    if args.copy_bytes:
+        import torch
+        src = torch.ones(args.copy_bytes // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("# copy kernel: %d bytes read, %d bytes written" % (args.copy_bytes, args.copy_bytes), file=sys.stderr,
+              flush=True)
+        del src, dst
 
     t_gen = time.time()
     events = itsca.generate_events(2, 2, args.batch)
@@ -202,7 +235,7 @@ def main():
                              "tracklets_kept": round(quality["tracklets_kept"], 4),
                              "cells_kept": round(quality["cells_kept"], 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(top, args.batch),
                      "kernel": top, "kernel_ms": round(top_ms, 4), "kernel_bytes_alg": top_bytes,
                      "pipeline_bytes_alg_per_step": balg,
                      "pipeline_GBps": round(balg / (ms_per_step * 1e-3) / 1e9, 2)},

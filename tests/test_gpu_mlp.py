@@ -259,3 +259,54 @@ def test_filter_keeps_exactly_the_tracklets_scored_above_threshold(tracker, quan
             dropped += int((~keep).sum())
     assert k == total and 0 < dropped < total
     tracker.set_filter(False)
+
+
+def _device_ms(tracker, runs=3):
+    """(all kernels, cell-building kernels) in ms per run, from HIP events."""
+    tracker.set_profiling(True)
+    total = cells = 0.0
+    for _ in range(runs):
+        tracker.run()
+        kt = tracker.kernel_times()
+        total += sum(ms for _, ms in kt) / runs
+        cells += sum(ms for name, ms in kt if name in ("kCellsCount", "kCellsFill")) / runs
+    tracker.set_profiling(False)
+    return total, cells
+
+
+def test_trained_filter_gain_over_plain_and_untrained(tracker):
+    """Ablation on held-out central events (seeds the training never saw): the
+    shipped network keeps >= 99.5 % of the plain tracks, finds none outside
+    them, and cuts the device time; an untrained network held to the same
+    recall of positive rows keeps far more tracklets."""
+    from itsca import filter_training as ft
+    say("gpu ablation: trained filter vs plain and vs untrained")
+    evs = itsca.generate_events(2, 30000, 24)
+    params, thr, _ = ft.load_filter()
+    tracker.set_mlp_params(params)
+    r = ft.evaluate(tracker, evs, thr)
+    say(f"  trained: recall {r['recall']:.5f}, outside {r['extra']}, tracklets kept {r['tracklets_kept']:.3f}")
+    assert r["extra"] == 0
+    assert r["recall"] >= 0.995
+    assert r["tracklets_kept"] < 0.3
+    learned_ms, learned_cells = _device_ms(tracker)
+    tracker.set_filter(False)
+    plain_ms, plain_cells = _device_ms(tracker)
+    say(f"  device time per run: plain {plain_ms:.3f} ms (cells {plain_cells:.3f}), "
+        f"learned {learned_ms:.3f} ms (cells {learned_cells:.3f})")
+    assert learned_cells < 0.5 * plain_cells
+    assert learned_ms < 0.9 * plain_ms
+
+    total = sum(sum(tracker.counts(i).tracklets) for i in range(tracker.num_items()))
+    tracker.train_clear(total)
+    tracker.train_collect(1, 0)
+    _, y = tracker.train_get(total)
+    z_trained = tracker.train_scores(total)
+    row_recall = float(np.mean(z_trained[y > 0] >= thr))
+    tracker.mlp_init(123)
+    z_untrained = tracker.train_scores(total)
+    thr_u = float(np.quantile(z_untrained[y > 0], 1.0 - row_recall))
+    kept_u = float(np.mean(z_untrained >= thr_u))
+    kept_t = float(np.mean(z_trained >= thr))
+    say(f"  positive-row recall {row_recall:.5f}: trained keeps {kept_t:.3f}, untrained keeps {kept_u:.3f}")
+    assert kept_u > 2.5 * kept_t
