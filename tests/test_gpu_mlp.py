@@ -228,7 +228,7 @@ def test_filter_keeps_exactly_the_tracklets_scored_above_threshold(tracker, quan
     """The in-pipeline scoring sees the same features and runs the same MFMA
     network as the training rows, so the filtered tracklet list is the plain
     list restricted to rows with logit >= threshold, in the same order
-    (clusters whose candidates overflowed the 32 slots are kept whole)."""
+    (clusters whose candidates overflowed their slots are kept whole)."""
     say(f"gpu filter keeps exactly the rows above the {quantile} quantile")
     evs = itsca.generate_events(2, 31, 1) + itsca.generate_events(5, 32, 1)
     tracker.set_filter(False)
@@ -252,7 +252,7 @@ def test_filter_keeps_exactly_the_tracklets_scored_above_threshold(tracker, quan
         for l in range(itsca.TRACKLET_LAYERS):
             t = plain[i][l]
             raw = np.bincount(t[:, 0], minlength=n.clusters[l]) if len(t) else np.zeros(n.clusters[l], int)
-            keep = (z[k:k + len(t)] >= threshold) | (raw[t[:, 0]] > 32)
+            keep = (z[k:k + len(t)] >= threshold) | (raw[t[:, 0]] > itsca.TRACKLET_SLOTS)
             k += len(t)
             got = tracker.download(i, itsca.TRACKLETS, l, n.tracklets[l], 4)[:, :2]
             assert np.array_equal(got, t[keep]), (i, l)

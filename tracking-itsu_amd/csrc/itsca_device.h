@@ -33,7 +33,7 @@ struct Seg {
 // chunkStart[s] is the first chunk of segment s.
 constexpr int kMinChunk = 64;
 // Accepted candidates kept per element between a count and its fill.
-constexpr int kTrkSlots = 32;  // per cluster (next-layer cluster positions)
+constexpr int kTrkSlots = 64;  // per cluster (next-layer cluster positions)
 constexpr int kCellSlots = 4;  // per tracklet (next-layer tracklet indices)
 constexpr int kEdgeSlots = 8;  // per cell (next-layer cell indices)
 
@@ -65,7 +65,7 @@ struct DevState {
   float4* crzp;  // (r, z, phi, 0) per sorted cluster: one load per tracklet candidate
   int *cid, *cmc, *cbin;
   float *sPhi, *sR;  // prep scratch, original order
-  int *sBin, *sTmp;
+  int *sBin, *sTmp, *sInv;  // per-cluster scratch of the sort: bin, bucket members, source of each sorted position
   int* table;  // W*7*401
   // tracklets
   int *trkCnt, *trkOff;  // per sorted cluster

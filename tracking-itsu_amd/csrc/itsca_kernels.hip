@@ -136,24 +136,34 @@ __global__ void __launch_bounds__(1024) kPrep(DevState s)
   }
   __threadfence_block();
   __syncthreads();
-  // ...then place each cluster at table[bin] + (number of bucket members with a
-  // smaller original index): a stable order, independent of the atomics.
+  // ...then each cluster's sorted position is table[bin] + (number of bucket
+  // members with a smaller original index): a stable order, independent of the
+  // atomics.  Only the source index is scattered to that position...
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const int b = s.sBin[base + i];
     const int lo = tab[b];
     const int hi = b < kBins ? tab[b + 1] : n;
     int rank = 0;
     for (int j = lo; j < hi; ++j) rank += s.sTmp[base + j] < i ? 1 : 0;
-    const int p = base + lo + rank;
-    s.cx[p] = s.rx[rawBase + i];
-    s.cy[p] = s.ry[rawBase + i];
-    s.cz[p] = s.rz[rawBase + i];
-    s.cphi[p] = s.sPhi[base + i];
-    s.cr[p] = s.sR[base + i];
-    s.crzp[p] = make_float4(s.sR[base + i], s.rz[rawBase + i], s.sPhi[base + i], 0.0f);
-    s.cid[p] = s.rid[rawBase + i];
-    s.cmc[p] = s.rmc[rawBase + i];
-    s.cbin[p] = b;
+    s.sInv[base + lo + rank] = i;
+  }
+  __threadfence_block();
+  __syncthreads();
+  // ...and the sorted records are written position by position (coalesced
+  // stores, gathered loads).
+  for (int p = threadIdx.x; p < n; p += blockDim.x) {
+    const int i = s.sInv[base + p];
+    const int q = base + p;
+    const float z = s.rz[rawBase + i], phi = s.sPhi[base + i], r = s.sR[base + i];
+    s.cx[q] = s.rx[rawBase + i];
+    s.cy[q] = s.ry[rawBase + i];
+    s.cz[q] = z;
+    s.cphi[q] = phi;
+    s.cr[q] = r;
+    s.crzp[q] = make_float4(r, z, phi, 0.0f);
+    s.cid[q] = s.rid[rawBase + i];
+    s.cmc[q] = s.rmc[rawBase + i];
+    s.cbin[q] = s.sBin[base + i];
   }
 }
 
@@ -284,22 +294,29 @@ __device__ __forceinline__ void forEachTracklet(const DevState& s, const int* ta
     const int jLast = table[maxBin];
     // Inclusive upper bound as in the reference, guarded by the layer size.
     const int jEnd = min(jLast + 1, nNext);
-    auto test = [&](const float4& q, int j) {
-      const float deltaZ = fabsf(tanLambda * (q.x - r) + z - q.y);
-      const float deltaPhi = fabsf(phi - q.z);
-      if (deltaZ < maxDeltaZ && (deltaPhi < kPhiCut || fabsf(deltaPhi - kTwoPi) < kPhiCut)) emit(j);
-    };
-    int j = jFirst;
-    // four loads in flight before the first test; emission order unchanged
-    for (; j + 4 <= jEnd; j += 4) {
-      const float4 q0 = s.crzp[baseNext + j], q1 = s.crzp[baseNext + j + 1], q2 = s.crzp[baseNext + j + 2],
-                   q3 = s.crzp[baseNext + j + 3];
-      test(q0, j);
-      test(q1, j + 1);
-      test(q2, j + 2);
-      test(q3, j + 3);
+    // Blocks of 8 candidates: all 8 loads are issued first (indices past the
+    // range are clamped to its last element and masked out), the cuts are
+    // combined without short-circuit branches, and the accepted candidates are
+    // emitted in ascending order from a bit mask.
+    for (int jb = jFirst; jb < jEnd; jb += 8) {
+      float4 q[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) q[k] = s.crzp[baseNext + min(jb + k, jEnd - 1)];
+      uint32_t mask = 0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const float deltaZ = fabsf(tanLambda * (q[k].x - r) + z - q[k].y);
+        const float deltaPhi = fabsf(phi - q[k].z);
+        const bool ok = (deltaZ < maxDeltaZ) & ((deltaPhi < kPhiCut) | (fabsf(deltaPhi - kTwoPi) < kPhiCut)) &
+                        (jb + k < jEnd);
+        mask |= (ok ? 1u : 0u) << k;
+      }
+      while (mask != 0u) {
+        const int k = __builtin_ctz(mask);
+        mask &= mask - 1u;
+        emit(jb + k);
+      }
     }
-    for (; j < jEnd; ++j) test(s.crzp[baseNext + j], j);
     ++row;
     if (row == kPhiBins) row = 0;
   }
@@ -471,18 +488,20 @@ __device__ __forceinline__ CellSeed cellSeed(const DevState& s, int w, int l, in
 
 // Cheap cuts of candidate iu (Tracker.cxx:161-173): tanLambda and phi
 // continuity, then the z of the line through the first cluster at the vertex.
+// The reference evaluates the z cut only after the first two pass; the cuts
+// have no side effects, so all three are evaluated and combined without
+// branches (same comparisons, same result).
 __device__ __forceinline__ bool cellCheapCuts(const CellSeed& c, float2 up)
 {
   const float uTanL = up.x, uPhi = up.y;
   const float deltaTanLambda = fabsf(c.tTanL - uTanL);
   const float deltaPhi = fabsf(c.tPhi - uPhi);
-  if (!(deltaTanLambda < kCellMaxDeltaTanLambda &&
-        (deltaPhi < kCellMaxDeltaPhi || fabsf(deltaPhi - kTwoPi) < kCellMaxDeltaPhi)))
-    return false;
+  const bool continuity = (deltaTanLambda < kCellMaxDeltaTanLambda) &
+                          ((deltaPhi < kCellMaxDeltaPhi) | (fabsf(deltaPhi - kTwoPi) < kCellMaxDeltaPhi));
   const float avgTanLambda = 0.5f * (c.tTanL + uTanL);
   const float dirZ = -avgTanLambda * c.r1 + c.z1;
   const float deltaZ = fabsf(dirZ - c.vz);
-  return deltaZ < cellMaxDeltaZ(c.l);
+  return continuity & (deltaZ < cellMaxDeltaZ(c.l));
 }
 
 // Conformal-plane fit and DCA cut (Tracker.cxx:175-218, MathUtils.h:53-60).
@@ -557,18 +576,21 @@ __global__ void __launch_bounds__(B) kCellsCount(DevState s)
         }
         ++m;
       };
-      int iu = c.uFirst;
-      // four loads in flight before the first test; survivor order unchanged
-      for (; iu + 4 <= uEnd; iu += 4) {
-        const float2 u0 = s.tTP[c.tnBase + iu], u1 = s.tTP[c.tnBase + iu + 1], u2 = s.tTP[c.tnBase + iu + 2],
-                     u3 = s.tTP[c.tnBase + iu + 3];
-        if (cellCheapCuts(c, u0)) survivor(iu);
-        if (cellCheapCuts(c, u1)) survivor(iu + 1);
-        if (cellCheapCuts(c, u2)) survivor(iu + 2);
-        if (cellCheapCuts(c, u3)) survivor(iu + 3);
+      // Blocks of 8 candidates: loads first (indices past the range clamped
+      // and masked out), cuts without branches, survivors in ascending order.
+      for (int ib = c.uFirst; ib < uEnd; ib += 8) {
+        float2 up[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) up[k] = s.tTP[c.tnBase + min(ib + k, uEnd - 1)];
+        uint32_t mask = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) mask |= (cellCheapCuts(c, up[k]) & (ib + k < uEnd) ? 1u : 0u) << k;
+        while (mask != 0u) {
+          const int k = __builtin_ctz(mask);
+          mask &= mask - 1u;
+          survivor(ib + k);
+        }
       }
-      for (; iu < uEnd; ++iu)
-        if (cellCheapCuts(c, s.tTP[c.tnBase + iu])) survivor(iu);
       if (m <= kCellList) {
         for (int q = 0; q < m; ++q) {
           CellOut o;

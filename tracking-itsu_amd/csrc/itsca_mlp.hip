@@ -179,7 +179,7 @@ __device__ __forceinline__ void candidateFeatures(const DevState& s, float vx, f
   trackletFeatures(in, f);
 }
 
-// Set bits of `bits` in [a, b), b - a <= 32.
+// Set bits of `bits` in [a, b).
 __device__ __forceinline__ int popRange(const uint32_t* bits, int a, int b)
 {
   int n = 0;
@@ -194,6 +194,8 @@ __device__ __forceinline__ int popRange(const uint32_t* bits, int a, int b)
   return n;
 }
 
+// Exclusive scan of v over the workgroup (wave scans by __shfl_up, then the
+// wave totals); total receives the sum.
 __device__ __forceinline__ int waveBlockScan(int v, int* scratch, int& total)
 {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nWaves = blockDim.x >> 6;
@@ -220,6 +222,7 @@ __device__ __forceinline__ int waveBlockScan(int v, int* scratch, int& total)
   return ex;
 }
 
+// Segment (item, layer) of a chunk: the largest sg with chunkStart[sg] <= chunk.
 __device__ __forceinline__ int segmentOf(const int* chunkStart, int S, int chunk)
 {
   int lo = 0, hi = S;
@@ -235,26 +238,31 @@ __device__ __forceinline__ int segmentOf(const int* chunkStart, int S, int chunk
 
 // ---------------------------------------------------------------------------
 // Pipeline filter.  Runs after the tracklet count, one chunk of B clusters per
-// iteration: (1) a scan of the candidate counts gives every candidate slot an
-// index o in the chunk, and an LDS map o -> owning cluster; (2) every
-// wavefront scores 64 candidates at a time (features, BF16 MFMA network), keeps
-// the candidate value in LDS and the keep decision as a ballot bit; (3) the
-// kept candidates are written back in place, compacted in order, each by its
-// own thread (its position is the number of kept bits before it in its
-// cluster).  Clusters whose candidates overflowed their slots are kept whole.
-// Kept counts go to trkKept (trkCnt stays the raw count that the features
-// read) and the chunk sums are rewritten.
+// iteration.  A scan of the candidate counts numbers every candidate slot of
+// the chunk (index o).  The slots are then taken in rounds of kRound: (1) an
+// LDS map o -> owning cluster; (2) every wavefront scores 64 candidates at a
+// time (features, BF16 MFMA network), keeps the candidate value in LDS and the
+// keep decision as a ballot bit; (3) every kept candidate is written back into
+// its cluster's slot row, in order, at position "kept before this round + kept
+// bits before it in this round".  A kept position never exceeds the
+// candidate's own position, and every slot of the round was read before the
+// barrier, so the in-place writes never overwrite a slot still to be read.
+// Clusters whose candidates overflowed their slots are kept whole.  Kept counts
+// go to trkKept (trkCnt stays the raw count that the features read) and the
+// chunk sums are rewritten.
 // ---------------------------------------------------------------------------
+constexpr int kRound = 4096;  // candidate slots per round (LDS: 4 KB owners, 16 KB values)
+
 template <int B>
 __global__ void __launch_bounds__(B) kTrackletsScore(DevState s, float threshold)
 {
   static_assert(B <= 256, "the owner map stores cluster indices as bytes");
-  constexpr int kCand = B * kTrkSlots;
   __shared__ int scratch[B / 64 + 1];
   __shared__ int exs[B + 1];
-  __shared__ uint8_t owner[kCand];
-  __shared__ int candv[kCand];
-  __shared__ uint32_t keep[kCand / 32];
+  __shared__ int keptBefore[B];
+  __shared__ uint8_t owner[kRound];
+  __shared__ int candv[kRound];
+  __shared__ uint32_t keep[kRound / 32];
   __shared__ __attribute__((aligned(16))) uint16_t tiles[B / 64][64 * kMlpInPad];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const MlpWave m = loadMlp(s.mlpW);
@@ -274,39 +282,48 @@ __global__ void __launch_bounds__(B) kTrackletsScore(DevState s, float threshold
     const int ex = waveBlockScan(scored, scratch, total);
     exs[threadIdx.x] = ex;
     if (threadIdx.x == 0) exs[B] = total;
-    for (int k = 0; k < scored; ++k) owner[ex + k] = (uint8_t)threadIdx.x;
-    __syncthreads();  // every keep word read below is written whole by the tile loop
+    keptBefore[threadIdx.x] = 0;
     const float vx = s.itemVertex[3 * w], vy = s.itemVertex[3 * w + 1], vz = s.itemVertex[3 * w + 2];
     const int rowBase = cur.base + first;
-    for (int tile = wave; tile * 64 < total; tile += B / 64) {
-      const int o = tile * 64 + lane;
-      float f[kMlpIn];
-      if (o < total) {
-        const int c = owner[o];
-        const int j = s.trkCand[(size_t)(rowBase + c) * kTrkSlots + (o - exs[c])];
-        candv[o] = j;
-        candidateFeatures(s, vx, vy, vz, l, rowBase + c, next.base + j, f);
-      } else {
-        for (int k = 0; k < kMlpIn; ++k) f[k] = 0.0f;
+    for (int r0 = 0; r0 < total; r0 += kRound) {
+      const int rn = min(kRound, total - r0);
+      for (int k = max(0, r0 - ex); k < scored && ex + k < r0 + rn; ++k) owner[ex + k - r0] = (uint8_t)threadIdx.x;
+      __syncthreads();  // owners (and exs, keptBefore) visible; every keep word read below is written whole
+      for (int tile = wave; tile * 64 < rn; tile += B / 64) {
+        const int o = tile * 64 + lane;  // slot index within the round
+        float f[kMlpIn];
+        if (o < rn) {
+          const int c = owner[o];
+          const int j = s.trkCand[(size_t)(rowBase + c) * kTrkSlots + (r0 + o - exs[c])];
+          candv[o] = j;
+          candidateFeatures(s, vx, vy, vz, l, rowBase + c, next.base + j, f);
+        } else {
+          for (int k = 0; k < kMlpIn; ++k) f[k] = 0.0f;
+        }
+        packFeatures(f, &tiles[wave][lane * kMlpInPad]);
+        waveSync();
+        const float z = mlpForward64(m, tiles[wave]);
+        const unsigned long long kept = __ballot(o < rn && z >= threshold);
+        if (lane == 0) {
+          keep[2 * tile] = (uint32_t)kept;
+          keep[2 * tile + 1] = (uint32_t)(kept >> 32);
+        }
+        waveSync();
       }
-      packFeatures(f, &tiles[wave][lane * kMlpInPad]);
-      waveSync();
-      const float z = mlpForward64(m, tiles[wave]);
-      const unsigned long long kept = __ballot(o < total && z >= threshold);
-      if (lane == 0) {
-        keep[2 * tile] = (uint32_t)kept;
-        keep[2 * tile + 1] = (uint32_t)(kept >> 32);
+      __syncthreads();
+      for (int o = threadIdx.x; o < rn; o += B) {
+        if ((keep[o >> 5] >> (o & 31)) & 1u) {
+          const int c = owner[o];
+          const int pos = keptBefore[c] + popRange(keep, max(exs[c] - r0, 0), o);
+          s.trkCand[(size_t)(rowBase + c) * kTrkSlots + pos] = candv[o];
+        }
       }
-      waveSync();
+      __syncthreads();  // positions above read keptBefore of this round
+      const int a = max(ex - r0, 0), b = min(ex + scored - r0, rn);
+      if (a < b) keptBefore[threadIdx.x] += popRange(keep, a, b);
+      __syncthreads();  // owner, candv, keep and keptBefore are reused by the next round
     }
-    __syncthreads();
-    for (int o = threadIdx.x; o < total; o += B) {
-      if ((keep[o >> 5] >> (o & 31)) & 1u) {
-        const int c = owner[o];
-        s.trkCand[(size_t)(rowBase + c) * kTrkSlots + popRange(keep, exs[c], o)] = candv[o];
-      }
-    }
-    const int kept = scored == raw ? popRange(keep, ex, ex + scored) : raw;
+    const int kept = scored == raw ? keptBefore[threadIdx.x] : raw;
     if (i < cur.count) s.trkKept[p] = kept;
     int chunkKept;
     waveBlockScan(i < cur.count ? kept : 0, scratch, chunkKept);

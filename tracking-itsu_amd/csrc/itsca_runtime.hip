@@ -109,7 +109,7 @@ struct itsca_ctx {
   DBuf<int> rid, rmc, evLayerOff, dItemEvent;
   DBuf<Seg> dClSeg, dTrkSeg, dCelSeg, dEdgSeg, dRoadSeg;
   DBuf<float> cx, cy, cz, cphi, cr, sPhi, sR;
-  DBuf<int> cid, cmc, cbin, sBin, sTmp, trkCnt, trkOff, trkKept, table;
+  DBuf<int> cid, cmc, cbin, sBin, sTmp, sInv, trkCnt, trkOff, trkKept, table;
   DBuf<float4> crzp;
   DBuf<int> tFirst, tSecond, cellCnt, cellOff, trkCand, cellCand;
   DBuf<float2> tTP;
@@ -181,6 +181,7 @@ struct itsca_ctx {
     HIP_TRY(cbin.reserve(SC));
     HIP_TRY(sBin.reserve(SC));
     HIP_TRY(sTmp.reserve(SC));
+    HIP_TRY(sInv.reserve(SC));
     HIP_TRY(trkCnt.reserve(SC));
     HIP_TRY(trkOff.reserve(SC));
     HIP_TRY(trkKept.reserve(SC));
@@ -301,6 +302,7 @@ struct itsca_ctx {
     s.sR = sR.p;
     s.sBin = sBin.p;
     s.sTmp = sTmp.p;
+    s.sInv = sInv.p;
     s.table = table.p;
     s.trkCnt = trkCnt.p;
     s.trkOff = trkOff.p;

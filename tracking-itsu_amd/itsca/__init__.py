@@ -63,6 +63,7 @@ HIP_SYMBOLS = [
     "itsca_train_epoch", "itsca_train_scores",
 ]
 MLP_PARAMS = 1441
+TRACKLET_SLOTS = 64  # candidate slots per cluster between the tracklet count and fill (kTrkSlots)
 MLP_FEATURES = 16
 HOST_SYMBOLS = [
     "itsca_events_generate", "itsca_events_load", "itsca_events_write", "itsca_events_free", "itsca_events_count",
