@@ -139,6 +139,62 @@ def kernel_profile(tracker, runs=5):
     return kernel_ms, launches, stage_ms
 
 
+def road_quality(tracker):
+    """Fake roads (Moore vote of the MC labels, Tracker.cxx:573-652), and
+    distinct MC labels among the other roads, over all work items."""
+    fake = correct = 0
+    for i in range(tracker.num_items()):
+        r = tracker.roads(i)
+        if len(r) == 0:
+            continue
+        fake += int((r[:, 7] != 0).sum())
+        good = r[r[:, 7] == 0][:, 6]
+        correct += len(set(int(x) for x in good.tolist() if x >= 0))
+    return fake, correct
+
+
+def mixed_batch(itsca, steps):
+    """256 events of log-uniform multiplicity in [50, 8000] tracks in one
+    launch, plain form, launch shape chosen by the autotuner, graph replay."""
+    evs = itsca.generate_events(3, 1000, 256)
+    t = itsca.GpuTracker(device=0)
+    t.set_events(evs)
+    t.run()
+    tuning, _ = t.autotune(3)
+    s = time_form(t, steps, 2)
+    out = {"workload": "256 events, multiplicity log-uniform in [50, 8000] tracks (seeds 1000..1255), one launch",
+           "form": "plain, hipGraph replay", "events_per_s": round(256 * steps / s, 2),
+           "ms_per_step": round(1e3 * s / steps, 4),
+           "tuning": {"block_size": tuning.block_size, "persistent_blocks": tuning.persistent_blocks}}
+    t.close()
+    return out
+
+
+def pileup_batch(itsca, ft, params, threshold, steps, events=64):
+    """Pile-up events (4 collisions, z ~ U(-10, 10) cm, 2000-8000 primaries
+    each, 3 % noise), every vertex its own work item, plain and learned."""
+    evs = itsca.generate_events(5, 40000, events)
+    t = itsca.GpuTracker(device=0)
+    t.set_events(evs)
+    t.set_mlp_params(params)
+    t.set_filter(False)
+    s_plain = time_form(t, steps, 1)
+    q_plain = road_quality(t)
+    t.set_filter(True, threshold)
+    s_learned = time_form(t, steps, 1)
+    q_learned = road_quality(t)
+    quality = ft.evaluate(t, evs, threshold)
+    t.close()
+    return {"workload": "%d pile-up events (4 vertices each, seeds 40000..%d), one launch" % (events, 40000 + events - 1),
+            "plain_events_per_s": round(events * steps / s_plain, 2),
+            "learned_events_per_s": round(events * steps / s_learned, 2),
+            "speedup": round(s_plain / s_learned, 4), "track_recall": round(quality["recall"], 6),
+            "plain_tracks": quality["plain_tracks"], "tracks_outside_plain": quality["extra"],
+            "fake_roads": {"plain": q_plain[0], "learned": q_learned[0]},
+            "correct_labels": {"plain": q_plain[1], "learned": q_learned[1]},
+            "tracklets_kept": round(quality["tracklets_kept"], 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +203,7 @@ def main():
     ap.add_argument("--batch", type=int, default=64, help="central events per step")
     ap.add_argument("--cpu-events", type=int, default=200, help="events in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the mixed-multiplicity and pile-up batches")
     ap.add_argument("--weights", default=None, help="learned filter weights (default: the shipped file)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy kernel of this many bytes (a reference dispatch of "
@@ -244,13 +301,21 @@ def main():
         "counts_event0": {"tracklets": list(counts[0].tracklets), "cells": list(counts[0].cells),
                           "roads": counts[0].roads},
     }
+    tracker.close()
+    if not args.no_extra:
+        for key, fn in (("mixed_batch", lambda: mixed_batch(itsca, 5)),
+                        ("pileup_batch", lambda: pileup_batch(itsca, ft, params, threshold, 3))):
+            try:
+                line[key] = fn()
+            except Exception as e:  # reported in the line; the main measurement stands
+                line[key] = {"error": "%s: %s" % (type(e).__name__, e)}
+            print("# %s done" % key, file=sys.stderr, flush=True)
     if not args.no_cpu:
         with tempfile.TemporaryDirectory() as tmp:
             p = os.path.join(tmp, "cpu_sample.txt")
             itsca.write_events(2, 2, args.cpu_events, p)
             line["cpu_baseline"] = cpu_baseline(p, args.cpu_events)
     print(json.dumps(line), flush=True)
-    tracker.close()
 
 
 if __name__ == "__main__":

@@ -66,6 +66,18 @@ unsigned long long ballot(bool pred)
   return m;
 }
 
+int shflXorInt(int v, int mask, int width)
+{
+  const int lane = (int)(tIdx.x % 64), wave = (int)(tIdx.x / 64);
+  std::barrier<>& b = *cur->waves[wave];
+  cur->shfl[tIdx.x] = v;
+  b.arrive_and_wait();
+  const int src = lane ^ mask;
+  const int r = (src < 64 && lane / width == src / width) ? cur->shfl[wave * 64 + src] : v;
+  b.arrive_and_wait();
+  return r;
+}
+
 void waveBarrier() { cur->waves[tIdx.x / 64]->arrive_and_wait(); }
 
 emu_f32x4 mfma16x16x16bf16(emu_bf16x4 a, emu_bf16x4 b, emu_f32x4 c)

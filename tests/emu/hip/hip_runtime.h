@@ -41,6 +41,7 @@ extern thread_local dim3 bDim, gDim;
 void syncthreads();
 int shflUp(int v, unsigned d, int width);
 float shflXor(float v, int mask, int width);
+int shflXorInt(int v, int mask, int width);
 unsigned long long ballot(bool pred);
 void waveBarrier();
 // v_mfma_f32_16x16x16_bf16 as documented for CDNA wave64: lane i holds
@@ -61,7 +62,20 @@ inline void __syncthreads() { emu::syncthreads(); }
 inline void __threadfence_block() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 inline int __shfl_up(int v, unsigned d, int width = 64) { return emu::shflUp(v, d, width); }
 inline float __shfl_xor(float v, int mask, int width = 64) { return emu::shflXor(v, mask, width); }
+inline int __shfl_xor(int v, int mask, int width = 64) { return emu::shflXorInt(v, mask, width); }
 inline unsigned long long __ballot(int pred) { return emu::ballot(pred != 0); }
+inline int __float_as_int(float f)
+{
+  int i;
+  std::memcpy(&i, &f, 4);
+  return i;
+}
+inline float __int_as_float(int i)
+{
+  float f;
+  std::memcpy(&f, &i, 4);
+  return f;
+}
 inline int __popc(unsigned v) { return __builtin_popcount(v); }
 
 // Wave-level builtins used by the MFMA code.
@@ -71,6 +85,20 @@ inline int __popc(unsigned v) { return __builtin_popcount(v); }
 #define __builtin_amdgcn_wave_barrier() emu::waveBarrier()
 inline float log1pf(float x) { return std::log1p(x); }
 inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline int atomicMin(int* p, int v)
+{
+  int old = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+  while (v < old && !__atomic_compare_exchange_n(p, &old, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {
+  }
+  return old;
+}
+inline int atomicMax(int* p, int v)
+{
+  int old = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+  while (v > old && !__atomic_compare_exchange_n(p, &old, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {
+  }
+  return old;
+}
 
 inline int min(int a, int b) { return a < b ? a : b; }
 inline int max(int a, int b) { return a > b ? a : b; }

@@ -63,6 +63,7 @@ struct DevState {
   const Seg* clSeg;  // W*7
   float *cx, *cy, *cz, *cphi, *cr;
   float4* crzp;  // (r, z, phi, 0) per sorted cluster: one load per tracklet candidate
+  float4* segGeo;  // per (item, layer): (min r, max r, 1 if every z bin index lies in [0, kZBins) else 0, 0)
   int *cid, *cmc, *cbin;
   float *sPhi, *sR;  // prep scratch, original order
   int *sBin, *sTmp, *sInv;  // per-cluster scratch of the sort: bin, bucket members, source of each sorted position

@@ -89,11 +89,15 @@ __device__ __forceinline__ int segmentOfChunk(const int* chunkStart, int S, int 
 // PrimaryVertexContext.cxx:31-142, Cluster.cxx:35-45.
 // One workgroup per (item, layer).
 // ---------------------------------------------------------------------------
+constexpr int kPrepLds = 16384;  // layers up to this many clusters keep the bucket array in LDS
+
 __global__ void __launch_bounds__(1024) kPrep(DevState s)
 {
+  __shared__ int lBucket[kPrepLds];
   __shared__ int hist[kTableSize];
   __shared__ int tab[kTableSize];
   __shared__ int scratch[1024 / 64 + 1];
+  __shared__ int geo[4];  // min r, max r (float bits; r >= 0 orders like ints), min and max z bin index
   const int w = blockIdx.x / kLayers, l = blockIdx.x % kLayers;
   if (blockIdx.x == 0 && threadIdx.x < kNumStatus) s.status[threadIdx.x] = 0;
   const Seg seg = s.clSeg[w * kLayers + l];
@@ -102,20 +106,47 @@ __global__ void __launch_bounds__(1024) kPrep(DevState s)
   const float vx = s.itemVertex[3 * w], vy = s.itemVertex[3 * w + 1];
   const int n = seg.count, base = seg.base;
   for (int b = threadIdx.x; b < kTableSize; b += blockDim.x) hist[b] = 0;
+  if (threadIdx.x == 0) {
+    geo[0] = __float_as_int(3.0e38f);
+    geo[1] = 0;
+    geo[2] = kZBins;
+    geo[3] = -1;
+  }
   __syncthreads();
+  int rMinBits = __float_as_int(3.0e38f), rMaxBits = 0, zMin = kZBins, zMax = -1;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const float x = s.rx[rawBase + i], y = s.ry[rawBase + i], z = s.rz[rawBase + i];
     const float dx = x - vx, dy = y - vy;
     const float phi = clusterPhi(dx, dy);
     const float r = sqrtf(dx * dx + dy * dy);
-    const int b = binIndex(zBinIndex(l, z), phiBinIndex(phi));
+    const int zb = zBinIndex(l, z);
+    const int b = binIndex(zb, phiBinIndex(phi));
     s.sPhi[base + i] = phi;
     s.sR[base + i] = r;
     s.sBin[base + i] = b;
     s.trkCnt[base + i] = 0;
     atomicAdd(&hist[b], 1);
+    rMinBits = min(rMinBits, __float_as_int(r));  // a NaN r has bits above every finite r
+    rMaxBits = max(rMaxBits, __float_as_int(r));
+    zMin = min(zMin, zb);
+    zMax = max(zMax, zb);
+  }
+  for (int d = 32; d >= 1; d >>= 1) {
+    rMinBits = min(rMinBits, __shfl_xor(rMinBits, d, 64));
+    rMaxBits = max(rMaxBits, __shfl_xor(rMaxBits, d, 64));
+    zMin = min(zMin, __shfl_xor(zMin, d, 64));
+    zMax = max(zMax, __shfl_xor(zMax, d, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(&geo[0], rMinBits);
+    atomicMax(&geo[1], rMaxBits);
+    atomicMin(&geo[2], zMin);
+    atomicMax(&geo[3], zMax);
   }
   __syncthreads();
+  if (threadIdx.x == 0)
+    s.segGeo[w * kLayers + l] = make_float4(__int_as_float(geo[0]), __int_as_float(geo[1]),
+                                            geo[2] >= 0 && geo[3] < kZBins ? 1.0f : 0.0f, 0.0f);
   // Exclusive scan of the 401 bin counts: table[b] = #clusters with bin < b.
   int* table = s.table + (size_t)(w * kLayers + l) * kTableSize;
   {
@@ -130,22 +161,36 @@ __global__ void __launch_bounds__(1024) kPrep(DevState s)
   }
   __syncthreads();
   // Bucket the original indices (order inside a bucket is arbitrary here)...
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const int pos = atomicAdd(&hist[s.sBin[base + i]], 1);
-    s.sTmp[base + pos] = i;
-  }
-  __threadfence_block();
-  __syncthreads();
-  // ...then each cluster's sorted position is table[bin] + (number of bucket
+  // then each cluster's sorted position is table[bin] + (number of bucket
   // members with a smaller original index): a stable order, independent of the
-  // atomics.  Only the source index is scattered to that position...
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const int b = s.sBin[base + i];
-    const int lo = tab[b];
-    const int hi = b < kBins ? tab[b + 1] : n;
-    int rank = 0;
-    for (int j = lo; j < hi; ++j) rank += s.sTmp[base + j] < i ? 1 : 0;
-    s.sInv[base + lo + rank] = i;
+  // atomics.  Only the source index is scattered to that position.  The bucket
+  // array lives in LDS when the layer fits, else in global scratch.
+  if (n <= kPrepLds) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) lBucket[atomicAdd(&hist[s.sBin[base + i]], 1)] = i;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const int b = s.sBin[base + i];
+      const int lo = tab[b];
+      const int hi = b < kBins ? tab[b + 1] : n;
+      int rank = 0;
+      for (int j = lo; j < hi; ++j) rank += lBucket[j] < i ? 1 : 0;
+      s.sInv[base + lo + rank] = i;
+    }
+  } else {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const int pos = atomicAdd(&hist[s.sBin[base + i]], 1);
+      s.sTmp[base + pos] = i;
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const int b = s.sBin[base + i];
+      const int lo = tab[b];
+      const int hi = b < kBins ? tab[b + 1] : n;
+      int rank = 0;
+      for (int j = lo; j < hi; ++j) rank += s.sTmp[base + j] < i ? 1 : 0;
+      s.sInv[base + lo + rank] = i;
+    }
   }
   __threadfence_block();
   __syncthreads();
@@ -274,14 +319,37 @@ __device__ __forceinline__ TrackletWindow trackletWindow(float z, float r, float
 
 // Visits every accepted candidate of cluster i in the reference's order
 // (window row order, then ascending candidate position).
+// Skipping z bins.  Within a window row the reference tests every cluster of
+// the z bins [win.x, win.z] (a +-1 cm window) although the cut keeps only
+// |tanL*(r_j - r) + z - z_j| < maxDeltaZ (0.1-0.3 cm).  A cluster sits in the z
+// bin slot of its row that zBinIndex (monotone in z) gives it, as long as that
+// index lies in [0, kZBins) -- otherwise getBinIndex spills it into another
+// row, and nothing is skipped.  With r_j in the next layer's [min r, max r],
+// a cluster can pass only if z_j lies in [zLo, zHi] = the range of
+// tanL*(r_j - r) + z, widened by maxDeltaZ and a margin (kZSkipMargin, far
+// above float rounding at these magnitudes); bins wholly below zBinIndex(zLo)
+// or above zBinIndex(zHi) are skipped.  The reference's extra candidate (the
+// inclusive upper bound) is still tested, after the others, so the accepted
+// candidates and their order are unchanged.
+constexpr float kZSkipMargin = 0.01f;
+
 template <typename F>
 __device__ __forceinline__ void forEachTracklet(const DevState& s, const int* table, int baseNext, int nNext,
-                                                float z, float r, float phi, float vz, int l, F&& emit)
+                                                float4 geoNext, float z, float r, float phi, float vz, int l, F&& emit)
 {
   const TrackletWindow win = trackletWindow(z, r, phi, vz, l);
   if (win.empty) return;
   const float tanLambda = (z - vz) / r;
   const float maxDeltaZ = trackletMaxDeltaZ(l);
+  int zFirst = win.x, zLast = win.z;  // z bins of each row whose clusters are tested
+  if (geoNext.z != 0.0f) {
+    const float zA = tanLambda * (geoNext.x - r) + z, zB = tanLambda * (geoNext.y - r) + z;
+    const float zLo = fminf(zA, zB) - maxDeltaZ - kZSkipMargin, zHi = fmaxf(zA, zB) + maxDeltaZ + kZSkipMargin;
+    if (zLo > -1.0e4f && zHi < 1.0e4f && zLo <= zHi) {  // finite and ordered; else test every bin
+      zFirst = max(zFirst, zBinIndex(l + 1, zLo));
+      zLast = min(zLast, zBinIndex(l + 1, zHi));
+    }
+  }
   int row = win.y;
   for (int count = 0; count < win.rows; ++count) {
     const int firstBin = binIndex(win.x, row);
@@ -290,10 +358,12 @@ __device__ __forceinline__ void forEachTracklet(const DevState& s, const int* ta
     // reference reads the following memory there (only reachable when phi
     // lies within 2 ulp below 0.3).  Clamp to the table end.
     if (maxBin > kBins) maxBin = kBins;
-    const int jFirst = table[firstBin];
-    const int jLast = table[maxBin];
-    // Inclusive upper bound as in the reference, guarded by the layer size.
-    const int jEnd = min(jLast + 1, nNext);
+    const int jLast = table[maxBin];  // the reference's inclusive upper bound
+    int jFirst = jLast, jEnd = jLast;  // the clusters of bins [zFirst, zLast] of this row
+    if (zFirst <= zLast) {  // table positions clamped like maxBin (firstBin <= position <= maxBin)
+      jFirst = table[min(firstBin + (zFirst - win.x), maxBin)];
+      jEnd = min(table[min(firstBin + (zLast - win.x) + 1, maxBin)], nNext);
+    }
     // Blocks of 8 candidates: all 8 loads are issued first (indices past the
     // range are clamped to its last element and masked out), the cuts are
     // combined without short-circuit branches, and the accepted candidates are
@@ -316,6 +386,12 @@ __device__ __forceinline__ void forEachTracklet(const DevState& s, const int* ta
         mask &= mask - 1u;
         emit(jb + k);
       }
+    }
+    if (jLast < nNext) {  // the extra candidate of the inclusive bound
+      const float4 q = s.crzp[baseNext + jLast];
+      const float deltaZ = fabsf(tanLambda * (q.x - r) + z - q.y);
+      const float deltaPhi = fabsf(phi - q.z);
+      if (deltaZ < maxDeltaZ && (deltaPhi < kPhiCut || fabsf(deltaPhi - kTwoPi) < kPhiCut)) emit(jLast);
     }
     ++row;
     if (row == kPhiBins) row = 0;
@@ -341,7 +417,8 @@ __global__ void __launch_bounds__(B) kTrackletsCount(DevState s)
       const int* table = s.table + (size_t)(w * kLayers + l + 1) * kTableSize;
       const int p = cur.base + i;
       int* slots = s.trkCand + (size_t)p * kTrkSlots;
-      forEachTracklet(s, table, next.base, next.count, s.cz[p], s.cr[p], s.cphi[p], s.itemVertex[3 * w + 2], l,
+      forEachTracklet(s, table, next.base, next.count, s.segGeo[w * kLayers + l + 1], s.cz[p], s.cr[p], s.cphi[p],
+                      s.itemVertex[3 * w + 2], l,
                       [&](int j) {
                         if (cnt < kTrkSlots) slots[cnt] = j;
                         ++cnt;
@@ -417,7 +494,8 @@ __global__ void __launch_bounds__(B) kTrackletsFill(DevState s)
     if (cnt > kTrkSlots) {
       const int* table = s.table + (size_t)(w * kLayers + l + 1) * kTableSize;
       int k = chunkBase + ex;
-      forEachTracklet(s, table, next.base, next.count, s.cz[p], s.cr[p], s.cphi[p], s.itemVertex[3 * w + 2], l,
+      forEachTracklet(s, table, next.base, next.count, s.segGeo[w * kLayers + l + 1], s.cz[p], s.cr[p], s.cphi[p],
+                      s.itemVertex[3 * w + 2], l,
                       [&](int j) { writeTracklet(s, k++, i, j, cur.base, next.base); });
     }
   }

@@ -110,7 +110,7 @@ struct itsca_ctx {
   DBuf<Seg> dClSeg, dTrkSeg, dCelSeg, dEdgSeg, dRoadSeg;
   DBuf<float> cx, cy, cz, cphi, cr, sPhi, sR;
   DBuf<int> cid, cmc, cbin, sBin, sTmp, sInv, trkCnt, trkOff, trkKept, table;
-  DBuf<float4> crzp;
+  DBuf<float4> crzp, segGeo;
   DBuf<int> tFirst, tSecond, cellCnt, cellOff, trkCand, cellCand;
   DBuf<float2> tTP;
   DBuf<int> succCnt, succCand, eFrom, eTo, eTmp;
@@ -188,6 +188,7 @@ struct itsca_ctx {
     HIP_TRY(crzp.reserve(SC));
     HIP_TRY(trkCand.reserve(SC * kTrkSlots));
     HIP_TRY(table.reserve((size_t)W * kLayers * kTableSize));
+    HIP_TRY(segGeo.reserve((size_t)std::max(W, 1) * kLayers));
     HIP_TRY(tFirst.reserve(tCap));
     HIP_TRY(tSecond.reserve(tCap));
     HIP_TRY(tTP.reserve(tCap));
@@ -295,6 +296,7 @@ struct itsca_ctx {
     s.cphi = cphi.p;
     s.cr = cr.p;
     s.crzp = crzp.p;
+    s.segGeo = segGeo.p;
     s.cid = cid.p;
     s.cmc = cmc.p;
     s.cbin = cbin.p;
