@@ -32,6 +32,13 @@ int itsca_events_layer_offsets(const itsca_events* ev, int event, int32_t offset
 int itsca_events_copy(const itsca_events* ev, int event, float* x, float* y, float* z, int32_t* cluster_id,
                       int32_t* mc_id, float* vertices);
 
+/* Learned tracklet filter weights (no counterpart in the reference): reads a
+ * weights file written by `python -m itsca.filter_training` (JSON: the 1441
+ * parameters as float32 bit patterns in hex under "params_f32_bits", the logit
+ * threshold under "threshold").  params receives n_params floats; returns 0 on
+ * success, -1 if the file cannot be read or does not hold n_params values. */
+int itsca_filter_load(const char* path, float* params, int n_params, float* threshold);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,3 +44,28 @@ def test_create_fails_loudly_without_a_device():
 def test_bad_device_is_an_argument_error():
     with pytest.raises(itsca.ItscaError):
         itsca.GpuTracker(device=4096)
+
+
+def test_native_weights_loader_matches_python():
+    """The C++ API reads the learned filter's weights file (itsca_filter_load)
+    to exactly the values the Python loader gives."""
+    import ctypes
+
+    import numpy as np
+
+    from itsca import filter_training as ft
+
+    params, thr, _ = ft.load_filter()
+    out = np.zeros(itsca.MLP_PARAMS, dtype=np.float32)
+    t = ctypes.c_float(0)
+    lib = itsca.host_lib()
+    rc = lib.itsca_filter_load(ft.DEFAULT_WEIGHTS.encode(), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                               itsca.MLP_PARAMS, ctypes.byref(t))
+    assert rc == 0
+    assert np.array_equal(out.view(np.uint32), params.view(np.uint32))
+    assert t.value == np.float32(thr)
+    # wrong parameter count and missing file are errors
+    assert lib.itsca_filter_load(ft.DEFAULT_WEIGHTS.encode(), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                 100, ctypes.byref(t)) == -1
+    assert lib.itsca_filter_load(b"/nonexistent/w.json", out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                 itsca.MLP_PARAMS, ctypes.byref(t)) == -1

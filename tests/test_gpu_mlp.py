@@ -310,3 +310,27 @@ def test_trained_filter_gain_over_plain_and_untrained(tracker):
     kept_t = float(np.mean(z_trained >= thr))
     say(f"  positive-row recall {row_recall:.5f}: trained keeps {kept_t:.3f}, untrained keeps {kept_u:.3f}")
     assert kept_u > 2.5 * kept_t
+
+
+def test_cli_plain_and_with_learned_filter(golden_dir):
+    """The reference-style CLI (bin/itsca-track) finds the fixture's road count
+    plain, and a subset of it with --filter and the shipped weights."""
+    import os
+    import re
+    import subprocess
+
+    from itsca import filter_training as ft
+    say("gpu cli plain and --filter")
+    exe = os.path.join(os.path.dirname(itsca.__file__), "..", "bin", "itsca-track")
+    events = os.path.join(golden_dir, "small_s1.events.txt")
+
+    def roads(extra):
+        out = subprocess.run([exe, events] + extra, capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stdout + out.stderr
+        return [int(n) for n in re.findall(r"Found (\d+) roads for vertex", out.stdout)]
+
+    want = int(re.search(r"^roads (\d+)$", fixture("small_s1", "stable", 0), re.M).group(1))
+    plain = roads([])
+    learned = roads(["--filter", ft.DEFAULT_WEIGHTS])
+    assert plain == [want]
+    assert len(learned) == 1 and 0 < learned[0] <= want

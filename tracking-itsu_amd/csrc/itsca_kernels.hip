@@ -442,20 +442,30 @@ __device__ __forceinline__ void writeTracklet(const DevState& s, int k, int i, i
   s.cellCnt[k] = 0;  // stays 0 where the cell stage does not run
 }
 
-// Position of output slot o among the chunk's elements: the largest c with
-// exs[c] <= o (exs is the block's exclusive scan, exs[B] = total > o).
-template <int B>
-__device__ __forceinline__ int ownerOfSlot(const int* exs, int o)
+// Enumerates the slot outputs of a chunk, one per thread: element c (thread c)
+// owns `capped` outputs (0 for an element whose outputs overflowed its slots),
+// numbered from its exclusive scan `exc`; exCap holds every element's exc and
+// must be written before the call.  The owner of each output comes from an LDS
+// map built in rounds of kOwnerRound outputs, so every output costs one LDS
+// lookup.  visit(c, k) handles the k-th output of element c.  B <= 256 (the
+// map stores element indices as bytes).
+constexpr int kOwnerRound = 4096;
+
+template <int B, typename F>
+__device__ __forceinline__ void forEachSlotOutput(int capped, int exc, int totalCap, const int* exCap,
+                                                  uint8_t* owner, F&& visit)
 {
-  int lo = 0, hi = B;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (exs[mid] <= o)
-      lo = mid;
-    else
-      hi = mid;
+  static_assert(B <= 256, "owner indices are bytes");
+  for (int r0 = 0; r0 < totalCap; r0 += kOwnerRound) {
+    const int rn = min(kOwnerRound, totalCap - r0);
+    for (int k = max(0, r0 - exc); k < capped && exc + k < r0 + rn; ++k) owner[exc + k - r0] = (uint8_t)threadIdx.x;
+    __syncthreads();
+    for (int q = threadIdx.x; q < rn; q += B) {
+      const int c = owner[q];
+      visit(c, r0 + q - exCap[c]);
+    }
+    __syncthreads();  // the map is rebuilt by the next round
   }
-  return lo;
 }
 
 // Fill: one output record per thread of the chunk (coalesced writes, no
@@ -465,7 +475,8 @@ template <int B>
 __global__ void __launch_bounds__(B) kTrackletsFill(DevState s)
 {
   __shared__ int scratch[B / 64 + 1];
-  __shared__ int exs[B + 1];
+  __shared__ int exFull[B], exCap[B];
+  __shared__ uint8_t owner[kOwnerRound];
   const int S = s.W * kTrackletLayers;
   const int* chunkStart = s.chunkStart + kStTracklets * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
@@ -477,20 +488,18 @@ __global__ void __launch_bounds__(B) kTrackletsFill(DevState s)
     const int i = first + threadIdx.x;
     const int p = cur.base + i;
     const int cnt = i < cur.count ? s.trkCnt[p] : 0;
-    int total;
+    const int capped = cnt <= kTrkSlots ? cnt : 0;
+    int total, totalCap;
     const int ex = blockExclusiveScan(cnt, scratch, total);
-    exs[threadIdx.x] = ex;
-    if (threadIdx.x == 0) exs[B] = total;
+    const int exc = blockExclusiveScan(capped, scratch, totalCap);
+    exFull[threadIdx.x] = ex;
+    exCap[threadIdx.x] = exc;
     const int chunkBase = s.chunkOff[(size_t)kStTracklets * s.maxChunks + chunk];
     if (i < cur.count) s.trkOff[p] = chunkBase + ex - s.trkSeg[sg].base;
-    __syncthreads();
-    for (int o = threadIdx.x; o < total; o += B) {
-      const int c = ownerOfSlot<B>(exs, o);
-      if (exs[c + 1] - exs[c] > kTrkSlots) continue;
-      const int n = o - exs[c];
-      const int j = s.trkCand[(size_t)(cur.base + first + c) * kTrkSlots + n];
-      writeTracklet(s, chunkBase + o, first + c, j, cur.base, next.base);
-    }
+    forEachSlotOutput<B>(capped, exc, totalCap, exCap, owner, [&](int c, int k) {
+      const int j = s.trkCand[(size_t)(cur.base + first + c) * kTrkSlots + k];
+      writeTracklet(s, chunkBase + exFull[c] + k, first + c, j, cur.base, next.base);
+    });
     if (cnt > kTrkSlots) {
       const int* table = s.table + (size_t)(w * kLayers + l + 1) * kTableSize;
       int k = chunkBase + ex;
@@ -710,7 +719,8 @@ template <int B>
 __global__ void __launch_bounds__(B) kCellsFill(DevState s)
 {
   __shared__ int scratch[B / 64 + 1];
-  __shared__ int exs[B + 1];
+  __shared__ int exFull[B], exCap[B];
+  __shared__ uint8_t owner[kOwnerRound];
   const int S = s.W * kCellLayers;
   const int* chunkStart = s.chunkStart + kStCells * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
@@ -722,24 +732,22 @@ __global__ void __launch_bounds__(B) kCellsFill(DevState s)
     const int it = firstT + threadIdx.x;
     const int t = tl.base + it;
     const int cnt = it < tl.count ? s.cellCnt[t] : 0;
-    int total;
+    const int capped = cnt <= kCellSlots ? cnt : 0;
+    int total, totalCap;
     const int ex = blockExclusiveScan(cnt, scratch, total);
-    exs[threadIdx.x] = ex;
-    if (threadIdx.x == 0) exs[B] = total;
+    const int exc = blockExclusiveScan(capped, scratch, totalCap);
+    exFull[threadIdx.x] = ex;
+    exCap[threadIdx.x] = exc;
     const int chunkBase = s.chunkOff[(size_t)kStCells * s.maxChunks + chunk];
     if (it < tl.count) s.cellOff[t] = chunkBase + ex - s.celSeg[sg].base;
-    __syncthreads();
-    for (int o = threadIdx.x; o < total; o += B) {
-      const int c = ownerOfSlot<B>(exs, o);
-      if (exs[c + 1] - exs[c] > kCellSlots) continue;
-      const int n = o - exs[c];
+    forEachSlotOutput<B>(capped, exc, totalCap, exCap, owner, [&](int c, int n) {
       const int tt = tl.base + firstT + c;
       const int iu = s.cellCand[(size_t)tt * kCellSlots + n];
       const CellSeed cs = cellSeed(s, w, l, tt);
       CellOut out;
       cellFit(s, cs, iu, out);  // accepted in the count pass; same inputs, same result
-      writeCell(s, cs, chunkBase + o, firstT + c, iu, out, s.tFirst[tt], s.tSecond[tt]);
-    }
+      writeCell(s, cs, chunkBase + exFull[c] + n, firstT + c, iu, out, s.tFirst[tt], s.tSecond[tt]);
+    });
     if (cnt > kCellSlots) {
       const CellSeed cs = cellSeed(s, w, l, t);
       int k = chunkBase + ex;
@@ -824,7 +832,8 @@ template <int B>
 __global__ void __launch_bounds__(B) kEdgesFill(DevState s)
 {
   __shared__ int scratch[B / 64 + 1];
-  __shared__ int exs[B + 1];
+  __shared__ int exFull[B], exCap[B];
+  __shared__ uint8_t owner[kOwnerRound];
   const int S = s.W * (kCellLayers - 1);
   const int* chunkStart = s.chunkStart + kStEdges * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
@@ -836,21 +845,20 @@ __global__ void __launch_bounds__(B) kEdgesFill(DevState s)
     const int ic = firstC + threadIdx.x;
     const int c = cl.base + ic;
     const int cnt = ic < cl.count ? s.succCnt[c] : 0;
-    int total;
+    const int capped = cnt <= kEdgeSlots ? cnt : 0;
+    int total, totalCap;
     const int ex = blockExclusiveScan(cnt, scratch, total);
-    exs[threadIdx.x] = ex;
-    if (threadIdx.x == 0) exs[B] = total;
+    const int exc = blockExclusiveScan(capped, scratch, totalCap);
+    exFull[threadIdx.x] = ex;
+    exCap[threadIdx.x] = exc;
     const int chunkBase = s.chunkOff[(size_t)kStEdges * s.maxChunks + chunk];
-    __syncthreads();
-    for (int o = threadIdx.x; o < total; o += B) {
-      const int owner = ownerOfSlot<B>(exs, o);
-      if (exs[owner + 1] - exs[owner] > kEdgeSlots) continue;
-      const int k = chunkBase + o;
+    forEachSlotOutput<B>(capped, exc, totalCap, exCap, owner, [&](int o, int n) {
+      const int k = chunkBase + exFull[o] + n;
       if (k < s.eCap) {
-        s.eFrom[k] = firstC + owner;
-        s.eTo[k] = s.succCand[(size_t)(cl.base + firstC + owner) * kEdgeSlots + (o - exs[owner])];
+        s.eFrom[k] = firstC + o;
+        s.eTo[k] = s.succCand[(size_t)(cl.base + firstC + o) * kEdgeSlots + n];
       }
-    }
+    });
     if (cnt > kEdgeSlots) {
       int k = chunkBase + ex;
       forEachSuccessor(s, w, l, c, [&](int in) {

@@ -45,30 +45,60 @@ struct TrackletFeatureInput {
   int layer, countI, countJ;
 };
 
-ITSCA_HD void trackletFeatures(const TrackletFeatureInput& in, float f[kMlpIn])
+// The part of the features that depends on cluster i only (shared by all of
+// its candidates).
+struct TrackletSeed {
+  float ax, ay, a2;  // cluster i relative to the vertex, and its squared length
+  float zi, ri, phii;
+  float pointing;    // (z_i - v_z) / r_i
+  float countI;      // min(count_i, 32) / 16
+};
+
+ITSCA_HD TrackletSeed trackletSeed(float xi, float yi, float zi, float ri, float phii, float vx, float vy, float vz,
+                                   int countI)
 {
-  const float tanL = (in.zi - in.zj) / (in.ri - in.rj);
-  float dphi = in.phij - in.phii;
+  TrackletSeed t;
+  t.ax = xi - vx;
+  t.ay = yi - vy;
+  t.a2 = t.ax * t.ax + t.ay * t.ay;
+  t.zi = zi;
+  t.ri = ri;
+  t.phii = phii;
+  t.pointing = (zi - vz) / ri;
+  t.countI = (float)(countI < 32 ? countI : 32) * (1.0f / 16.0f);
+  return t;
+}
+
+// Features of candidate j of layer l+1 (bx, by relative to the vertex).
+ITSCA_HD void candidateFeatures(const TrackletSeed& t, int layer, float bx, float by, float zj, float rj, float phij,
+                                float vz, int countJ, float f[kMlpIn])
+{
+  const float tanL = (t.zi - zj) / (t.ri - rj);
+  float dphi = phij - t.phii;
   if (dphi > kPi) dphi -= kTwoPi;
   if (dphi < -kPi) dphi += kTwoPi;
-  const float ax = in.xi - in.vx, ay = in.yi - in.vy, bx = in.xj - in.vx, by = in.yj - in.vy;
-  const float cross = ax * by - ay * bx;
-  const float dx = bx - ax, dy = by - ay;
-  const float denom = sqrtf((ax * ax + ay * ay) * (bx * bx + by * by) * (dx * dx + dy * dy));
+  const float cross = t.ax * by - t.ay * bx;
+  const float dx = bx - t.ax, dy = by - t.ay;
+  const float denom = sqrtf(t.a2 * (bx * bx + by * by) * (dx * dx + dy * dy));
   const float curvature = denom > 0.0f ? 2.0f * cross / denom : 0.0f;  // circle through vertex, i, j
-  const float pointing = (in.zi - in.vz) / in.ri;
-  f[0] = (float)in.layer * 0.2f;
+  f[0] = (float)layer * 0.2f;
   f[1] = dphi * (1.0f / 0.3f);
-  f[2] = (tanL - pointing) * 20.0f;
+  f[2] = (tanL - t.pointing) * 20.0f;
   f[3] = curvature * 50.0f;
-  f[4] = in.ri * 0.025f;
-  f[5] = in.rj * 0.025f;
-  f[6] = (in.zi - tanL * in.ri - in.vz) * (1.0f / 0.3f);
+  f[4] = t.ri * 0.025f;
+  f[5] = rj * 0.025f;
+  f[6] = (t.zi - tanL * t.ri - vz) * (1.0f / 0.3f);
   f[7] = tanL * (1.0f / 1.5f);
-  f[8] = (float)(in.countJ < 32 ? in.countJ : 32) * (1.0f / 16.0f);
-  f[9] = (float)(in.countI < 32 ? in.countI : 32) * (1.0f / 16.0f);
+  f[8] = (float)(countJ < 32 ? countJ : 32) * (1.0f / 16.0f);
+  f[9] = t.countI;
   for (int k = 0; k < kMlpIn; ++k)
     if (!(f[k] == f[k])) f[k] = 0.0f;  // NaN guard (degenerate geometry)
+}
+
+ITSCA_HD void trackletFeatures(const TrackletFeatureInput& in, float f[kMlpIn])
+{
+  const TrackletSeed t = trackletSeed(in.xi, in.yi, in.zi, in.ri, in.phii, in.vx, in.vy, in.vz, in.countI);
+  candidateFeatures(t, in.layer, in.xj - in.vx, in.yj - in.vy, in.zj, in.rj, in.phij, in.vz, in.countJ, f);
 }
 
 }  // namespace itsca

@@ -153,30 +153,20 @@ __device__ __forceinline__ void packFeatures(const float f[kMlpIn], uint16_t* ro
   r[1] = make_uint4(packBf16x2(f[8], f[9]), 0u, 0u, 0u);
 }
 
-// Features of the tracklet candidate (pi of layer l, pj of layer l+1) of work
-// item w, whose vertex is (vx, vy, vz).
-__device__ __forceinline__ void candidateFeatures(const DevState& s, float vx, float vy, float vz, int l, int pi, int pj,
-                                                  float f[kMlpIn])
+// The cluster-i part of the features (pi: sorted cluster of layer l).
+__device__ __forceinline__ TrackletSeed seedOf(const DevState& s, float vx, float vy, float vz, int pi)
 {
-  const float4 a = s.crzp[pi], b = s.crzp[pj];  // (r, z, phi, 0)
-  TrackletFeatureInput in;
-  in.xi = s.cx[pi];
-  in.yi = s.cy[pi];
-  in.zi = a.y;
-  in.ri = a.x;
-  in.phii = a.z;
-  in.xj = s.cx[pj];
-  in.yj = s.cy[pj];
-  in.zj = b.y;
-  in.rj = b.x;
-  in.phij = b.z;
-  in.vx = vx;
-  in.vy = vy;
-  in.vz = vz;
-  in.layer = l;
-  in.countI = s.trkCnt[pi];
-  in.countJ = l + 1 < kTrackletLayers ? s.trkCnt[pj] : 0;
-  trackletFeatures(in, f);
+  const float4 a = s.crzp[pi];  // (r, z, phi, 0)
+  return trackletSeed(s.cx[pi], s.cy[pi], a.y, a.x, a.z, vx, vy, vz, s.trkCnt[pi]);
+}
+
+// Features of the candidate pj (sorted cluster of layer l+1) of seed t.
+__device__ __forceinline__ void featuresOf(const DevState& s, const TrackletSeed& t, float vx, float vy, float vz, int l,
+                                           int pj, float f[kMlpIn])
+{
+  const float4 b = s.crzp[pj];
+  candidateFeatures(t, l, s.cx[pj] - vx, s.cy[pj] - vy, b.y, b.x, b.z, vz,
+                    l + 1 < kTrackletLayers ? s.trkCnt[pj] : 0, f);
 }
 
 // Set bits of `bits` in [a, b).
@@ -251,7 +241,8 @@ __device__ __forceinline__ int segmentOf(const int* chunkStart, int S, int chunk
 // go to trkKept (trkCnt stays the raw count that the features read) and the
 // chunk sums are rewritten.
 // ---------------------------------------------------------------------------
-constexpr int kRound = 4096;  // candidate slots per round (LDS: 4 KB owners, 16 KB values)
+constexpr int kRound = 2048;  // candidate slots per round (LDS: 2 KB owners, 8 KB values)
+constexpr int kSeedFloats = 8;  // TrackletSeed
 
 template <int B>
 __global__ void __launch_bounds__(B) kTrackletsScore(DevState s, float threshold)
@@ -260,6 +251,7 @@ __global__ void __launch_bounds__(B) kTrackletsScore(DevState s, float threshold
   __shared__ int scratch[B / 64 + 1];
   __shared__ int exs[B + 1];
   __shared__ int keptBefore[B];
+  __shared__ float seeds[kSeedFloats][B];
   __shared__ uint8_t owner[kRound];
   __shared__ int candv[kRound];
   __shared__ uint32_t keep[kRound / 32];
@@ -285,6 +277,12 @@ __global__ void __launch_bounds__(B) kTrackletsScore(DevState s, float threshold
     keptBefore[threadIdx.x] = 0;
     const float vx = s.itemVertex[3 * w], vy = s.itemVertex[3 * w + 1], vz = s.itemVertex[3 * w + 2];
     const int rowBase = cur.base + first;
+    if (scored > 0) {  // the cluster-i features, once per cluster
+      static_assert(sizeof(TrackletSeed) == kSeedFloats * sizeof(float), "seed layout");
+      const TrackletSeed t = seedOf(s, vx, vy, vz, p);
+      const float* tv = reinterpret_cast<const float*>(&t);
+      for (int k = 0; k < kSeedFloats; ++k) seeds[k][threadIdx.x] = tv[k];
+    }
     for (int r0 = 0; r0 < total; r0 += kRound) {
       const int rn = min(kRound, total - r0);
       for (int k = max(0, r0 - ex); k < scored && ex + k < r0 + rn; ++k) owner[ex + k - r0] = (uint8_t)threadIdx.x;
@@ -296,7 +294,10 @@ __global__ void __launch_bounds__(B) kTrackletsScore(DevState s, float threshold
           const int c = owner[o];
           const int j = s.trkCand[(size_t)(rowBase + c) * kTrkSlots + (r0 + o - exs[c])];
           candv[o] = j;
-          candidateFeatures(s, vx, vy, vz, l, rowBase + c, next.base + j, f);
+          TrackletSeed t;
+          float* tv = reinterpret_cast<float*>(&t);
+          for (int k = 0; k < kSeedFloats; ++k) tv[k] = seeds[k][c];
+          featuresOf(s, t, vx, vy, vz, l, next.base + j, f);
         } else {
           for (int k = 0; k < kMlpIn; ++k) f[k] = 0.0f;
         }
@@ -390,7 +391,8 @@ __global__ void kCollect(DevState s, const uint8_t* label, int total, int stride
     const int pi = s.clSeg[w * kLayers + l].base + s.tFirst[t];
     const int pj = s.clSeg[w * kLayers + l + 1].base + s.tSecond[t];
     float f[kMlpIn];
-    candidateFeatures(s, s.itemVertex[3 * w], s.itemVertex[3 * w + 1], s.itemVertex[3 * w + 2], l, pi, pj, f);
+    const float vx = s.itemVertex[3 * w], vy = s.itemVertex[3 * w + 1], vz = s.itemVertex[3 * w + 2];
+    featuresOf(s, seedOf(s, vx, vy, vz, pi), vx, vy, vz, l, pj, f);
     packFeatures(f, outX + (size_t)(base + row) * kMlpInPad);
     outY[base + row] = label[t];
   }

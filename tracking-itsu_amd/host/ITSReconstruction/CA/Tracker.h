@@ -47,6 +47,15 @@ class Tracker<true> {
   // (the batch's work item order: event by event, vertex by vertex).
   PrimaryVertexContext getPrimaryVertexContext(int item) const;
 
+  // Learned tracklet filter (no counterpart in the reference): tracklet
+  // candidates whose network logit is below the threshold are dropped before
+  // cell building; every later cut is the exact one, so the roads found are a
+  // subset of the plain roads.  The weights file is the one written by
+  // `python -m itsca.filter_training` (1441 parameters and a threshold).
+  void useLearnedFilter(const std::string& weightsFile);
+  void useLearnedFilter(const std::vector<float>& params, float threshold);
+  void disableLearnedFilter();
+
  private:
   void runBatch(const std::vector<const Event*>& events);
   std::vector<Road> downloadRoads(int item) const;

@@ -1,7 +1,11 @@
 // host_capi.cpp -- C entry points of libitsca.so for event batches
 // (include/itsca_host.h).
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <new>
+#include <sstream>
+#include <string>
 
 #include "events.hpp"
 #include "itsca_host.h"
@@ -84,3 +88,34 @@ int itsca_events_copy(const itsca_events* ev, int event, float* x, float* y, flo
 }
 
 }  // extern "C"
+
+// C linkage from the declaration in itsca_host.h.
+int itsca_filter_load(const char* path, float* params, int n_params, float* threshold)
+{
+  if (!path || !params || !threshold || n_params < 1) return -1;
+  std::ifstream in(path);
+  if (!in) return -1;
+  std::stringstream ss;
+  ss << in.rdbuf();
+  const std::string text = ss.str();
+  size_t pos = text.find("\"params_f32_bits\"");
+  if (pos == std::string::npos) return -1;
+  pos = text.find('[', pos);
+  const size_t end = pos == std::string::npos ? std::string::npos : text.find(']', pos);
+  if (end == std::string::npos) return -1;
+  int n = 0;
+  for (size_t q = text.find('"', pos); q != std::string::npos && q < end; q = text.find('"', q)) {
+    const size_t r = text.find('"', q + 1);
+    if (r == std::string::npos || r > end || n >= n_params) return -1;
+    const uint32_t bits = (uint32_t)std::stoul(text.substr(q + 1, r - q - 1), nullptr, 16);
+    std::memcpy(&params[n++], &bits, 4);
+    q = r + 1;
+  }
+  if (n != n_params) return -1;
+  pos = text.find("\"threshold\"");
+  if (pos == std::string::npos) return -1;
+  pos = text.find(':', pos);
+  if (pos == std::string::npos) return -1;
+  *threshold = std::strtof(text.c_str() + pos + 1, nullptr);
+  return 0;
+}

@@ -1,11 +1,12 @@
 // itsca-track -- tracker CLI, the counterpart of the reference main.cpp
 // (main.cpp:27-153) on the gfx950 back end.
 //
-// usage: itsca-track <events.txt> [labels.txt]
+// usage: itsca-track <events.txt> [labels.txt] [--filter weights.json]
 // Runs a warm-up pass, then every event with its own timing, and prints the
 // road count per vertex and the average time per vertex.  With labels it
 // writes CorrectRoads.txt, DuplicateRoads.txt and FakeRoads.txt next to the
-// events file, as the reference does.
+// events file, as the reference does.  --filter turns on the learned tracklet
+// filter with the given weights file.
 #include <chrono>
 #include <cstdio>
 #include <iostream>
@@ -19,11 +20,20 @@ using namespace o2::ITS::CA;
 
 int main(int argc, char** argv)
 {
-  if (argc < 2) {
+  std::vector<std::string> args;
+  std::string filterWeights;
+  for (int a = 1; a < argc; ++a) {
+    const std::string v = argv[a];
+    if (v == "--filter" && a + 1 < argc)
+      filterWeights = argv[++a];
+    else
+      args.push_back(v);
+  }
+  if (args.empty()) {
     std::cerr << "Please, provide a data file." << std::endl;
     return 1;
   }
-  const std::string eventsFile = argv[1];
+  const std::string eventsFile = args[0];
   const std::string dir = eventsFile.find('/') == std::string::npos ? "" : eventsFile.substr(0, eventsFile.rfind('/') + 1);
   std::vector<Event> events = IOUtils::loadEventData(eventsFile);
   if (events.empty()) {
@@ -32,9 +42,9 @@ int main(int argc, char** argv)
   }
   std::vector<std::unordered_map<int, Label>> labels;
   std::ofstream correct, duplicate, fake;
-  const bool report = argc > 2;
+  const bool report = args.size() > 1;
   if (report) {
-    labels = IOUtils::loadLabels((int)events.size(), argv[2]);
+    labels = IOUtils::loadLabels((int)events.size(), args[1]);
     correct.open(dir + "CorrectRoads.txt");
     duplicate.open(dir + "DuplicateRoads.txt");
     fake.open(dir + "FakeRoads.txt");
@@ -43,6 +53,7 @@ int main(int argc, char** argv)
   for (const Event& e : events) vertices += e.getPrimaryVerticesNum();
   try {
     Tracker<true> tracker;
+    if (!filterWeights.empty()) tracker.useLearnedFilter(filterWeights);
     tracker.clustersToTracks(events[0]);  // warm-up
     double total = 0, minT = std::numeric_limits<double>::max(), maxT = 0;
     for (size_t i = 0; i < events.size(); ++i) {

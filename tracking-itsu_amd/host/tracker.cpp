@@ -11,6 +11,7 @@
 #include "ITSReconstruction/CA/IOUtils.h"
 #include "ITSReconstruction/CA/Tracker.h"
 #include "itsca.h"
+#include "itsca_host.h"
 
 namespace o2 {
 namespace ITS {
@@ -63,6 +64,24 @@ Tracker<true>::Tracker(int device)
 }
 
 Tracker<true>::~Tracker() { itsca_destroy(mCtx); }
+
+void Tracker<true>::useLearnedFilter(const std::string& weightsFile)
+{
+  std::vector<float> params(ITSCA_MLP_PARAMS);
+  float threshold = 0.0f;
+  if (itsca_filter_load(weightsFile.c_str(), params.data(), ITSCA_MLP_PARAMS, &threshold) != 0)
+    throw std::runtime_error("cannot read learned filter weights from " + weightsFile);
+  useLearnedFilter(params, threshold);
+}
+
+void Tracker<true>::useLearnedFilter(const std::vector<float>& params, float threshold)
+{
+  if (params.size() != ITSCA_MLP_PARAMS) throw std::runtime_error("learned filter: expected 1441 parameters");
+  check(mCtx, itsca_mlp_set_params(mCtx, params.data()), "itsca_mlp_set_params");
+  check(mCtx, itsca_set_filter(mCtx, 1, threshold), "itsca_set_filter");
+}
+
+void Tracker<true>::disableLearnedFilter() { check(mCtx, itsca_set_filter(mCtx, 0, 0.0f), "itsca_set_filter"); }
 
 void Tracker<true>::runBatch(const std::vector<const Event*>& events)
 {

@@ -67,7 +67,7 @@ TRACKLET_SLOTS = 64  # candidate slots per cluster between the tracklet count an
 MLP_FEATURES = 16
 HOST_SYMBOLS = [
     "itsca_events_generate", "itsca_events_load", "itsca_events_write", "itsca_events_free", "itsca_events_count",
-    "itsca_events_num_vertices", "itsca_events_layer_offsets", "itsca_events_copy",
+    "itsca_events_num_vertices", "itsca_events_layer_offsets", "itsca_events_copy", "itsca_filter_load",
 ]
 
 
@@ -146,6 +146,7 @@ def host_lib() -> ctypes.CDLL:
         lib.itsca_events_layer_offsets.argtypes = [P, I, ctypes.POINTER(ctypes.c_int32)]
         F, I32 = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32)
         lib.itsca_events_copy.argtypes = [P, I, F, F, F, I32, I32, F]
+        lib.itsca_filter_load.argtypes = [ctypes.c_char_p, F, I, F]
         _host = lib
     return _host
 
