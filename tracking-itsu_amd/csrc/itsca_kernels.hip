@@ -10,6 +10,8 @@
 // so each float expression rounds as the reference's x86-64 SSE build does.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "itsca_device.h"
 #include "itsca_kernels.h"
 #include "itsca_mlp_kernels.h"
@@ -364,21 +366,26 @@ __device__ __forceinline__ void forEachTracklet(const DevState& s, const int* ta
       jFirst = table[min(firstBin + (zFirst - win.x), maxBin)];
       jEnd = min(table[min(firstBin + (zLast - win.x) + 1, maxBin)], nNext);
     }
-    // Blocks of 8 candidates: all 8 loads are issued first (indices past the
-    // range are clamped to its last element and masked out), the cuts are
+    // Blocks of 8 candidates: all 8 loads are issued first, the cuts are
     // combined without short-circuit branches, and the accepted candidates are
-    // emitted in ascending order from a bit mask.
-    for (int jb = jFirst; jb < jEnd; jb += 8) {
+    // emitted in ascending order from a bit mask.  In the last, partial block
+    // of a row the indices past the range are clamped to its last element and
+    // masked out.  Candidate records are addressed as a uniform base plus an
+    // unsigned 32-bit byte offset.
+    const char* nextBytes = reinterpret_cast<const char*>(s.crzp + baseNext);
+    auto block = [&](int jb, auto full) {
+      constexpr bool kFull = decltype(full)::value;
       float4 q[8];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) q[k] = s.crzp[baseNext + min(jb + k, jEnd - 1)];
+      for (int k = 0; k < 8; ++k)
+        q[k] = *reinterpret_cast<const float4*>(nextBytes + 16u * (unsigned)(kFull ? jb + k : min(jb + k, jEnd - 1)));
       uint32_t mask = 0;
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         const float deltaZ = fabsf(tanLambda * (q[k].x - r) + z - q[k].y);
         const float deltaPhi = fabsf(phi - q[k].z);
         const bool ok = (deltaZ < maxDeltaZ) & ((deltaPhi < kPhiCut) | (fabsf(deltaPhi - kTwoPi) < kPhiCut)) &
-                        (jb + k < jEnd);
+                        (kFull || jb + k < jEnd);
         mask |= (ok ? 1u : 0u) << k;
       }
       while (mask != 0u) {
@@ -386,7 +393,10 @@ __device__ __forceinline__ void forEachTracklet(const DevState& s, const int* ta
         mask &= mask - 1u;
         emit(jb + k);
       }
-    }
+    };
+    int jb = jFirst;
+    for (; jb + 8 <= jEnd; jb += 8) block(jb, std::true_type{});
+    if (jb < jEnd) block(jb, std::false_type{});
     if (jLast < nNext) {  // the extra candidate of the inclusive bound
       const float4 q = s.crzp[baseNext + jLast];
       const float deltaZ = fabsf(tanLambda * (q.x - r) + z - q.y);
