@@ -39,17 +39,19 @@ def bytes_alg(c):
     return b
 
 
-def kernel_bytes(name, c):
+def kernel_bytes(name, c, kept=None):
     """Algorithmic HBM bytes of one launch of kernel `name` for one work item,
-    from the same record sizes (plain-CA counts): each kernel reads its inputs
-    once and writes its outputs once."""
+    from the same record sizes (plain-CA counts; `kept`: the learned form's
+    counts, for the filter): each kernel reads its inputs once and writes its
+    outputs once."""
     N, T, C, E, R = list(c.clusters), list(c.tracklets), list(c.cells), list(c.neighbours), c.roads
     if name == "kPrep":  # raw x, y, z, id, mc in; sorted cluster + 401-entry table out
         return sum(n * (20 + 36) for n in N) + 7 * 401 * 4
     if name == "kTrackletsCount":  # both layers' clusters in; one count per cluster out
         return sum((N[l] + N[l + 1]) * 36 + N[l] * 4 for l in range(6))
-    if name == "kTrackletsScore":  # per candidate: its slot and both clusters in; kept count per cluster out
-        return sum(T[l] * (4 + 2 * 36) + N[l] * 4 for l in range(6))
+    if name == "kTrackletsScore":  # candidate slots in, both cluster layers in once; kept slots and counts out
+        K = list(kept.tracklets) if kept is not None else T
+        return sum(T[l] * 4 + (N[l] + N[l + 1]) * 36 + K[l] * 4 + N[l] * 4 for l in range(6))
     if name == "kTrackletsFill":  # tracklet records and per-cluster offsets out
         return sum(T[l] * 16 + N[l] * 4 for l in range(6))
     if name == "kCellsCount":  # two tracklet layers and three cluster layers in; one count per tracklet out
@@ -248,6 +250,7 @@ def main():
     kernel_ms, launches, stage_ms = kernel_profile(tracker)
     print("# learned: %.1f events/s" % (args.batch * args.steps / learned_s), file=sys.stderr, flush=True)
 
+    kept = [tracker.counts(i) for i in range(tracker.num_items())]  # learned-form counts
     quality = ft.evaluate(tracker, events, threshold)  # plain and filtered roads of the same batch
     quality.pop("plain")
     tracker.set_filter(False)
@@ -259,7 +262,7 @@ def main():
     # roofline of the learned form's dominant kernel: algorithmic bytes of one
     # launch (one launch covers the whole batch) over its mean HIP-event time
     top = max(kernel_ms, key=kernel_ms.get)
-    top_bytes = sum(kernel_bytes(top, c) for c in counts)
+    top_bytes = sum(kernel_bytes(top, c, k) for c, k in zip(counts, kept))
     top_ms = kernel_ms[top] / launches.get(top, 1)
     achieved = top_bytes / (top_ms * 1e-3) / 1e9
 

@@ -335,23 +335,55 @@ __device__ __forceinline__ TrackletWindow trackletWindow(float z, float r, float
 // candidates and their order are unchanged.
 constexpr float kZSkipMargin = 0.01f;
 
-template <typename F>
-__device__ __forceinline__ void forEachTracklet(const DevState& s, const int* table, int baseNext, int nNext,
-                                                float4 geoNext, float z, float r, float phi, float vz, int l, F&& emit)
+// The search of one cluster: its window, and the z bins of each window row
+// whose clusters are tested.
+struct TrackletSearch {
+  TrackletWindow win;
+  float tanLambda, maxDeltaZ;
+  int zFirst, zLast;
+};
+
+__device__ __forceinline__ TrackletSearch trackletSearch(float4 geoNext, float z, float r, float phi, float vz, int l)
 {
-  const TrackletWindow win = trackletWindow(z, r, phi, vz, l);
-  if (win.empty) return;
-  const float tanLambda = (z - vz) / r;
-  const float maxDeltaZ = trackletMaxDeltaZ(l);
-  int zFirst = win.x, zLast = win.z;  // z bins of each row whose clusters are tested
-  if (geoNext.z != 0.0f) {
-    const float zA = tanLambda * (geoNext.x - r) + z, zB = tanLambda * (geoNext.y - r) + z;
-    const float zLo = fminf(zA, zB) - maxDeltaZ - kZSkipMargin, zHi = fmaxf(zA, zB) + maxDeltaZ + kZSkipMargin;
+  TrackletSearch t;
+  t.win = trackletWindow(z, r, phi, vz, l);
+  t.tanLambda = (z - vz) / r;
+  t.maxDeltaZ = trackletMaxDeltaZ(l);
+  t.zFirst = t.win.x;
+  t.zLast = t.win.z;
+  if (!t.win.empty && geoNext.z != 0.0f) {
+    const float zA = t.tanLambda * (geoNext.x - r) + z, zB = t.tanLambda * (geoNext.y - r) + z;
+    const float zLo = fminf(zA, zB) - t.maxDeltaZ - kZSkipMargin, zHi = fmaxf(zA, zB) + t.maxDeltaZ + kZSkipMargin;
     if (zLo > -1.0e4f && zHi < 1.0e4f && zLo <= zHi) {  // finite and ordered; else test every bin
-      zFirst = max(zFirst, zBinIndex(l + 1, zLo));
-      zLast = min(zLast, zBinIndex(l + 1, zHi));
+      t.zFirst = max(t.zFirst, zBinIndex(l + 1, zLo));
+      t.zLast = min(t.zLast, zBinIndex(l + 1, zHi));
     }
   }
+  return t;
+}
+
+// Where the candidate records of the next layer are read from: global memory
+// (a uniform base plus an unsigned 32-bit byte offset), or an LDS copy of some
+// window rows (kTrackletsCount): row ri = (row - rowRef) mod kPhiBins of the
+// copy holds the positions [lo[ri], hi[ri]) at LDS index j + off[ri].
+struct CandidateSource {
+  const char* global;
+  const float4* lds;
+  const int *lo, *hi, *off;
+  int rowRef;
+};
+
+// Visits every accepted candidate of the search, in the reference's order
+// (window row order, then ascending candidate position).  kStaged: the tested
+// bins of every row are in the LDS copy; the extra candidate of the inclusive
+// bound is read from LDS when it is in the copy, else from global memory.
+template <bool kStaged, typename F>
+__device__ __forceinline__ void searchRows(const TrackletSearch& t, const int* table, int nNext, float z, float r,
+                                           float phi, const CandidateSource& src, F&& emit)
+{
+  if (t.win.empty) return;
+  const TrackletWindow& win = t.win;
+  const float tanLambda = t.tanLambda, maxDeltaZ = t.maxDeltaZ;
   int row = win.y;
   for (int count = 0; count < win.rows; ++count) {
     const int firstBin = binIndex(win.x, row);
@@ -362,23 +394,29 @@ __device__ __forceinline__ void forEachTracklet(const DevState& s, const int* ta
     if (maxBin > kBins) maxBin = kBins;
     const int jLast = table[maxBin];  // the reference's inclusive upper bound
     int jFirst = jLast, jEnd = jLast;  // the clusters of bins [zFirst, zLast] of this row
-    if (zFirst <= zLast) {  // table positions clamped like maxBin (firstBin <= position <= maxBin)
-      jFirst = table[min(firstBin + (zFirst - win.x), maxBin)];
-      jEnd = min(table[min(firstBin + (zLast - win.x) + 1, maxBin)], nNext);
+    if (t.zFirst <= t.zLast) {  // table positions clamped like maxBin (firstBin <= position <= maxBin)
+      jFirst = table[min(firstBin + (t.zFirst - win.x), maxBin)];
+      jEnd = min(table[min(firstBin + (t.zLast - win.x) + 1, maxBin)], nNext);
     }
+    int ri = 0;
+    if (kStaged) {
+      ri = row - src.rowRef;
+      if (ri < 0) ri += kPhiBins;
+    }
+    const int off = kStaged ? src.off[ri] : 0;
+    auto load = [&](int j) {
+      return kStaged ? src.lds[j + off] : *reinterpret_cast<const float4*>(src.global + 16u * (unsigned)j);
+    };
     // Blocks of 8 candidates: all 8 loads are issued first, the cuts are
     // combined without short-circuit branches, and the accepted candidates are
     // emitted in ascending order from a bit mask.  In the last, partial block
     // of a row the indices past the range are clamped to its last element and
-    // masked out.  Candidate records are addressed as a uniform base plus an
-    // unsigned 32-bit byte offset.
-    const char* nextBytes = reinterpret_cast<const char*>(s.crzp + baseNext);
+    // masked out.
     auto block = [&](int jb, auto full) {
       constexpr bool kFull = decltype(full)::value;
       float4 q[8];
 #pragma unroll
-      for (int k = 0; k < 8; ++k)
-        q[k] = *reinterpret_cast<const float4*>(nextBytes + 16u * (unsigned)(kFull ? jb + k : min(jb + k, jEnd - 1)));
+      for (int k = 0; k < 8; ++k) q[k] = load(kFull ? jb + k : min(jb + k, jEnd - 1));
       uint32_t mask = 0;
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
@@ -398,7 +436,9 @@ __device__ __forceinline__ void forEachTracklet(const DevState& s, const int* ta
     for (; jb + 8 <= jEnd; jb += 8) block(jb, std::true_type{});
     if (jb < jEnd) block(jb, std::false_type{});
     if (jLast < nNext) {  // the extra candidate of the inclusive bound
-      const float4 q = s.crzp[baseNext + jLast];
+      const bool inCopy = kStaged && jLast >= src.lo[ri] && jLast < src.hi[ri];
+      const float4 q = inCopy ? src.lds[jLast + off]
+                              : *reinterpret_cast<const float4*>(src.global + 16u * (unsigned)jLast);
       const float deltaZ = fabsf(tanLambda * (q.x - r) + z - q.y);
       const float deltaPhi = fabsf(phi - q.z);
       if (deltaZ < maxDeltaZ && (deltaPhi < kPhiCut || fabsf(deltaPhi - kTwoPi) < kPhiCut)) emit(jLast);
@@ -408,12 +448,34 @@ __device__ __forceinline__ void forEachTracklet(const DevState& s, const int* ta
   }
 }
 
+// The search of one cluster with every candidate read from global memory
+// (the fill's path for clusters that overflowed their slots).
+template <typename F>
+__device__ __forceinline__ void forEachTracklet(const DevState& s, const int* table, int baseNext, int nNext,
+                                                float4 geoNext, float z, float r, float phi, float vz, int l, F&& emit)
+{
+  const TrackletSearch t = trackletSearch(geoNext, z, r, phi, vz, l);
+  const CandidateSource src{reinterpret_cast<const char*>(s.crzp + baseNext), nullptr, nullptr, nullptr, nullptr, 0};
+  searchRows<false>(t, table, nNext, z, r, phi, src, emit);
+}
+
+// LDS copy of the next layer for the tracklet count.  A chunk's clusters are
+// consecutive in (phi row, z bin) order, so the union of their windows is a
+// few phi rows times a range of z bins; those clusters of the next layer are
+// copied once per chunk and read from LDS by every cluster of the chunk.
+constexpr int kStageCap = 2048;  // next-layer clusters per chunk (32 KB)
+constexpr int kStageRows = 8;    // phi rows of the copy
+
 // Count: each cluster's accepted candidates, in emission order, are kept in
 // kTrkSlots scratch slots so that the fill does not repeat the window search.
 template <int B>
 __global__ void __launch_bounds__(B) kTrackletsCount(DevState s)
 {
   __shared__ int scratch[B / 64 + 1];
+  __shared__ float4 stage[kStageCap];
+  __shared__ int uni[5];  // min row offset, max row end, min z bin, max z bin, no-copy flag
+  __shared__ int rowLo[kStageRows], rowHi[kStageRows], rowOff[kStageRows], rowStart[kStageRows + 1];
+  __shared__ int copyRows, copyRef;  // rows of the LDS copy (0: none) and the window row of its first row
   const int S = s.W * kTrackletLayers;
   const int* chunkStart = s.chunkStart + kStTracklets * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
@@ -421,21 +483,111 @@ __global__ void __launch_bounds__(B) kTrackletsCount(DevState s)
     const int sg = segmentOfChunk(chunkStart, S, chunk);
     const int w = sg / kTrackletLayers, l = sg % kTrackletLayers;
     const Seg cur = s.clSeg[w * kLayers + l], next = s.clSeg[w * kLayers + l + 1];
-    const int i = (chunk - chunkStart[sg]) * B + threadIdx.x;
+    const int first = (chunk - chunkStart[sg]) * B;
+    const int i = first + threadIdx.x;
+    const int p = cur.base + i;
+    const int* table = s.table + (size_t)(w * kLayers + l + 1) * kTableSize;
+    TrackletSearch t;
+    float z = 0.0f, r = 0.0f, phi = 0.0f;
+    t.win.empty = true;
+    if (i < cur.count) {
+      z = s.cz[p];
+      r = s.cr[p];
+      phi = s.cphi[p];
+      t = trackletSearch(s.segGeo[w * kLayers + l + 1], z, r, phi, s.itemVertex[3 * w + 2], l);
+    }
+    // Union of the windows.  Rows are taken relative to the row before the
+    // chunk's first cluster: the chunk's clusters sit in consecutive rows, and
+    // a window starts at most one row before its cluster's row.
+    const int rowRef = (s.cbin[cur.base + first] / kPhiBins + kPhiBins - 1) % kPhiBins;
+    if (threadIdx.x == 0) {
+      uni[0] = kPhiBins;
+      uni[1] = -kPhiBins;
+      uni[2] = kZBins;
+      uni[3] = -1;
+      uni[4] = 0;
+    }
+    __syncthreads();
+    int dMin = kPhiBins, dMax = -kPhiBins, zMin = kZBins, zMax = -1, noCopy = 0;
+    if (!t.win.empty) {
+      if (t.win.y >= kPhiBins || t.win.rows > kStageRows) noCopy = 1;  // the phi == 2pi row, or a wide window
+      int d = t.win.y - rowRef;
+      if (d < 0) d += kPhiBins;
+      if (d >= kPhiBins / 2) d -= kPhiBins;
+      dMin = d;
+      dMax = d + t.win.rows - 1;
+      if (t.zFirst <= t.zLast) {
+        zMin = t.zFirst;
+        zMax = t.zLast;
+      }
+    }
+    for (int m = 32; m >= 1; m >>= 1) {
+      dMin = min(dMin, __shfl_xor(dMin, m, 64));
+      dMax = max(dMax, __shfl_xor(dMax, m, 64));
+      zMin = min(zMin, __shfl_xor(zMin, m, 64));
+      zMax = max(zMax, __shfl_xor(zMax, m, 64));
+      noCopy = max(noCopy, __shfl_xor(noCopy, m, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicMin(&uni[0], dMin);
+      atomicMax(&uni[1], dMax);
+      atomicMin(&uni[2], zMin);
+      atomicMax(&uni[3], zMax);
+      atomicMax(&uni[4], noCopy);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {  // the copy: rows [dMin, dMax] of the union, z bins [zMin, zMax] of each
+      const int nr = uni[1] - uni[0] + 1;
+      int rows = 0, ref = 0;
+      if (uni[4] == 0 && nr >= 1 && nr <= kStageRows) {
+        ref = ((rowRef + uni[0]) % kPhiBins + kPhiBins) % kPhiBins;
+        int total = 0;
+        for (int ri = 0; ri < nr; ++ri) {
+          const int row = (ref + ri) % kPhiBins;
+          int lo = 0, hi = 0;
+          if (uni[2] <= uni[3]) {
+            lo = table[binIndex(uni[2], row)];
+            hi = max(lo, min(table[binIndex(uni[3], row) + 1], next.count));
+          }
+          rowLo[ri] = lo;
+          rowHi[ri] = hi;
+          rowStart[ri] = total;
+          rowOff[ri] = total - lo;
+          total += hi - lo;
+        }
+        rowStart[nr] = total;
+        if (total <= kStageCap) rows = nr;
+      }
+      copyRows = rows;
+      copyRef = ref;
+    }
+    __syncthreads();
+    const int nrows = copyRows;
+    if (nrows > 0) {
+      const int total = rowStart[nrows];
+      for (int k = threadIdx.x; k < total; k += B) {
+        int ri = 0;
+        while (ri + 1 < nrows && rowStart[ri + 1] <= k) ++ri;
+        stage[k] = s.crzp[next.base + rowLo[ri] + (k - rowStart[ri])];
+      }
+      __syncthreads();
+    }
     int cnt = 0;
     if (i < cur.count) {
-      const int* table = s.table + (size_t)(w * kLayers + l + 1) * kTableSize;
-      const int p = cur.base + i;
       int* slots = s.trkCand + (size_t)p * kTrkSlots;
-      forEachTracklet(s, table, next.base, next.count, s.segGeo[w * kLayers + l + 1], s.cz[p], s.cr[p], s.cphi[p],
-                      s.itemVertex[3 * w + 2], l,
-                      [&](int j) {
-                        if (cnt < kTrkSlots) slots[cnt] = j;
-                        ++cnt;
-                      });
+      auto emit = [&](int j) {
+        if (cnt < kTrkSlots) slots[cnt] = j;
+        ++cnt;
+      };
+      const CandidateSource src{reinterpret_cast<const char*>(s.crzp + next.base), stage, rowLo, rowHi, rowOff,
+                                copyRef};
+      if (nrows > 0)
+        searchRows<true>(t, table, next.count, z, r, phi, src, emit);
+      else
+        searchRows<false>(t, table, next.count, z, r, phi, src, emit);
       s.trkCnt[p] = cnt;
     }
-    const int total = blockSum(cnt, scratch);
+    const int total = blockSum(cnt, scratch);  // its barriers also end this chunk's use of the copy
     if (threadIdx.x == 0) s.chunkSum[(size_t)kStTracklets * s.maxChunks + chunk] = total;
   }
 }
