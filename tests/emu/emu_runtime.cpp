@@ -66,6 +66,18 @@ unsigned long long ballot(bool pred)
   return m;
 }
 
+int shflInt(int v, int srcLane, int width)
+{
+  const int lane = (int)(tIdx.x % 64), wave = (int)(tIdx.x / 64);
+  std::barrier<>& b = *cur->waves[wave];
+  cur->shfl[tIdx.x] = v;
+  b.arrive_and_wait();
+  const int src = (lane / width) * width + (srcLane % width);
+  const int r = cur->shfl[wave * 64 + src];
+  b.arrive_and_wait();
+  return r;
+}
+
 int shflXorInt(int v, int mask, int width)
 {
   const int lane = (int)(tIdx.x % 64), wave = (int)(tIdx.x / 64);

@@ -42,6 +42,7 @@ void syncthreads();
 int shflUp(int v, unsigned d, int width);
 float shflXor(float v, int mask, int width);
 int shflXorInt(int v, int mask, int width);
+int shflInt(int v, int src, int width);
 unsigned long long ballot(bool pred);
 void waveBarrier();
 // v_mfma_f32_16x16x16_bf16 as documented for CDNA wave64: lane i holds
@@ -63,6 +64,7 @@ inline void __threadfence_block() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 inline int __shfl_up(int v, unsigned d, int width = 64) { return emu::shflUp(v, d, width); }
 inline float __shfl_xor(float v, int mask, int width = 64) { return emu::shflXor(v, mask, width); }
 inline int __shfl_xor(int v, int mask, int width = 64) { return emu::shflXorInt(v, mask, width); }
+inline int __shfl(int v, int src, int width = 64) { return emu::shflInt(v, src, width); }
 inline unsigned long long __ballot(int pred) { return emu::ballot(pred != 0); }
 inline int __float_as_int(float f)
 {

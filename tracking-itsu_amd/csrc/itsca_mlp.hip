@@ -169,49 +169,6 @@ __device__ __forceinline__ void featuresOf(const DevState& s, const TrackletSeed
                     l + 1 < kTrackletLayers ? s.trkCnt[pj] : 0, f);
 }
 
-// Set bits of `bits` in [a, b).
-__device__ __forceinline__ int popRange(const uint32_t* bits, int a, int b)
-{
-  int n = 0;
-  while (a < b) {
-    const int word = a >> 5, sh = a & 31;
-    const int len = min(b - a, 32 - sh);
-    uint32_t v = bits[word] >> sh;
-    if (len < 32) v &= (1u << len) - 1u;
-    n += __popc(v);
-    a += len;
-  }
-  return n;
-}
-
-// Exclusive scan of v over the workgroup (wave scans by __shfl_up, then the
-// wave totals); total receives the sum.
-__device__ __forceinline__ int waveBlockScan(int v, int* scratch, int& total)
-{
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nWaves = blockDim.x >> 6;
-  int inc = v;
-  for (int d = 1; d < 64; d <<= 1) {
-    const int t = __shfl_up(inc, d, 64);
-    if (lane >= d) inc += t;
-  }
-  if (lane == 63) scratch[wave] = inc;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int acc = 0;
-    for (int k = 0; k < nWaves; ++k) {
-      const int t = scratch[k];
-      scratch[k] = acc;
-      acc += t;
-    }
-    scratch[nWaves] = acc;
-  }
-  __syncthreads();
-  const int ex = inc - v + scratch[wave];
-  total = scratch[nWaves];
-  __syncthreads();
-  return ex;
-}
-
 // Segment (item, layer) of a chunk: the largest sg with chunkStart[sg] <= chunk.
 __device__ __forceinline__ int segmentOf(const int* chunkStart, int S, int chunk)
 {
@@ -227,35 +184,32 @@ __device__ __forceinline__ int segmentOf(const int* chunkStart, int S, int chunk
 }
 
 // ---------------------------------------------------------------------------
-// Pipeline filter.  Runs after the tracklet count, one chunk of B clusters per
-// iteration.  A scan of the candidate counts numbers every candidate slot of
-// the chunk (index o).  The slots are then taken in rounds of kRound: (1) an
-// LDS map o -> owning cluster; (2) every wavefront scores 64 candidates at a
-// time (features, BF16 MFMA network), keeps the candidate value in LDS and the
-// keep decision as a ballot bit; (3) every kept candidate is written back into
-// its cluster's slot row, in order, at position "kept before this round + kept
-// bits before it in this round".  A kept position never exceeds the
-// candidate's own position, and every slot of the round was read before the
-// barrier, so the in-place writes never overwrite a slot still to be read.
-// Clusters whose candidates overflowed their slots are kept whole.  Kept counts
-// go to trkKept (trkCnt stays the raw count that the features read) and the
-// chunk sums are rewritten.
+// Pipeline filter.  Runs after the tracklet count.  Every wavefront handles
+// the 64 clusters of its lanes on its own (wavefront barriers only): a
+// shuffle scan of the candidate counts numbers their candidate slots (index
+// o); the slots are taken in rounds of kWaveRound: (1) an LDS map o -> owning
+// lane; (2) 64 candidates at a time are scored (features, BF16 MFMA network),
+// the candidate values kept in LDS and the keep decisions as ballot words;
+// (3) each lane writes the kept candidates of its own cluster back into the
+// cluster's slot row, in order (a kept position never exceeds the candidate's
+// own position, and the round's slots were all read before, so nothing still
+// to be read is overwritten).  Clusters whose candidates overflowed their
+// slots are kept whole.  Kept counts go to trkKept (trkCnt stays the raw count
+// that the features read) and the chunk sums are rewritten.
 // ---------------------------------------------------------------------------
-constexpr int kRound = 2048;  // candidate slots per round (LDS: 2 KB owners, 8 KB values)
-constexpr int kSeedFloats = 8;  // TrackletSeed
+constexpr int kWaveRound = 1024;  // candidate slots per wavefront round
+constexpr int kSeedFloats = 8;    // TrackletSeed
 
 template <int B>
 __global__ void __launch_bounds__(B) kTrackletsScore(DevState s, float threshold)
 {
-  static_assert(B <= 256, "the owner map stores cluster indices as bytes");
-  __shared__ int scratch[B / 64 + 1];
-  __shared__ int exs[B + 1];
-  __shared__ int keptBefore[B];
+  constexpr int kWaves = B / 64;
+  __shared__ uint8_t owner[kWaves][kWaveRound];
+  __shared__ int candv[kWaves][kWaveRound];
+  __shared__ uint32_t keep[kWaves][kWaveRound / 32];
   __shared__ float seeds[kSeedFloats][B];
-  __shared__ uint8_t owner[kRound];
-  __shared__ int candv[kRound];
-  __shared__ uint32_t keep[kRound / 32];
-  __shared__ __attribute__((aligned(16))) uint16_t tiles[B / 64][64 * kMlpInPad];
+  __shared__ __attribute__((aligned(16))) uint16_t tiles[kWaves][64 * kMlpInPad];
+  __shared__ int waveKept[kWaves];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const MlpWave m = loadMlp(s.mlpW);
   const int S = s.W * kTrackletLayers;
@@ -270,33 +224,38 @@ __global__ void __launch_bounds__(B) kTrackletsScore(DevState s, float threshold
     const int p = cur.base + i;
     const int raw = i < cur.count ? s.trkCnt[p] : 0;
     const int scored = raw <= kTrkSlots ? raw : 0;
-    int total;
-    const int ex = waveBlockScan(scored, scratch, total);
-    exs[threadIdx.x] = ex;
-    if (threadIdx.x == 0) exs[B] = total;
-    keptBefore[threadIdx.x] = 0;
+    int inc = scored;  // wavefront inclusive scan
+    for (int d = 1; d < 64; d <<= 1) {
+      const int t = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += t;
+    }
+    const int ex = inc - scored;
+    const int total = __shfl(inc, 63, 64);
     const float vx = s.itemVertex[3 * w], vy = s.itemVertex[3 * w + 1], vz = s.itemVertex[3 * w + 2];
-    const int rowBase = cur.base + first;
+    const int rowBase = cur.base + first + wave * 64;  // slot row of the cluster of lane c: rowBase + c
     if (scored > 0) {  // the cluster-i features, once per cluster
       static_assert(sizeof(TrackletSeed) == kSeedFloats * sizeof(float), "seed layout");
       const TrackletSeed t = seedOf(s, vx, vy, vz, p);
       const float* tv = reinterpret_cast<const float*>(&t);
       for (int k = 0; k < kSeedFloats; ++k) seeds[k][threadIdx.x] = tv[k];
     }
-    for (int r0 = 0; r0 < total; r0 += kRound) {
-      const int rn = min(kRound, total - r0);
-      for (int k = max(0, r0 - ex); k < scored && ex + k < r0 + rn; ++k) owner[ex + k - r0] = (uint8_t)threadIdx.x;
-      __syncthreads();  // owners (and exs, keptBefore) visible; every keep word read below is written whole
-      for (int tile = wave; tile * 64 < rn; tile += B / 64) {
+    int kept = 0;
+    int* myRow = s.trkCand + (size_t)p * kTrkSlots;
+    for (int r0 = 0; r0 < total; r0 += kWaveRound) {
+      const int rn = min(kWaveRound, total - r0);
+      for (int k = max(0, r0 - ex); k < scored && ex + k < r0 + rn; ++k) owner[wave][ex + k - r0] = (uint8_t)lane;
+      waveSync();  // owners and seeds visible to the wavefront
+      for (int tile = 0; tile * 64 < rn; ++tile) {
         const int o = tile * 64 + lane;  // slot index within the round
+        const int c = o < rn ? owner[wave][o] : 0;
+        const int exc = __shfl(ex, c, 64);
         float f[kMlpIn];
         if (o < rn) {
-          const int c = owner[o];
-          const int j = s.trkCand[(size_t)(rowBase + c) * kTrkSlots + (r0 + o - exs[c])];
-          candv[o] = j;
+          const int j = s.trkCand[(size_t)(rowBase + c) * kTrkSlots + (r0 + o - exc)];
+          candv[wave][o] = j;
           TrackletSeed t;
           float* tv = reinterpret_cast<float*>(&t);
-          for (int k = 0; k < kSeedFloats; ++k) tv[k] = seeds[k][c];
+          for (int k = 0; k < kSeedFloats; ++k) tv[k] = seeds[k][wave * 64 + c];
           featuresOf(s, t, vx, vy, vz, l, next.base + j, f);
         } else {
           for (int k = 0; k < kMlpIn; ++k) f[k] = 0.0f;
@@ -304,31 +263,40 @@ __global__ void __launch_bounds__(B) kTrackletsScore(DevState s, float threshold
         packFeatures(f, &tiles[wave][lane * kMlpInPad]);
         waveSync();
         const float z = mlpForward64(m, tiles[wave]);
-        const unsigned long long kept = __ballot(o < rn && z >= threshold);
+        const unsigned long long kb = __ballot(o < rn && z >= threshold);
         if (lane == 0) {
-          keep[2 * tile] = (uint32_t)kept;
-          keep[2 * tile + 1] = (uint32_t)(kept >> 32);
+          keep[wave][2 * tile] = (uint32_t)kb;
+          keep[wave][2 * tile + 1] = (uint32_t)(kb >> 32);
         }
         waveSync();
       }
-      __syncthreads();
-      for (int o = threadIdx.x; o < rn; o += B) {
-        if ((keep[o >> 5] >> (o & 31)) & 1u) {
-          const int c = owner[o];
-          const int pos = keptBefore[c] + popRange(keep, max(exs[c] - r0, 0), o);
-          s.trkCand[(size_t)(rowBase + c) * kTrkSlots + pos] = candv[o];
+      // this lane's candidates in the round: [a, b); kept ones go to the row in order
+      const int a = max(ex - r0, 0), b = min(ex + scored - r0, rn);
+      for (int wd = a >> 5; a < b && wd <= (b - 1) >> 5; ++wd) {
+        uint32_t bits = keep[wave][wd];
+        const int lo = wd * 32 < a ? a - wd * 32 : 0, hi = (wd + 1) * 32 > b ? b - wd * 32 : 32;
+        if (lo > 0) bits &= ~((1u << lo) - 1u);
+        if (hi < 32) bits &= (1u << hi) - 1u;
+        while (bits != 0u) {
+          const int o = wd * 32 + __builtin_ctz(bits);
+          bits &= bits - 1u;
+          myRow[kept++] = candv[wave][o];
         }
       }
-      __syncthreads();  // positions above read keptBefore of this round
-      const int a = max(ex - r0, 0), b = min(ex + scored - r0, rn);
-      if (a < b) keptBefore[threadIdx.x] += popRange(keep, a, b);
-      __syncthreads();  // owner, candv, keep and keptBefore are reused by the next round
+      waveSync();  // owner, candv and keep are rebuilt by the next round
     }
-    const int kept = scored == raw ? keptBefore[threadIdx.x] : raw;
-    if (i < cur.count) s.trkKept[p] = kept;
-    int chunkKept;
-    waveBlockScan(i < cur.count ? kept : 0, scratch, chunkKept);
-    if (threadIdx.x == 0) s.chunkSum[(size_t)kStTracklets * s.maxChunks + chunk] = chunkKept;
+    const int keptCount = scored == raw ? kept : raw;
+    if (i < cur.count) s.trkKept[p] = keptCount;
+    int sum = i < cur.count ? keptCount : 0;
+    for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
+    if (lane == 0) waveKept[wave] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int t = 0;
+      for (int k = 0; k < kWaves; ++k) t += waveKept[k];
+      s.chunkSum[(size_t)kStTracklets * s.maxChunks + chunk] = t;
+    }
+    __syncthreads();  // waveKept and the seeds are reused by the next chunk
   }
 }
 
