@@ -1,7 +1,7 @@
 // hip_runtime.h -- CPU emulation shim for the kernel and runtime sources of
 // libitsca_hip.so (TEST INFRASTRUCTURE ONLY).
 //
-// Compiling the product's .hip files with g++ against this header runs every
+// Compiling the product's .hip files for the host with clang++ against this header runs every
 // kernel on the CPU: one std::thread per GPU thread, a std::barrier per
 // workgroup for __syncthreads, a per-wavefront exchange for __shfl_up, and
 // plain host memory for device buffers.  Built with AddressSanitizer, an

@@ -1,0 +1,77 @@
+"""The gfx950 kernels run on the CPU: the product's kernel and runtime sources
+compiled for the host against the HIP shim of tests/emu/ (one thread per GPU
+thread, the wavefront exchanges and the MFMA instructions emulated), with
+AddressSanitizer.  Checks the kernels' logic and bounds without a GPU: golden
+fixtures bit for bit, and the learned filter's paths."""
+import glob
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+EMU = os.path.join(ROOT, "tests", "emu")
+CLANG = "/opt/rocm/llvm/bin/clang++"
+
+SCRIPT = r"""
+import ctypes, os, sys
+from collections import Counter
+import numpy as np
+sys.path[:0] = [os.path.join(ROOT, "tracking-itsu_amd"), os.path.join(ROOT, "tests")]
+import itsca
+from golden_cases import fixture, small_cases
+lib = itsca.bind_hip(ctypes.CDLL(os.path.join(ROOT, "tests", "emu", "libitsca_emu.so")))
+cases = {n: p for n, p, _ in small_cases()}
+for name in ("small_s1", "sparse_s6"):
+    ev = itsca.load_events(cases[name])[0]
+    t = itsca.GpuTracker(persistent_blocks=4, lib=lib)
+    t.set_events([ev])
+    t.run()
+    assert itsca.dump_item(t, 0, ev.vertices[0]) == fixture(name, "stable", 0), name
+    if name == "small_s1":
+        plain = Counter(itsca.road_tracks(t, 0))
+        n = t.counts(0)
+        trk = [t.download(0, itsca.TRACKLETS, l, n.tracklets[l], 4)[:, :2] for l in range(6)]
+        total = sum(len(x) for x in trk)
+        t.mlp_init(3)
+        t.train_clear(total)
+        assert t.train_collect(1, 0) == total
+        z = t.train_scores(total)
+        t.set_filter(True, -3.0e38)
+        t.run()
+        assert itsca.dump_item(t, 0, ev.vertices[0]) == fixture(name, "stable", 0)
+        thr = float(np.quantile(z, 0.5))
+        t.set_filter(True, thr)
+        t.run()
+        assert not (Counter(itsca.road_tracks(t, 0)) - plain)
+        k = 0
+        for l in range(6):
+            raw = np.bincount(trk[l][:, 0], minlength=n.clusters[l])
+            keep = (z[k:k + len(trk[l])] >= thr) | (raw[trk[l][:, 0]] > itsca.TRACKLET_SLOTS)
+            k += len(trk[l])
+            got = t.download(0, itsca.TRACKLETS, l, t.counts(0).tracklets[l], 4)[:, :2]
+            assert np.array_equal(got, trk[l][keep]), l
+    t.close()
+    print(name, "ok", flush=True)
+print("ALL OK")
+"""
+
+
+def _asan_runtime():
+    found = sorted(glob.glob("/opt/rocm/llvm/lib/clang/*/lib/linux/libclang_rt.asan-x86_64.so"))
+    return found[-1] if found else None
+
+
+def test_kernels_under_cpu_emulation_with_asan():
+    rt = _asan_runtime()
+    if not os.path.exists(CLANG) or rt is None:
+        pytest.skip("ROCm clang++ with its AddressSanitizer runtime is needed")
+    build = subprocess.run(["make", "-C", EMU], capture_output=True, text=True, timeout=1200)
+    assert build.returncode == 0, build.stdout[-2000:] + build.stderr[-4000:]
+    env = dict(os.environ)
+    env["LD_PRELOAD"] = rt + (":" + env["LD_PRELOAD"] if env.get("LD_PRELOAD") else "")
+    env["ASAN_OPTIONS"] = "detect_leaks=0"
+    out = subprocess.run([sys.executable, "-c", "ROOT = %r\n" % ROOT + SCRIPT], env=env, capture_output=True,
+                         text=True, timeout=1200)
+    assert out.returncode == 0 and "ALL OK" in out.stdout, out.stdout[-2000:] + out.stderr[-6000:]
