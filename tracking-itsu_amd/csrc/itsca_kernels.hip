@@ -91,7 +91,9 @@ __device__ __forceinline__ int segmentOfChunk(const int* chunkStart, int S, int 
 // PrimaryVertexContext.cxx:31-142, Cluster.cxx:35-45.
 // One workgroup per (item, layer).
 // ---------------------------------------------------------------------------
-constexpr int kPrepLds = 16384;  // layers up to this many clusters keep the bucket array in LDS
+constexpr int kPrepLds = 16384;  // layers up to this many clusters are sorted through LDS
+constexpr int kPrepThreads = 1024;
+constexpr int kPrepPer = kPrepLds / kPrepThreads;  // clusters per thread on the LDS path
 
 __global__ void __launch_bounds__(1024) kPrep(DevState s)
 {
@@ -165,19 +167,66 @@ __global__ void __launch_bounds__(1024) kPrep(DevState s)
   // Bucket the original indices (order inside a bucket is arbitrary here)...
   // then each cluster's sorted position is table[bin] + (number of bucket
   // members with a smaller original index): a stable order, independent of the
-  // atomics.  Only the source index is scattered to that position.  The bucket
-  // array lives in LDS when the layer fits, else in global scratch.
+  // atomics.  When the layer fits in LDS, the bucket array is in LDS and the
+  // records are permuted through LDS; else the bucket array is in global
+  // scratch and only the source index is scattered to each sorted position.
   if (n <= kPrepLds) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) lBucket[atomicAdd(&hist[s.sBin[base + i]], 1)] = i;
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      const int b = s.sBin[base + i];
-      const int lo = tab[b];
-      const int hi = b < kBins ? tab[b + 1] : n;
-      int rank = 0;
-      for (int j = lo; j < hi; ++j) rank += lBucket[j] < i ? 1 : 0;
-      s.sInv[base + lo + rank] = i;
+    // The sorted position of each of this thread's clusters, i = threadIdx.x + k * 1024,
+    // two 16-bit positions per register (n <= 16384).
+    uint32_t pos2[kPrepPer / 2];
+#pragma unroll
+    for (int k = 0; k < kPrepPer / 2; ++k) pos2[k] = 0;
+#pragma unroll
+    for (int k = 0; k < kPrepPer; ++k) {
+      const int i = threadIdx.x + k * kPrepThreads;
+      if (i < n) {
+        const int b = s.sBin[base + i];
+        const int lo = tab[b];
+        const int hi = b < kBins ? tab[b + 1] : n;
+        int rank = 0;
+        for (int j = lo; j < hi; ++j) rank += lBucket[j] < i ? 1 : 0;
+        pos2[k >> 1] |= (uint32_t)(lo + rank) << (16 * (k & 1));
+      }
     }
+    __syncthreads();  // the bucket array becomes the staging buffer
+    // Each field goes through LDS: read in original order (coalesced), stored
+    // at its sorted position in LDS, written out in sorted order (coalesced).
+    // Addresses are a uniform base pointer plus an unsigned 32-bit byte offset.
+    auto ld = [](const void* b, int i) {
+      return *reinterpret_cast<const int*>(static_cast<const char*>(b) + 4u * (unsigned)i);
+    };
+    auto st = [](void* b, int i, int v) { *reinterpret_cast<int*>(static_cast<char*>(b) + 4u * (unsigned)i) = v; };
+    auto stageField = [&](const void* src) {
+#pragma unroll
+      for (int k = 0; k < kPrepPer; ++k) {
+        const int i = threadIdx.x + k * kPrepThreads;
+        if (i < n) lBucket[(pos2[k >> 1] >> (16 * (k & 1))) & 0xffffu] = ld(src, i);
+      }
+      __syncthreads();
+    };
+    auto field = [&](const void* src, void* dst) {
+      stageField(src);
+      for (int p = threadIdx.x; p < n; p += kPrepThreads) st(dst, p, lBucket[p]);
+      __syncthreads();
+    };
+    field(s.rx + rawBase, s.cx + base);
+    field(s.ry + rawBase, s.cy + base);
+    field(s.rid + rawBase, s.cid + base);
+    field(s.rmc + rawBase, s.cmc + base);
+    field(s.sBin + base, s.cbin + base);
+    field(s.sR + base, s.cr + base);
+    field(s.rz + rawBase, s.cz + base);
+    // A sorted position is always written by the same thread, which reads back
+    // its own r and z for the packed record.
+    stageField(s.sPhi + base);
+    for (int p = threadIdx.x; p < n; p += kPrepThreads) {
+      const float phi = __int_as_float(lBucket[p]);
+      st(s.cphi + base, p, __float_as_int(phi));
+      s.crzp[base + p] = make_float4(__int_as_float(ld(s.cr + base, p)), __int_as_float(ld(s.cz + base, p)), phi, 0.0f);
+    }
+    return;
   } else {
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
       const int pos = atomicAdd(&hist[s.sBin[base + i]], 1);
@@ -196,8 +245,8 @@ __global__ void __launch_bounds__(1024) kPrep(DevState s)
   }
   __threadfence_block();
   __syncthreads();
-  // ...and the sorted records are written position by position (coalesced
-  // stores, gathered loads).
+  // ...and (layers too large for the LDS path) the sorted records are written
+  // position by position (coalesced stores, gathered loads).
   for (int p = threadIdx.x; p < n; p += blockDim.x) {
     const int i = s.sInv[base + p];
     const int q = base + p;
