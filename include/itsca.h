@@ -105,21 +105,27 @@ itsca_status itsca_get_counts(itsca_ctx* ctx, int item, itsca_counts* out);
 itsca_status itsca_download(itsca_ctx* ctx, int item, int what, int layer, void* dst, size_t cap_bytes,
                             size_t* written);
 
-/* Launch shape of the stage kernels.  It never changes a result, only the time
- * (the autotuner, tracking-itsu_amd/itsca/autotune.py, searches it).
+/* Launch shape of the stage kernels and binning of the tracklet search index.
+ * None of it changes a result, only the time (itsca_autotune searches it).
  *   block_size:        workgroup size and chunk length, 64, 128 or 256
- *   persistent_blocks: workgroups of each persistent stage kernel (0 = 8 per CU) */
+ *   persistent_blocks: workgroups of each persistent stage kernel (0 = 8 per CU)
+ *   z_subbins:         z sub-bins per z bin of the search index, 1..8 (0 = 4);
+ *                      the search tests 20 * z_subbins z slices per layer instead
+ *                      of the reference's 20 bins.  The reference's own 20 x 20
+ *                      binning is kept for the results: it decides which
+ *                      candidate its inclusive bound adds (Tracker.cxx:84-89). */
 typedef struct itsca_tuning {
   int block_size;
   int persistent_blocks;
+  int z_subbins;
 } itsca_tuning;
 itsca_status itsca_set_tuning(itsca_ctx* ctx, const itsca_tuning* tuning);
 itsca_status itsca_get_tuning(itsca_ctx* ctx, itsca_tuning* tuning);
 
-/* Autotuner: times every launch shape (block size 64/128/256 x 4/8/16
- * workgroups per CU) over `repeats` runs of the uploaded batch, keeps the
- * fastest, and leaves the results of one run with it.  *best_ms gets its time
- * per run. */
+/* Autotuner: times every tuning (block size 64/128/256 x 4/8/16 workgroups
+ * per CU x 1/2/4/8 z sub-bins) over `repeats` runs of the uploaded batch,
+ * keeps the fastest, and leaves the results of one run with it.  *best_ms gets
+ * its time per run. */
 itsca_status itsca_autotune(itsca_ctx* ctx, int repeats, itsca_tuning* chosen, float* best_ms);
 
 /* Runs the pipeline as one captured HIP graph (captured on first use, replayed

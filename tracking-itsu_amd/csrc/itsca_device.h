@@ -37,6 +37,13 @@ constexpr int kTrkSlots = 64;  // per cluster (next-layer cluster positions)
 constexpr int kCellSlots = 4;  // per tracklet (next-layer tracklet indices)
 constexpr int kEdgeSlots = 8;  // per cell (next-layer cell indices)
 
+// Tracklet search index: inside every z-phi bin of at most kSubMaxBin
+// clusters, the clusters ordered by z sub-bin (zSubIndex, S = DevState::zSub
+// parts per bin, at most kMaxZSub), with the start of each sub-bin as a byte.
+constexpr int kMaxZSub = 8;
+constexpr int kSubMaxBin = 128;
+constexpr int kDefaultZSub = 4;
+
 enum Stage { kStTracklets = 0, kStCells = 1, kStEdges = 2, kStLevels = 3, kStRoads = 7, kNumStages = 8 };
 // kStLevels + (l-1) for cell layers 1..4 (levels of layer 0 are all 1).
 
@@ -63,6 +70,9 @@ struct DevState {
   const Seg* clSeg;  // W*7
   float *cx, *cy, *cz, *cphi, *cr;
   float4* crzp;  // (r, z, phi, 0) per sorted cluster: one load per tracklet candidate
+  float4* crzpS;  // the same records, z-sub-bin order inside each bin; w = offset in the bin (int bits)
+  uint4* subTab;  // per (item, layer) and bin: byte t = offset of sub-bin t in crzpS (t <= zSub)
+  int zSub;       // z sub-bins per bin of the search index (1..kMaxZSub)
   float4* segGeo;  // per (item, layer): (min r, max r, 1 if every z bin index lies in [0, kZBins) else 0, 0)
   int *cid, *cmc, *cbin;
   float *sPhi, *sR;  // prep scratch, original order

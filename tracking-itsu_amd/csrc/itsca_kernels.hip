@@ -97,6 +97,7 @@ constexpr int kPrepPer = kPrepLds / kPrepThreads;  // clusters per thread on the
 
 __global__ void __launch_bounds__(1024) kPrep(DevState s)
 {
+  static_assert(kBins * kMaxZSub <= kPrepLds, "sub-bin counters live in the bucket array");
   __shared__ int lBucket[kPrepLds];
   __shared__ int hist[kTableSize];
   __shared__ int tab[kTableSize];
@@ -226,7 +227,6 @@ __global__ void __launch_bounds__(1024) kPrep(DevState s)
       st(s.cphi + base, p, __float_as_int(phi));
       s.crzp[base + p] = make_float4(__int_as_float(ld(s.cr + base, p)), __int_as_float(ld(s.cz + base, p)), phi, 0.0f);
     }
-    return;
   } else {
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
       const int pos = atomicAdd(&hist[s.sBin[base + i]], 1);
@@ -242,24 +242,69 @@ __global__ void __launch_bounds__(1024) kPrep(DevState s)
       for (int j = lo; j < hi; ++j) rank += s.sTmp[base + j] < i ? 1 : 0;
       s.sInv[base + lo + rank] = i;
     }
+    __threadfence_block();
+    __syncthreads();
+    // ...and the sorted records are written position by position (coalesced
+    // stores, gathered loads).
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+      const int i = s.sInv[base + p];
+      const int q = base + p;
+      const float z = s.rz[rawBase + i], phi = s.sPhi[base + i], r = s.sR[base + i];
+      s.cx[q] = s.rx[rawBase + i];
+      s.cy[q] = s.ry[rawBase + i];
+      s.cz[q] = z;
+      s.cphi[q] = phi;
+      s.cr[q] = r;
+      s.crzp[q] = make_float4(r, z, phi, 0.0f);
+      s.cid[q] = s.rid[rawBase + i];
+      s.cmc[q] = s.rmc[rawBase + i];
+      s.cbin[q] = s.sBin[base + i];
+    }
   }
+  // The tracklet search index (no counterpart in the reference; only with
+  // more than one z sub-bin per bin).  Inside every bin of at most kSubMaxBin
+  // clusters the clusters are grouped by z sub-bin (a counting sort with LDS
+  // counters; the order inside a sub-bin is arbitrary and never matters) and
+  // written to crzpS with their offset in the bin; subTab gets the start of
+  // every sub-bin.  Larger bins and the phi == 2pi slot keep the sorted order.
+  // Every sorted position p is visited by the thread that wrote its record.
+  const int S = s.zSub;
+  if (S <= 1) return;
   __threadfence_block();
   __syncthreads();
-  // ...and (layers too large for the LDS path) the sorted records are written
-  // position by position (coalesced stores, gathered loads).
+  int* subCnt = lBucket;  // kBins * S counters, then cursors
+  for (int k = threadIdx.x; k < kBins * S; k += blockDim.x) subCnt[k] = 0;
+  __syncthreads();
+  auto smallBin = [&](int b) { return b < kBins && tab[b + 1] - tab[b] <= kSubMaxBin; };
   for (int p = threadIdx.x; p < n; p += blockDim.x) {
-    const int i = s.sInv[base + p];
-    const int q = base + p;
-    const float z = s.rz[rawBase + i], phi = s.sPhi[base + i], r = s.sR[base + i];
-    s.cx[q] = s.rx[rawBase + i];
-    s.cy[q] = s.ry[rawBase + i];
-    s.cz[q] = z;
-    s.cphi[q] = phi;
-    s.cr[q] = r;
-    s.crzp[q] = make_float4(r, z, phi, 0.0f);
-    s.cid[q] = s.rid[rawBase + i];
-    s.cmc[q] = s.rmc[rawBase + i];
-    s.cbin[q] = s.sBin[base + i];
+    const int b = s.cbin[base + p];
+    if (smallBin(b)) atomicAdd(&subCnt[b * S + zSubIndex(l, s.cz[base + p], b % kZBins, S)], 1);
+  }
+  __syncthreads();
+  // Byte t of a bin's entry: members with sub-bin < t (t = S and above: all).
+  uint4* subTab = s.subTab + (size_t)(w * kLayers + l) * kBins;
+  for (int b = threadIdx.x; b < kBins; b += blockDim.x) {
+    uint32_t e[4] = {0u, 0u, 0u, 0u};
+    if (smallBin(b)) {
+      int acc = 0;
+      for (int t = 0; t < S; ++t) {
+        const int c = subCnt[b * S + t];
+        subCnt[b * S + t] = acc;
+        e[t >> 2] |= (uint32_t)acc << (8 * (t & 3));
+        acc += c;
+      }
+      for (int t = S; t < 16; ++t) e[t >> 2] |= (uint32_t)acc << (8 * (t & 3));
+    }
+    subTab[b] = make_uint4(e[0], e[1], e[2], e[3]);
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < n; p += blockDim.x) {
+    const int b = s.cbin[base + p];
+    const int lo = b < kBins ? tab[b] : tab[kBins];
+    int q = p;
+    if (smallBin(b)) q = lo + atomicAdd(&subCnt[b * S + zSubIndex(l, s.cz[base + p], b % kZBins, S)], 1);
+    const float4 rec = s.crzp[base + p];
+    s.crzpS[base + q] = make_float4(rec.x, rec.y, rec.z, __int_as_float(p - lo));
   }
 }
 
@@ -390,6 +435,8 @@ struct TrackletSearch {
   TrackletWindow win;
   float tanLambda, maxDeltaZ;
   int zFirst, zLast;
+  float zLo, zHi;  // every candidate that can pass has z in [zLo, zHi] (when sub)
+  bool sub;        // the z range holds: bins, then z sub-bins, outside it are skipped
 };
 
 __device__ __forceinline__ TrackletSearch trackletSearch(float4 geoNext, float z, float r, float phi, float vz, int l)
@@ -400,39 +447,111 @@ __device__ __forceinline__ TrackletSearch trackletSearch(float4 geoNext, float z
   t.maxDeltaZ = trackletMaxDeltaZ(l);
   t.zFirst = t.win.x;
   t.zLast = t.win.z;
+  t.zLo = t.zHi = 0.0f;
+  t.sub = false;
   if (!t.win.empty && geoNext.z != 0.0f) {
     const float zA = t.tanLambda * (geoNext.x - r) + z, zB = t.tanLambda * (geoNext.y - r) + z;
     const float zLo = fminf(zA, zB) - t.maxDeltaZ - kZSkipMargin, zHi = fmaxf(zA, zB) + t.maxDeltaZ + kZSkipMargin;
     if (zLo > -1.0e4f && zHi < 1.0e4f && zLo <= zHi) {  // finite and ordered; else test every bin
       t.zFirst = max(t.zFirst, zBinIndex(l + 1, zLo));
       t.zLast = min(t.zLast, zBinIndex(l + 1, zHi));
+      t.zLo = zLo;
+      t.zHi = zHi;
+      t.sub = true;
     }
   }
   return t;
 }
 
 // Where the candidate records of the next layer are read from: global memory
-// (a uniform base plus an unsigned 32-bit byte offset), or an LDS copy of some
-// window rows (kTrackletsCount): row ri = (row - rowRef) mod kPhiBins of the
-// copy holds the positions [lo[ri], hi[ri]) at LDS index j + off[ri].
+// (a uniform base plus an unsigned 32-bit byte offset; `global` in sorted
+// order, `globalS` in search-index order, `subTab` the layer's sub-bin
+// starts), or an LDS copy of some window rows of the search-index order
+// (kTrackletsCount): row ri = (row - rowRef) mod kPhiBins of the copy holds
+// the positions of its searched bins at LDS index j + off[ri].
 struct CandidateSource {
   const char* global;
+  const char* globalS;
+  const uint4* subTab;
+  int zSub;
   const float4* lds;
-  const int *lo, *hi, *off;
+  const int* off;
   int rowRef;
 };
 
+// Byte t (0..15) of a sub-bin entry.
+__device__ __forceinline__ int subStart(uint4 e, int t)
+{
+  const uint32_t d = t < 4 ? e.x : (t < 8 ? e.y : (t < 12 ? e.z : e.w));
+  return (int)((d >> (8 * (t & 3))) & 0xffu);
+}
+
 // Visits every accepted candidate of the search, in the reference's order
-// (window row order, then ascending candidate position).  kStaged: the tested
-// bins of every row are in the LDS copy; the extra candidate of the inclusive
-// bound is read from LDS when it is in the copy, else from global memory.
+// (window row order, then ascending candidate position).
+//
+// With the z range known (t.sub) and more than one z sub-bin per bin, each
+// row is searched bin by bin over the
+// bins [zFirst, zLast].  In a bin of at most kSubMaxBin clusters only the z
+// sub-bins [zSubIndex(zLo), zSubIndex(zHi)] of the search index are tested: a
+// candidate that passes has z in [zLo, zHi], hence (zSubIndex being monotone)
+// a sub-bin in that range.  The accepted ones are collected as bits of their
+// offsets in the bin and emitted in ascending order, which is the sorted
+// order.  Larger bins keep the sorted order in the search index and are
+// tested whole.  Otherwise the row's bins [zFirst, zLast] are tested in
+// sorted order, as one range.  The reference's extra candidate (the inclusive
+// upper bound) is tested last, from the sorted records in global memory.
+// kStaged: the records of the tested bins are in the LDS copy (search-index
+// order with more than one z sub-bin, else sorted order).
 template <bool kStaged, typename F>
-__device__ __forceinline__ void searchRows(const TrackletSearch& t, const int* table, int nNext, float z, float r,
-                                           float phi, const CandidateSource& src, F&& emit)
+__device__ __forceinline__ void searchRows(const TrackletSearch& t, const int* table, int nNext, int l, float z,
+                                           float r, float phi, const CandidateSource& src, F&& emit)
 {
   if (t.win.empty) return;
   const TrackletWindow& win = t.win;
   const float tanLambda = t.tanLambda, maxDeltaZ = t.maxDeltaZ;
+  auto passes = [&](const float4& q) {
+    const float deltaZ = fabsf(tanLambda * (q.x - r) + z - q.y);
+    const float deltaPhi = fabsf(phi - q.z);
+    return (deltaZ < maxDeltaZ) & ((deltaPhi < kPhiCut) | (fabsf(deltaPhi - kTwoPi) < kPhiCut));
+  };
+  auto loadGlobal = [](const char* g, int j) { return *reinterpret_cast<const float4*>(g + 16u * (unsigned)j); };
+  uint64_t bits0 = 0, bits1 = 0;  // accepted offsets 0..63 and 64..127 of the current bin
+  // Tests the candidates [j0, j1) 8 at a time: all 8 loads are issued first,
+  // the cuts are combined without short-circuit branches, and in the last,
+  // partial block the indices past the range are clamped to its last element
+  // and masked out.  kBits: accepted candidates set the bit of their offset in
+  // the bin (record w); else they are emitted in ascending order.
+  auto scan = [&](int j0, int j1, auto load, auto bitsMode) {
+    constexpr bool kBits = decltype(bitsMode)::value;
+    auto block = [&](int jb, auto full) {
+      constexpr bool kFull = decltype(full)::value;
+      float4 q[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) q[k] = load(kFull ? jb + k : min(jb + k, j1 - 1));
+      uint32_t mask = 0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const bool ok = passes(q[k]) & (kFull || jb + k < j1);
+        if (kBits) {
+          const int o = __float_as_int(q[k].w);
+          bits0 |= (ok & (o < 64)) ? 1ull << (o & 63) : 0ull;
+          bits1 |= (ok & (o >= 64)) ? 1ull << (o & 63) : 0ull;
+        } else {
+          mask |= (ok ? 1u : 0u) << k;
+        }
+      }
+      if (!kBits) {
+        while (mask != 0u) {
+          const int k = __builtin_ctz(mask);
+          mask &= mask - 1u;
+          emit(jb + k);
+        }
+      }
+    };
+    int jb = j0;
+    for (; jb + 8 <= j1; jb += 8) block(jb, std::true_type{});
+    if (jb < j1) block(jb, std::false_type{});
+  };
   int row = win.y;
   for (int count = 0; count < win.rows; ++count) {
     const int firstBin = binIndex(win.x, row);
@@ -442,55 +561,47 @@ __device__ __forceinline__ void searchRows(const TrackletSearch& t, const int* t
     // lies within 2 ulp below 0.3).  Clamp to the table end.
     if (maxBin > kBins) maxBin = kBins;
     const int jLast = table[maxBin];  // the reference's inclusive upper bound
-    int jFirst = jLast, jEnd = jLast;  // the clusters of bins [zFirst, zLast] of this row
-    if (t.zFirst <= t.zLast) {  // table positions clamped like maxBin (firstBin <= position <= maxBin)
-      jFirst = table[min(firstBin + (t.zFirst - win.x), maxBin)];
-      jEnd = min(table[min(firstBin + (t.zLast - win.x) + 1, maxBin)], nNext);
-    }
-    int ri = 0;
+    int off = 0;
     if (kStaged) {
-      ri = row - src.rowRef;
+      int ri = row - src.rowRef;
       if (ri < 0) ri += kPhiBins;
+      off = src.off[ri];
     }
-    const int off = kStaged ? src.off[ri] : 0;
-    auto load = [&](int j) {
-      return kStaged ? src.lds[j + off] : *reinterpret_cast<const float4*>(src.global + 16u * (unsigned)j);
-    };
-    // Blocks of 8 candidates: all 8 loads are issued first, the cuts are
-    // combined without short-circuit branches, and the accepted candidates are
-    // emitted in ascending order from a bit mask.  In the last, partial block
-    // of a row the indices past the range are clamped to its last element and
-    // masked out.
-    auto block = [&](int jb, auto full) {
-      constexpr bool kFull = decltype(full)::value;
-      float4 q[8];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) q[k] = load(kFull ? jb + k : min(jb + k, jEnd - 1));
-      uint32_t mask = 0;
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const float deltaZ = fabsf(tanLambda * (q[k].x - r) + z - q[k].y);
-        const float deltaPhi = fabsf(phi - q[k].z);
-        const bool ok = (deltaZ < maxDeltaZ) & ((deltaPhi < kPhiCut) | (fabsf(deltaPhi - kTwoPi) < kPhiCut)) &
-                        (kFull || jb + k < jEnd);
-        mask |= (ok ? 1u : 0u) << k;
+    if (t.sub && src.zSub > 1) {
+      auto loadS = [&](int j) { return kStaged ? src.lds[j + off] : loadGlobal(src.globalS, j); };
+      for (int zb = t.zFirst; zb <= t.zLast; ++zb) {  // table positions clamped like maxBin
+        const int b = min(firstBin + (zb - win.x), maxBin);
+        const int bS = table[b];
+        const int bE = min(table[min(b + 1, maxBin)], nNext);
+        if (bE <= bS) continue;
+        if (bE - bS <= kSubMaxBin) {
+          const uint4 e = src.subTab[b];
+          const int j0 = bS + subStart(e, zSubIndex(l + 1, t.zLo, zb, src.zSub));
+          const int j1 = bS + subStart(e, zSubIndex(l + 1, t.zHi, zb, src.zSub) + 1);
+          bits0 = bits1 = 0;
+          scan(j0, j1, loadS, std::true_type{});
+          while (bits0 != 0ull) {
+            const int k = __builtin_ctzll(bits0);
+            bits0 &= bits0 - 1ull;
+            emit(bS + k);
+          }
+          while (bits1 != 0ull) {
+            const int k = __builtin_ctzll(bits1);
+            bits1 &= bits1 - 1ull;
+            emit(bS + 64 + k);
+          }
+        } else {
+          scan(bS, bE, loadS, std::false_type{});
+        }
       }
-      while (mask != 0u) {
-        const int k = __builtin_ctz(mask);
-        mask &= mask - 1u;
-        emit(jb + k);
-      }
-    };
-    int jb = jFirst;
-    for (; jb + 8 <= jEnd; jb += 8) block(jb, std::true_type{});
-    if (jb < jEnd) block(jb, std::false_type{});
+    } else if (t.zFirst <= t.zLast) {
+      const int jFirst = table[min(firstBin + (t.zFirst - win.x), maxBin)];
+      const int jEnd = min(table[min(firstBin + (t.zLast - win.x) + 1, maxBin)], nNext);
+      scan(jFirst, jEnd, [&](int j) { return kStaged ? src.lds[j + off] : loadGlobal(src.global, j); },
+           std::false_type{});
+    }
     if (jLast < nNext) {  // the extra candidate of the inclusive bound
-      const bool inCopy = kStaged && jLast >= src.lo[ri] && jLast < src.hi[ri];
-      const float4 q = inCopy ? src.lds[jLast + off]
-                              : *reinterpret_cast<const float4*>(src.global + 16u * (unsigned)jLast);
-      const float deltaZ = fabsf(tanLambda * (q.x - r) + z - q.y);
-      const float deltaPhi = fabsf(phi - q.z);
-      if (deltaZ < maxDeltaZ && (deltaPhi < kPhiCut || fabsf(deltaPhi - kTwoPi) < kPhiCut)) emit(jLast);
+      if (passes(loadGlobal(src.global, jLast))) emit(jLast);
     }
     ++row;
     if (row == kPhiBins) row = 0;
@@ -500,12 +611,19 @@ __device__ __forceinline__ void searchRows(const TrackletSearch& t, const int* t
 // The search of one cluster with every candidate read from global memory
 // (the fill's path for clusters that overflowed their slots).
 template <typename F>
-__device__ __forceinline__ void forEachTracklet(const DevState& s, const int* table, int baseNext, int nNext,
-                                                float4 geoNext, float z, float r, float phi, float vz, int l, F&& emit)
+__device__ __forceinline__ void forEachTracklet(const DevState& s, int w, int l, const int* table, int baseNext,
+                                                int nNext, float4 geoNext, float z, float r, float phi, float vz,
+                                                F&& emit)
 {
   const TrackletSearch t = trackletSearch(geoNext, z, r, phi, vz, l);
-  const CandidateSource src{reinterpret_cast<const char*>(s.crzp + baseNext), nullptr, nullptr, nullptr, nullptr, 0};
-  searchRows<false>(t, table, nNext, z, r, phi, src, emit);
+  const CandidateSource src{reinterpret_cast<const char*>(s.crzp + baseNext),
+                            reinterpret_cast<const char*>(s.crzpS + baseNext),
+                            s.subTab + (size_t)(w * kLayers + l + 1) * kBins,
+                            s.zSub,
+                            nullptr,
+                            nullptr,
+                            0};
+  searchRows<false>(t, table, nNext, l, z, r, phi, src, emit);
 }
 
 // LDS copy of the next layer for the tracklet count.  A chunk's clusters are
@@ -612,12 +730,13 @@ __global__ void __launch_bounds__(B) kTrackletsCount(DevState s)
     }
     __syncthreads();
     const int nrows = copyRows;
+    const float4* copySrc = s.zSub > 1 ? s.crzpS : s.crzp;
     if (nrows > 0) {
       const int total = rowStart[nrows];
       for (int k = threadIdx.x; k < total; k += B) {
         int ri = 0;
         while (ri + 1 < nrows && rowStart[ri + 1] <= k) ++ri;
-        stage[k] = s.crzp[next.base + rowLo[ri] + (k - rowStart[ri])];
+        stage[k] = copySrc[next.base + rowLo[ri] + (k - rowStart[ri])];
       }
       __syncthreads();
     }
@@ -628,12 +747,17 @@ __global__ void __launch_bounds__(B) kTrackletsCount(DevState s)
         if (cnt < kTrkSlots) slots[cnt] = j;
         ++cnt;
       };
-      const CandidateSource src{reinterpret_cast<const char*>(s.crzp + next.base), stage, rowLo, rowHi, rowOff,
+      const CandidateSource src{reinterpret_cast<const char*>(s.crzp + next.base),
+                                reinterpret_cast<const char*>(s.crzpS + next.base),
+                                s.subTab + (size_t)(w * kLayers + l + 1) * kBins,
+                                s.zSub,
+                                stage,
+                                rowOff,
                                 copyRef};
-      if (nrows > 0)
-        searchRows<true>(t, table, next.count, z, r, phi, src, emit);
+      if (nrows > 0 && (t.sub || s.zSub == 1))  // with sub-bins the copy serves only the bin-by-bin search
+        searchRows<true>(t, table, next.count, l, z, r, phi, src, emit);
       else
-        searchRows<false>(t, table, next.count, z, r, phi, src, emit);
+        searchRows<false>(t, table, next.count, l, z, r, phi, src, emit);
       s.trkCnt[p] = cnt;
     }
     const int total = blockSum(cnt, scratch);  // its barriers also end this chunk's use of the copy
@@ -714,8 +838,8 @@ __global__ void __launch_bounds__(B) kTrackletsFill(DevState s)
     if (cnt > kTrkSlots) {
       const int* table = s.table + (size_t)(w * kLayers + l + 1) * kTableSize;
       int k = chunkBase + ex;
-      forEachTracklet(s, table, next.base, next.count, s.segGeo[w * kLayers + l + 1], s.cz[p], s.cr[p], s.cphi[p],
-                      s.itemVertex[3 * w + 2], l,
+      forEachTracklet(s, w, l, table, next.base, next.count, s.segGeo[w * kLayers + l + 1], s.cz[p], s.cr[p],
+                      s.cphi[p], s.itemVertex[3 * w + 2],
                       [&](int j) { writeTracklet(s, k++, i, j, cur.base, next.base); });
     }
   }

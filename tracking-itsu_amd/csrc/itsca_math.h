@@ -240,6 +240,16 @@ ITSCA_HD int zBinIndex(int layer, float z)
 {
   return (int)((z + layerHalfZ(layer)) * inverseZBinSize(layer));
 }
+// z sub-bin of the tracklet search index (no counterpart in the reference):
+// z bin zb of layer l split into S parts.  For a fixed zb it is monotone in z
+// (truncation of a monotone float expression, then a clamp into [0, S)), which
+// is all the search relies on.
+ITSCA_HD int zSubIndex(int layer, float z, int zb, int S)
+{
+  const int f = (int)((z + layerHalfZ(layer)) * (inverseZBinSize(layer) * (float)S));
+  const int sub = f - zb * S;
+  return sub < 0 ? 0 : (sub >= S ? S - 1 : sub);
+}
 // IndexTableUtils.h:53-56
 ITSCA_HD int phiBinIndex(float phi) { return (int)(phi * kInversePhiBinSize); }
 // IndexTableUtils.h:58-62 (row stride PhiBins, clamped to the 401st slot)

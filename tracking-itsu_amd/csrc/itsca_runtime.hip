@@ -67,6 +67,7 @@ struct itsca_ctx {
   int cus = 0;
   int persistentBlocks = 0;
   int blockSize = 256;  // workgroup (and chunk) size of the stage kernels
+  int zSub = kDefaultZSub;  // z sub-bins per bin of the tracklet search index
   double capacityScale = 1.0;
   hipStream_t stream = nullptr;
   std::string err;
@@ -110,7 +111,8 @@ struct itsca_ctx {
   DBuf<Seg> dClSeg, dTrkSeg, dCelSeg, dEdgSeg, dRoadSeg;
   DBuf<float> cx, cy, cz, cphi, cr, sPhi, sR;
   DBuf<int> cid, cmc, cbin, sBin, sTmp, sInv, trkCnt, trkOff, trkKept, table;
-  DBuf<float4> crzp, segGeo;
+  DBuf<float4> crzp, crzpS, segGeo;
+  DBuf<uint4> subTab;
   DBuf<int> tFirst, tSecond, cellCnt, cellOff, trkCand, cellCand;
   DBuf<float2> tTP;
   DBuf<int> succCnt, succCand, eFrom, eTo, eTmp;
@@ -186,6 +188,8 @@ struct itsca_ctx {
     HIP_TRY(trkOff.reserve(SC));
     HIP_TRY(trkKept.reserve(SC));
     HIP_TRY(crzp.reserve(SC));
+    HIP_TRY(crzpS.reserve(SC));
+    HIP_TRY(subTab.reserve((size_t)std::max(W, 1) * kLayers * kBins));
     HIP_TRY(trkCand.reserve(SC * kTrkSlots));
     HIP_TRY(table.reserve((size_t)W * kLayers * kTableSize));
     HIP_TRY(segGeo.reserve((size_t)std::max(W, 1) * kLayers));
@@ -296,6 +300,9 @@ struct itsca_ctx {
     s.cphi = cphi.p;
     s.cr = cr.p;
     s.crzp = crzp.p;
+    s.crzpS = crzpS.p;
+    s.subTab = subTab.p;
+    s.zSub = zSub;
     s.segGeo = segGeo.p;
     s.cid = cid.p;
     s.cmc = cmc.p;
@@ -772,8 +779,11 @@ itsca_status itsca_set_tuning(itsca_ctx* ctx, const itsca_tuning* t)
   if (t->block_size != 64 && t->block_size != 128 && t->block_size != 256)
     return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_set_tuning: block_size must be 64, 128 or 256");
   if (t->persistent_blocks < 0) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_set_tuning: negative persistent_blocks");
+  if (t->z_subbins < 0 || t->z_subbins > kMaxZSub)
+    return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_set_tuning: z_subbins must be 0..8");
   ctx->blockSize = t->block_size;
   ctx->persistentBlocks = t->persistent_blocks > 0 ? t->persistent_blocks : 8 * ctx->cus;
+  ctx->zSub = t->z_subbins > 0 ? t->z_subbins : kDefaultZSub;
   ctx->dropGraph();
   return ITSCA_OK;
 }
@@ -783,12 +793,14 @@ itsca_status itsca_get_tuning(itsca_ctx* ctx, itsca_tuning* t)
   if (!ctx || !t) return ITSCA_ERR_ARGUMENT;
   t->block_size = ctx->blockSize;
   t->persistent_blocks = ctx->persistentBlocks;
+  t->z_subbins = ctx->zSub;
   return ITSCA_OK;
 }
 
-// Autotuner: times every launch shape on the uploaded batch and keeps the
-// fastest.  Launch shapes never change results (the chunk plans only decide
-// which workgroup computes which element), so every candidate is valid.
+// Autotuner: times every tuning on the uploaded batch and keeps the fastest.
+// No tuning changes a result: the chunk plans only decide which workgroup
+// computes which element, and the z sub-bins only decide which candidates the
+// tracklet search can skip without testing, so every candidate is valid.
 itsca_status itsca_autotune(itsca_ctx* ctx, int repeats, itsca_tuning* chosen, float* best_ms)
 {
   return guarded(ctx, "itsca_autotune", [&]() -> itsca_status {
@@ -800,12 +812,14 @@ itsca_status itsca_autotune(itsca_ctx* ctx, int repeats, itsca_tuning* chosen, f
     hipEvent_t a, b;
     HIP_TRY(hipEventCreate(&a));
     HIP_TRY(hipEventCreate(&b));
-    int bestBlock = ctx->blockSize, bestBlocks = ctx->persistentBlocks;
+    int bestBlock = ctx->blockSize, bestBlocks = ctx->persistentBlocks, bestSub = ctx->zSub;
     float best = -1.0f;
     for (int block : {64, 128, 256}) {
       for (int perCu : {4, 8, 16}) {
+       for (int zSub : {1, 2, 4, 8}) {
         ctx->blockSize = block;
         ctx->persistentBlocks = perCu * ctx->cus;
+        ctx->zSub = zSub;
         ctx->dropGraph();
         const itsca_status st = itsca_run(ctx);  // sizes the buffers, captures the graph if enabled
         if (st != ITSCA_OK) {
@@ -824,13 +838,16 @@ itsca_status itsca_autotune(itsca_ctx* ctx, int repeats, itsca_tuning* chosen, f
           best = ms;
           bestBlock = block;
           bestBlocks = ctx->persistentBlocks;
+          bestSub = zSub;
         }
+       }
       }
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     ctx->blockSize = bestBlock;
     ctx->persistentBlocks = bestBlocks;
+    ctx->zSub = bestSub;
     ctx->profiling = prof;
     ctx->useGraph = graph;
     ctx->dropGraph();
@@ -838,6 +855,7 @@ itsca_status itsca_autotune(itsca_ctx* ctx, int repeats, itsca_tuning* chosen, f
     if (chosen) {
       chosen->block_size = bestBlock;
       chosen->persistent_blocks = bestBlocks;
+      chosen->z_subbins = bestSub;
     }
     if (best_ms) *best_ms = best;
     return st;
