@@ -1,12 +1,14 @@
 #!/usr/bin/env python3
 """Throughput benchmark: plain GPU CA vs batched CA with the learned tracklet filter.
 
-Workload: central Pb-Pb-like events (8000 primaries + 2% secondaries + 2% noise,
-about 8.3e3 clusters per layer, one primary vertex), generated from seeds that
-the filter never saw in training, and uploaded to HBM before timing.  One step
+Workload: 256 central Pb-Pb-like events (8000 primaries + 2% secondaries + 2%
+noise, about 8.3e3 clusters per layer, one primary vertex), generated from the
+held-out seeds 20000..20255 that the filter never saw in training or threshold
+tuning, and uploaded to HBM before timing.  One step
 = one pass of the pipeline (index tables, tracklets, [filter], cells,
 neighbours, levels, roads) over a batch of such events, replayed as one HIP
-graph.  Both forms run on the same batch; `value` is the learned form, whose
+graph, with the launch shape and the search-index binning that the autotuner
+picks for that form.  Both forms run on the same batch; `value` is the learned form, whose
 roads are checked against the plain ones in the same run (recall, and tracks
 the plain CA does not find).  Prints one JSON line.
 """
@@ -67,15 +69,19 @@ def kernel_bytes(name, c, kept=None):
     return 0
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ca_central_batch64_hbm_traffic.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ca_central_batch256_hbm_traffic.md")
 
 
-def measured_traffic(kernel, batch):
+HELD_OUT_SEED = 20000  # first seed of the held-out central events
+TRAFFIC_BATCH = 256     # the workload of the counter runs in TRAFFIC_FILE
+
+
+def measured_traffic(kernel, batch, seed):
     """HBM bytes of one launch of `kernel` (learned form), from the calibrated
     FETCH_SIZE + WRITE_SIZE counter runs summarised in TRAFFIC_FILE; those runs
-    used this benchmark's default workload (64 central events, seeds 2..65), so
-    the figure is reported only for that workload."""
-    if batch != 64 or not os.path.exists(TRAFFIC_FILE):
+    used this benchmark's default workload (256 central events, seeds
+    20000..20255), so the figure is reported only for that workload."""
+    if batch != TRAFFIC_BATCH or seed != HELD_OUT_SEED or not os.path.exists(TRAFFIC_FILE):
         return None
     name = kernel.split("<")[0]
     with open(TRAFFIC_FILE) as f:
@@ -86,7 +92,7 @@ def measured_traffic(kernel, batch):
     return None
 
 
-def cpu_baseline(events_path, n_events):
+def cpu_baseline(events_path, n_events, seed):
     """The reference CPU tracker itself (built from its sources into
     oracle/_ref/), single thread, on a bounded sample of the same workload."""
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
@@ -94,8 +100,8 @@ def cpu_baseline(events_path, n_events):
         out = subprocess.run([exe, "time", events_path, "1"], check=True, capture_output=True, text=True).stdout
         r = json.loads(out.strip().splitlines()[-1])
         return {"value": round(r["events"] / r["seconds"], 3), "unit": "events/s", "cores": 1, "kind": "reference",
-                "sample": "%d central events (seeds 2..%d), single pass after one warm-up event, %.1f s"
-                          % (n_events, n_events + 1, r["seconds"])}
+                "sample": "%d central events (seeds %d..%d), single pass after one warm-up event, %.1f s"
+                          % (n_events, seed, seed + n_events - 1, r["seconds"])}
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle  # noqa: E402
     import itsca  # noqa: E402
@@ -155,6 +161,20 @@ def road_quality(tracker):
     return fake, correct
 
 
+def single_events(itsca, steps=50):
+    """One event per launch, plain form, graph replay: the latency of a small
+    event (100 tracks) and of one central event (8000 tracks)."""
+    out = {}
+    for key, preset, seed in (("small_100_tracks", 1, 1), ("central_8000_tracks", 2, HELD_OUT_SEED)):
+        t = itsca.GpuTracker(device=0)
+        t.set_events(itsca.generate_events(preset, seed, 1))
+        s = time_form(t, steps, 3)
+        t.close()
+        out[key] = {"ms_per_event": round(1e3 * s / steps, 4), "events_per_s": round(steps / s, 1),
+                    "form": "plain, one event per launch, hipGraph replay", "seed": seed}
+    return out
+
+
 def mixed_batch(itsca, steps):
     """256 events of log-uniform multiplicity in [50, 8000] tracks in one
     launch, plain form, launch shape chosen by the autotuner, graph replay."""
@@ -167,7 +187,8 @@ def mixed_batch(itsca, steps):
     out = {"workload": "256 events, multiplicity log-uniform in [50, 8000] tracks (seeds 1000..1255), one launch",
            "form": "plain, hipGraph replay", "events_per_s": round(256 * steps / s, 2),
            "ms_per_step": round(1e3 * s / steps, 4),
-           "tuning": {"block_size": tuning.block_size, "persistent_blocks": tuning.persistent_blocks}}
+           "tuning": {"block_size": tuning.block_size, "persistent_blocks": tuning.persistent_blocks,
+                      "z_subbins": tuning.z_subbins}}
     t.close()
     return out
 
@@ -202,7 +223,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=64, help="central events per step")
+    ap.add_argument("--batch", type=int, default=256, help="central events per step")
+    ap.add_argument("--seed", type=int, default=HELD_OUT_SEED, help="first seed of the central events")
     ap.add_argument("--cpu-events", type=int, default=200, help="events in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed-multiplicity and pile-up batches")
@@ -231,7 +253,7 @@ def main():
         del src, dst
 
     t_gen = time.time()
-    events = itsca.generate_events(2, 2, args.batch)
+    events = itsca.generate_events(2, args.seed, args.batch)
     tracker = itsca.GpuTracker(device=0)
     tracker.set_events(events)
     params, threshold, meta = ft.load_filter(args.weights)
@@ -239,13 +261,21 @@ def main():
     print("# generated and uploaded %d events in %.1f s" % (len(events), time.time() - t_gen), file=sys.stderr,
           flush=True)
 
+    def tuned(form):  # the autotuner's choice for this form on this batch (results never depend on it)
+        t, _ = tracker.autotune(2)
+        print("# %s: tuning block %d, %d workgroups, %d z sub-bins" % (form, t.block_size, t.persistent_blocks,
+                                                                       t.z_subbins), file=sys.stderr, flush=True)
+        return {"block_size": t.block_size, "persistent_blocks": t.persistent_blocks, "z_subbins": t.z_subbins}
+
     tracker.set_filter(False)
+    plain_tuning = tuned("plain")
     plain_s = time_form(tracker, args.steps, args.warmup)
     counts = [tracker.counts(i) for i in range(tracker.num_items())]  # plain-CA counts
     plain_kernels, _, plain_stages = kernel_profile(tracker)
     print("# plain: %.1f events/s" % (args.batch * args.steps / plain_s), file=sys.stderr, flush=True)
 
     tracker.set_filter(True, threshold)
+    learned_tuning = tuned("learned")
     learned_s = time_form(tracker, args.steps, args.warmup)
     kernel_ms, launches, stage_ms = kernel_profile(tracker)
     print("# learned: %.1f events/s" % (args.batch * args.steps / learned_s), file=sys.stderr, flush=True)
@@ -279,14 +309,16 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "central Pb-Pb-like events (8000 tracks, ~8.3e3 clusters/layer, seeds 2..%d), "
-                               "%d events per step, inputs resident in HBM, pipeline replayed as one HIP graph"
-                               % (args.batch + 1, args.batch),
+        "config": {"workload": "central Pb-Pb-like events (8000 tracks, ~8.3e3 clusters/layer, held-out seeds "
+                               "%d..%d), %d events per step, inputs resident in HBM, pipeline replayed as one HIP graph"
+                               % (args.seed, args.seed + args.batch - 1, args.batch),
                    "events_per_step": args.batch,
                    "form": "batched CA + learned tracklet filter (10-32-32-1, BF16 MFMA, FP32 accumulate)",
                    "filter_threshold": threshold,
-                   "filter_training": meta.get("training_events")},
+                   "filter_training": meta.get("training_events"),
+                   "tuning": learned_tuning},
         "plain": {"value": round(plain_eps, 2), "unit": "events/s", "ms_per_step": round(1e3 * plain_s / args.steps, 4),
+                  "tuning": plain_tuning,
                   "kernel_ms": {k: round(v, 4) for k, v in sorted(plain_kernels.items(), key=lambda kv: -kv[1])},
                   "stage_ms": {STAGES[k]: round(plain_stages[k], 4) for k in range(5)}},
         "learned_vs_plain": {"speedup": round(events_per_s / plain_eps, 4),
@@ -295,7 +327,8 @@ def main():
                              "tracklets_kept": round(quality["tracklets_kept"], 4),
                              "cells_kept": round(quality["cells_kept"], 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(top, args.batch),
+                     "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": measured_traffic(top, args.batch, args.seed),
                      "kernel": top, "kernel_ms": round(top_ms, 4), "kernel_bytes_alg": top_bytes,
                      "pipeline_bytes_alg_per_step": balg,
                      "pipeline_GBps": round(balg / (ms_per_step * 1e-3) / 1e9, 2)},
@@ -306,7 +339,8 @@ def main():
     }
     tracker.close()
     if not args.no_extra:
-        for key, fn in (("mixed_batch", lambda: mixed_batch(itsca, 5)),
+        for key, fn in (("single_event", lambda: single_events(itsca)),
+                        ("mixed_batch", lambda: mixed_batch(itsca, 5)),
                         ("pileup_batch", lambda: pileup_batch(itsca, ft, params, threshold, 3))):
             try:
                 line[key] = fn()
@@ -316,8 +350,8 @@ def main():
     if not args.no_cpu:
         with tempfile.TemporaryDirectory() as tmp:
             p = os.path.join(tmp, "cpu_sample.txt")
-            itsca.write_events(2, 2, args.cpu_events, p)
-            line["cpu_baseline"] = cpu_baseline(p, args.cpu_events)
+            itsca.write_events(2, args.seed, args.cpu_events, p)
+            line["cpu_baseline"] = cpu_baseline(p, args.cpu_events, args.seed)
     print(json.dumps(line), flush=True)
 
 

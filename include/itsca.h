@@ -109,7 +109,7 @@ itsca_status itsca_download(itsca_ctx* ctx, int item, int what, int layer, void*
  * None of it changes a result, only the time (itsca_autotune searches it).
  *   block_size:        workgroup size and chunk length, 64, 128 or 256
  *   persistent_blocks: workgroups of each persistent stage kernel (0 = 8 per CU)
- *   z_subbins:         z sub-bins per z bin of the search index, 1..8 (0 = 4);
+ *   z_subbins:         z sub-bins per z bin of the search index, 1..8 (0 = 8);
  *                      the search tests 20 * z_subbins z slices per layer instead
  *                      of the reference's 20 bins.  The reference's own 20 x 20
  *                      binning is kept for the results: it decides which

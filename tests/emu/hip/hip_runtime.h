@@ -79,6 +79,7 @@ inline float __int_as_float(int i)
   return f;
 }
 inline int __popc(unsigned v) { return __builtin_popcount(v); }
+inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
 
 // Wave-level builtins used by the MFMA code.
 #define __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, cbsz, abid, blgp) emu::mfma16x16x16bf16(a, b, c)

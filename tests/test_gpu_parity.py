@@ -79,6 +79,21 @@ def test_mixed_multiplicity_batch_matches_oracle(tracker):
         assert itsca.dump_item(tracker, i, ev.vertices[0]) == pyoracle.OracleVertex(ev, 0, stable=True).dump()
 
 
+def test_pileup_event_with_large_layers_matches_oracle(tracker):
+    """A pile-up event (4 vertices, up to 27 088 clusters per layer): layers
+    above the LDS sort size take the global-scratch sort, and dense bins exceed
+    the z sub-bin index (they are searched whole)."""
+    say("gpu pile-up event (large layers) vs oracle")
+    ev = itsca.generate_events(5, 40003, 1)[0]
+    assert int(np.diff(ev.layer_offsets).max()) > 16384
+    tracker.set_tuning(256, 0, 8)
+    tracker.set_events([ev])
+    tracker.run()
+    tracker.set_tuning(256, 0, 0)
+    for v in (0, 3):
+        assert itsca.dump_item(tracker, v, ev.vertices[v]) == pyoracle.OracleVertex(ev, v, stable=True).dump(), v
+
+
 def test_runs_are_deterministic(tracker):
     say("gpu determinism")
     evs = itsca.generate_events(2, 5, 2)

@@ -42,7 +42,7 @@ constexpr int kEdgeSlots = 8;  // per cell (next-layer cell indices)
 // parts per bin, at most kMaxZSub), with the start of each sub-bin as a byte.
 constexpr int kMaxZSub = 8;
 constexpr int kSubMaxBin = 128;
-constexpr int kDefaultZSub = 4;
+constexpr int kDefaultZSub = 8;
 
 enum Stage { kStTracklets = 0, kStCells = 1, kStEdges = 2, kStLevels = 3, kStRoads = 7, kNumStages = 8 };
 // kStLevels + (l-1) for cell layers 1..4 (levels of layer 0 are all 1).
@@ -76,7 +76,7 @@ struct DevState {
   float4* segGeo;  // per (item, layer): (min r, max r, 1 if every z bin index lies in [0, kZBins) else 0, 0)
   int *cid, *cmc, *cbin;
   float *sPhi, *sR;  // prep scratch, original order
-  int *sBin, *sTmp, *sInv;  // per-cluster scratch of the sort: bin, bucket members, source of each sorted position
+  int *sBin, *sInv;  // per-cluster scratch of the sort: bin, source of each sorted position
   int* table;  // W*7*401
   // tracklets
   int *trkCnt, *trkOff;  // per sorted cluster

@@ -165,15 +165,50 @@ __global__ void __launch_bounds__(1024) kPrep(DevState s)
     }
   }
   __syncthreads();
-  // Bucket the original indices (order inside a bucket is arbitrary here)...
-  // then each cluster's sorted position is table[bin] + (number of bucket
-  // members with a smaller original index): a stable order, independent of the
-  // atomics.  When the layer fits in LDS, the bucket array is in LDS and the
-  // records are permuted through LDS; else the bucket array is in global
-  // scratch and only the source index is scattered to each sorted position.
-  if (n <= kPrepLds) {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) lBucket[atomicAdd(&hist[s.sBin[base + i]], 1)] = i;
+  // Stable sorted positions, one tile of 1024 clusters (in index order) at a
+  // time: clusters of equal bin keep their index order.  Inside a wavefront,
+  // a cluster's rank among those of its bin comes from a bin match (one
+  // ballot per bit of the bin); the wavefronts' bin counts of the tile are
+  // scanned across wavefronts on top of each bin's running cursor (hist).
+  // Every thread calls it with the same t0; it returns the position of
+  // cluster t0 + threadIdx.x, or -1 past the end.
+  constexpr int kPrepWaves = kPrepThreads / 64;
+  static_assert(kPrepWaves * kTableSize <= kPrepLds, "per-wavefront bin counts live in the bucket array");
+  int* waveHist = lBucket;  // [kPrepWaves][kTableSize]
+  auto tilePositions = [&](int t0) -> int {
+    const int i = t0 + threadIdx.x;
+    const bool valid = i < n;
+    const int b = valid ? s.sBin[base + i] : 511;  // bins are 0..400 (9 bits)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned long long peers = ~0ull;
+#pragma unroll
+    for (int bit = 0; bit < 9; ++bit) {
+      const unsigned long long set = __ballot((b >> bit) & 1);
+      peers &= ((b >> bit) & 1) ? set : ~set;
+    }
+    const unsigned long long below = peers & ((1ull << lane) - 1ull);
+    for (int k = threadIdx.x; k < kPrepWaves * kTableSize; k += blockDim.x) waveHist[k] = 0;
     __syncthreads();
+    if (valid && below == 0ull) waveHist[wave * kTableSize + b] = __popcll(peers);
+    __syncthreads();
+    for (int bb = threadIdx.x; bb < kTableSize; bb += blockDim.x) {
+      int run = hist[bb];
+      for (int wv = 0; wv < kPrepWaves; ++wv) {
+        const int c = waveHist[wv * kTableSize + bb];
+        waveHist[wv * kTableSize + bb] = run;
+        run += c;
+      }
+      hist[bb] = run;
+    }
+    __syncthreads();
+    const int pos = valid ? waveHist[wave * kTableSize + b] + __popcll(below) : -1;
+    __syncthreads();  // the counts are cleared by the next tile
+    return pos;
+  };
+  // When the layer fits in LDS the positions stay in registers and the
+  // records are permuted through LDS; else the source of every sorted
+  // position goes to global scratch and the records are gathered.
+  if (n <= kPrepLds) {
     // The sorted position of each of this thread's clusters, i = threadIdx.x + k * 1024,
     // two 16-bit positions per register (n <= 16384).
     uint32_t pos2[kPrepPer / 2];
@@ -181,17 +216,11 @@ __global__ void __launch_bounds__(1024) kPrep(DevState s)
     for (int k = 0; k < kPrepPer / 2; ++k) pos2[k] = 0;
 #pragma unroll
     for (int k = 0; k < kPrepPer; ++k) {
-      const int i = threadIdx.x + k * kPrepThreads;
-      if (i < n) {
-        const int b = s.sBin[base + i];
-        const int lo = tab[b];
-        const int hi = b < kBins ? tab[b + 1] : n;
-        int rank = 0;
-        for (int j = lo; j < hi; ++j) rank += lBucket[j] < i ? 1 : 0;
-        pos2[k >> 1] |= (uint32_t)(lo + rank) << (16 * (k & 1));
+      if (k * kPrepThreads < n) {
+        const int pos = tilePositions(k * kPrepThreads);
+        if (pos >= 0) pos2[k >> 1] |= (uint32_t)pos << (16 * (k & 1));
       }
     }
-    __syncthreads();  // the bucket array becomes the staging buffer
     // Each field goes through LDS: read in original order (coalesced), stored
     // at its sorted position in LDS, written out in sorted order (coalesced).
     // Addresses are a uniform base pointer plus an unsigned 32-bit byte offset.
@@ -228,24 +257,14 @@ __global__ void __launch_bounds__(1024) kPrep(DevState s)
       s.crzp[base + p] = make_float4(__int_as_float(ld(s.cr + base, p)), __int_as_float(ld(s.cz + base, p)), phi, 0.0f);
     }
   } else {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      const int pos = atomicAdd(&hist[s.sBin[base + i]], 1);
-      s.sTmp[base + pos] = i;
+    for (int t0 = 0; t0 < n; t0 += blockDim.x) {
+      const int pos = tilePositions(t0);
+      if (pos >= 0) s.sInv[base + pos] = t0 + threadIdx.x;
     }
     __threadfence_block();
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      const int b = s.sBin[base + i];
-      const int lo = tab[b];
-      const int hi = b < kBins ? tab[b + 1] : n;
-      int rank = 0;
-      for (int j = lo; j < hi; ++j) rank += s.sTmp[base + j] < i ? 1 : 0;
-      s.sInv[base + lo + rank] = i;
-    }
-    __threadfence_block();
-    __syncthreads();
-    // ...and the sorted records are written position by position (coalesced
-    // stores, gathered loads).
+    // The sorted records are written position by position (coalesced stores,
+    // gathered loads).
     for (int p = threadIdx.x; p < n; p += blockDim.x) {
       const int i = s.sInv[base + p];
       const int q = base + p;
@@ -287,13 +306,15 @@ __global__ void __launch_bounds__(1024) kPrep(DevState s)
     uint32_t e[4] = {0u, 0u, 0u, 0u};
     if (smallBin(b)) {
       int acc = 0;
-      for (int t = 0; t < S; ++t) {
-        const int c = subCnt[b * S + t];
-        subCnt[b * S + t] = acc;
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
         e[t >> 2] |= (uint32_t)acc << (8 * (t & 3));
-        acc += c;
+        if (t < S) {
+          const int c = subCnt[b * S + t];
+          subCnt[b * S + t] = acc;
+          acc += c;
+        }
       }
-      for (int t = S; t < 16; ++t) e[t >> 2] |= (uint32_t)acc << (8 * (t & 3));
     }
     subTab[b] = make_uint4(e[0], e[1], e[2], e[3]);
   }
@@ -344,24 +365,52 @@ __global__ void __launch_bounds__(1024) kPlanTracklets(DevState s, int* counts)
 
 // Single-workgroup scan of a stage's chunk sums into global output offsets and
 // per-segment (base, count); flags overflow of the output capacity.
-__global__ void __launch_bounds__(1024) kScanChunks(DevState s, int stage, int S, Seg* outSeg, int cap,
-                                                    int statusSlot)
+constexpr int kScanThreads = 1024;
+constexpr int kScanPer = 16;
+
+__global__ void __launch_bounds__(kScanThreads) kScanChunks(DevState s, int stage, int S, Seg* outSeg, int cap,
+                                                            int statusSlot)
 {
-  __shared__ int scratch[1024 / 64 + 1];
+  __shared__ int scratch[kScanThreads / 64 + 1];
+  __shared__ int tile[kScanThreads * (kScanPer + 1)];
   const int* chunkStart = s.chunkStart + stage * (s.maxSegs + 1);
   const int* sums = s.chunkSum + (size_t)stage * s.maxChunks;
   int* offs = s.chunkOff + (size_t)stage * s.maxChunks;
   const int nChunks = chunkStart[S];
+  // Tiles of 1024 * kScanPer sums go through LDS (coalesced loads and
+  // stores); each thread scans kScanPer consecutive sums of the tile (an LDS
+  // row padded by one against bank conflicts), then one block scan of the
+  // thread totals.
   int carry = 0;
-  for (int c0 = 0; c0 < nChunks; c0 += blockDim.x) {
-    const int c = c0 + threadIdx.x;
-    const int v = c < nChunks ? sums[c] : 0;
+  auto slot = [](int idx) { return (idx / kScanPer) * (kScanPer + 1) + idx % kScanPer; };
+  for (int c0 = 0; c0 < nChunks; c0 += kScanThreads * kScanPer) {
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+      const int idx = k * kScanThreads + threadIdx.x;
+      tile[slot(idx)] = c0 + idx < nChunks ? sums[c0 + idx] : 0;
+    }
+    __syncthreads();
+    int* row = tile + threadIdx.x * (kScanPer + 1);
+    int mine = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+      const int t = row[k];
+      row[k] = mine;
+      mine += t;
+    }
     int total;
-    const int ex = blockExclusiveScan(v, scratch, total);
-    if (c < nChunks) offs[c] = carry + ex;
+    const int ex = carry + blockExclusiveScan(mine, scratch, total);  // its barriers order the row writes
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) row[k] += ex;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+      const int idx = k * kScanThreads + threadIdx.x;
+      if (c0 + idx < nChunks) offs[c0 + idx] = tile[slot(idx)];
+    }
     carry += total;
+    __syncthreads();  // the tile is refilled
   }
-  __syncthreads();
   const int grand = carry;
   // On overflow every segment of the stage is emptied, so no later stage reads
   // past the capacity; the host grows the buffers and runs again.

@@ -110,7 +110,7 @@ struct itsca_ctx {
   DBuf<int> rid, rmc, evLayerOff, dItemEvent;
   DBuf<Seg> dClSeg, dTrkSeg, dCelSeg, dEdgSeg, dRoadSeg;
   DBuf<float> cx, cy, cz, cphi, cr, sPhi, sR;
-  DBuf<int> cid, cmc, cbin, sBin, sTmp, sInv, trkCnt, trkOff, trkKept, table;
+  DBuf<int> cid, cmc, cbin, sBin, sInv, trkCnt, trkOff, trkKept, table;
   DBuf<float4> crzp, crzpS, segGeo;
   DBuf<uint4> subTab;
   DBuf<int> tFirst, tSecond, cellCnt, cellOff, trkCand, cellCand;
@@ -182,7 +182,6 @@ struct itsca_ctx {
     HIP_TRY(cmc.reserve(SC));
     HIP_TRY(cbin.reserve(SC));
     HIP_TRY(sBin.reserve(SC));
-    HIP_TRY(sTmp.reserve(SC));
     HIP_TRY(sInv.reserve(SC));
     HIP_TRY(trkCnt.reserve(SC));
     HIP_TRY(trkOff.reserve(SC));
@@ -310,7 +309,6 @@ struct itsca_ctx {
     s.sPhi = sPhi.p;
     s.sR = sR.p;
     s.sBin = sBin.p;
-    s.sTmp = sTmp.p;
     s.sInv = sInv.p;
     s.table = table.p;
     s.trkCnt = trkCnt.p;
