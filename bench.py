@@ -229,6 +229,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed-multiplicity and pile-up batches")
     ap.add_argument("--weights", default=None, help="learned filter weights (default: the shipped file)")
+    ap.add_argument("--tuning-plain", default=None, metavar="B,P,Z",
+                    help="fixed block size, persistent workgroups and z sub-bins of the plain form instead of the "
+                         "autotuner's choice (for profiler runs)")
+    ap.add_argument("--tuning-learned", default=None, metavar="B,P,Z", help="the same for the learned form")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy kernel of this many bytes (a reference dispatch of "
                          "known traffic for hardware-counter runs)")
@@ -262,7 +266,12 @@ def main():
           flush=True)
 
     def tuned(form):  # the autotuner's choice for this form on this batch (results never depend on it)
-        t, _ = tracker.autotune(2)
+        fixed = args.tuning_plain if form == "plain" else args.tuning_learned
+        if fixed:
+            tracker.set_tuning(*[int(x) for x in fixed.split(",")])
+            t = tracker.tuning()
+        else:
+            t, _ = tracker.autotune(2)
         print("# %s: tuning block %d, %d workgroups, %d z sub-bins" % (form, t.block_size, t.persistent_blocks,
                                                                        t.z_subbins), file=sys.stderr, flush=True)
         return {"block_size": t.block_size, "persistent_blocks": t.persistent_blocks, "z_subbins": t.z_subbins}
