@@ -1,10 +1,19 @@
 // emu_runtime.cpp -- the launch machinery and HIP API subset of the CPU
 // emulation shim (TEST INFRASTRUCTURE ONLY; see hip/hip_runtime.h).
-#include <barrier>
+//
+// A workgroup runs on one OS thread: every GPU thread is a fiber (ucontext)
+// with its own stack, and the fibers are resumed round-robin.  A barrier
+// (workgroup or wavefront) is a counter: a fiber that arrives before the last
+// one yields until the barrier's phase advances.  Stack switches are announced
+// to AddressSanitizer.
+#include <sanitizer/common_interface_defs.h>
+#include <sys/mman.h>
+#include <ucontext.h>
+
 #include <chrono>
+#include <cstdio>
 #include <cstdlib>
 #include <memory>
-#include <thread>
 #include <vector>
 
 #include "hip/hip_runtime.h"
@@ -14,9 +23,78 @@ thread_local uint3e tIdx, bIdx;
 thread_local dim3 bDim, gDim;
 
 namespace {
+constexpr size_t kStack = 256 * 1024;  // per fiber
+constexpr unsigned kMaxThreads = 1024;
+
+struct Fiber {
+  ucontext_t ctx;
+  unsigned tid = 0;
+  bool done = false;
+  void* fake = nullptr;  // ASan fake-stack handle while switched out
+};
+
+struct Sched {
+  ucontext_t ctx;
+  const void* bottom = nullptr;  // the scheduler's stack, as ASan reports it
+  size_t size = 0;
+  void* fake = nullptr;
+  char* stacks = nullptr;  // kMaxThreads * kStack, mapped once
+  Fiber fibers[kMaxThreads];
+  Fiber* cur = nullptr;
+  const std::function<void()>* body = nullptr;
+};
+Sched* sched()
+{
+  static Sched* s = [] {
+    Sched* p = new Sched();
+    void* m = mmap(nullptr, kMaxThreads * kStack, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE,
+                   -1, 0);
+    if (m == MAP_FAILED) {
+      std::perror("emu: mmap");
+      std::abort();
+    }
+    p->stacks = static_cast<char*>(m);
+    return p;
+  }();
+  return s;
+}
+
+// Fiber -> scheduler.
+void yield()
+{
+  Sched* s = sched();
+  Fiber* f = s->cur;
+  __sanitizer_start_switch_fiber(&f->fake, s->bottom, s->size);
+  swapcontext(&f->ctx, &s->ctx);
+  __sanitizer_finish_switch_fiber(f->fake, &s->bottom, &s->size);
+}
+
+void fiberEntry()
+{
+  Sched* s = sched();
+  __sanitizer_finish_switch_fiber(nullptr, &s->bottom, &s->size);
+  (*s->body)();
+  s->cur->done = true;
+  __sanitizer_start_switch_fiber(nullptr, s->bottom, s->size);  // this fiber's stack dies
+}
+
+struct Barrier {
+  int n = 0, count = 0, phase = 0;
+  void wait()
+  {
+    const int ph = phase;
+    if (++count == n) {
+      count = 0;
+      ++phase;
+      return;
+    }
+    while (phase == ph) yield();
+  }
+};
+
 struct Block {
-  std::unique_ptr<std::barrier<>> all;
-  std::vector<std::unique_ptr<std::barrier<>>> waves;
+  Barrier all;
+  std::vector<Barrier> waves;
   std::vector<int> shfl;
   std::vector<float> shflF;
   std::vector<emu_bf16x4> ma, mb;
@@ -26,80 +104,80 @@ struct Block {
 thread_local Block* cur = nullptr;
 }  // namespace
 
-void syncthreads() { cur->all->arrive_and_wait(); }
+void syncthreads() { cur->all.wait(); }
 
 int shflUp(int v, unsigned d, int width)
 {
   const int lane = (int)(tIdx.x % 64), wave = (int)(tIdx.x / 64);
-  std::barrier<>& b = *cur->waves[wave];
+  Barrier& b = cur->waves[wave];
   cur->shfl[tIdx.x] = v;
-  b.arrive_and_wait();
+  b.wait();
   const int src = lane - (int)d;
   const int r = (src >= 0 && lane / width == src / width) ? cur->shfl[wave * 64 + src] : v;
-  b.arrive_and_wait();
+  b.wait();
   return r;
 }
 
 float shflXor(float v, int mask, int width)
 {
   const int lane = (int)(tIdx.x % 64), wave = (int)(tIdx.x / 64);
-  std::barrier<>& b = *cur->waves[wave];
+  Barrier& b = cur->waves[wave];
   cur->shflF[tIdx.x] = v;
-  b.arrive_and_wait();
+  b.wait();
   const int src = lane ^ mask;
   const float r = (src < 64 && lane / width == src / width) ? cur->shflF[wave * 64 + src] : v;
-  b.arrive_and_wait();
+  b.wait();
   return r;
 }
 
 unsigned long long ballot(bool pred)
 {
   const int wave = (int)(tIdx.x / 64);
-  std::barrier<>& b = *cur->waves[wave];
+  Barrier& b = cur->waves[wave];
   cur->shfl[tIdx.x] = pred ? 1 : 0;
-  b.arrive_and_wait();
+  b.wait();
   unsigned long long m = 0;
   const int lanes = (int)std::min<unsigned>(64u, bDim.x - 64u * wave);
   for (int l = 0; l < lanes; ++l)
     if (cur->shfl[wave * 64 + l]) m |= 1ull << l;
-  b.arrive_and_wait();
+  b.wait();
   return m;
 }
 
 int shflInt(int v, int srcLane, int width)
 {
   const int lane = (int)(tIdx.x % 64), wave = (int)(tIdx.x / 64);
-  std::barrier<>& b = *cur->waves[wave];
+  Barrier& b = cur->waves[wave];
   cur->shfl[tIdx.x] = v;
-  b.arrive_and_wait();
+  b.wait();
   const int src = (lane / width) * width + (srcLane % width);
   const int r = cur->shfl[wave * 64 + src];
-  b.arrive_and_wait();
+  b.wait();
   return r;
 }
 
 int shflXorInt(int v, int mask, int width)
 {
   const int lane = (int)(tIdx.x % 64), wave = (int)(tIdx.x / 64);
-  std::barrier<>& b = *cur->waves[wave];
+  Barrier& b = cur->waves[wave];
   cur->shfl[tIdx.x] = v;
-  b.arrive_and_wait();
+  b.wait();
   const int src = lane ^ mask;
   const int r = (src < 64 && lane / width == src / width) ? cur->shfl[wave * 64 + src] : v;
-  b.arrive_and_wait();
+  b.wait();
   return r;
 }
 
-void waveBarrier() { cur->waves[tIdx.x / 64]->arrive_and_wait(); }
+void waveBarrier() { cur->waves[tIdx.x / 64].wait(); }
 
 emu_f32x4 mfma16x16x16bf16(emu_bf16x4 a, emu_bf16x4 b, emu_f32x4 c)
 {
   const int lane = (int)(tIdx.x % 64), wave = (int)(tIdx.x / 64);
-  std::barrier<>& bar = *cur->waves[wave];
+  Barrier& bar = cur->waves[wave];
   cur->ma[tIdx.x] = a;
   cur->mb[tIdx.x] = b;
   cur->mc[tIdx.x] = c;
-  bar.arrive_and_wait();
+  bar.wait();
   auto bf = [](short h) {
     const uint32_t u = (uint32_t)(uint16_t)h << 16;
     float f;
@@ -116,17 +194,17 @@ emu_f32x4 mfma16x16x16bf16(emu_bf16x4 a, emu_bf16x4 b, emu_f32x4 c)
       acc += bf(cur->ma[wave * 64 + row + 16 * (k / 4)][k % 4]) * bf(cur->mb[wave * 64 + col + 16 * (k / 4)][k % 4]);
     d[r] = acc;
   }
-  bar.arrive_and_wait();
+  bar.wait();
   return d;
 }
 
 emu_f32x4 mfma16x16x32bf16(emu_bf16x8 a, emu_bf16x8 b, emu_f32x4 c)
 {
   const int lane = (int)(tIdx.x % 64), wave = (int)(tIdx.x / 64);
-  std::barrier<>& bar = *cur->waves[wave];
+  Barrier& bar = cur->waves[wave];
   cur->ma8[tIdx.x] = a;
   cur->mb8[tIdx.x] = b;
-  bar.arrive_and_wait();
+  bar.wait();
   // A[m][k]: lane m + 16*(k/8), element k%8; B[k][n]: lane n + 16*(k/8), element k%8
   emu_f32x4 d;
   const int col = lane % 16, grp = lane / 16;
@@ -137,18 +215,25 @@ emu_f32x4 mfma16x16x32bf16(emu_bf16x8 a, emu_bf16x8 b, emu_f32x4 c)
       acc += (float)cur->ma8[wave * 64 + row + 16 * (k / 8)][k % 8] * (float)cur->mb8[wave * 64 + col + 16 * (k / 8)][k % 8];
     d[r] = acc;
   }
-  bar.arrive_and_wait();
+  bar.wait();
   return d;
 }
 
 void launch(dim3 grid, dim3 block, const std::function<void()>& body)
 {
   const unsigned nt = block.x;
+  if (nt == 0 || nt > kMaxThreads) {
+    std::fprintf(stderr, "emu: workgroup of %u threads\n", nt);
+    std::abort();
+  }
+  Sched* s = sched();
+  s->body = &body;
   for (unsigned b = 0; b < grid.x * grid.y * grid.z; ++b) {
     Block blk;
-    blk.all = std::make_unique<std::barrier<>>(nt);
+    blk.all.n = (int)nt;
     const unsigned nw = (nt + 63) / 64;
-    for (unsigned w = 0; w < nw; ++w) blk.waves.push_back(std::make_unique<std::barrier<>>(std::min(64u, nt - 64 * w)));
+    blk.waves.resize(nw);
+    for (unsigned w = 0; w < nw; ++w) blk.waves[w].n = (int)std::min(64u, nt - 64 * w);
     blk.shfl.assign(nt, 0);
     blk.shflF.assign(nt, 0.0f);
     blk.ma.assign(nt, emu_bf16x4{0, 0, 0, 0});
@@ -156,20 +241,38 @@ void launch(dim3 grid, dim3 block, const std::function<void()>& body)
     blk.mc.assign(nt, emu_f32x4{0, 0, 0, 0});
     blk.ma8.assign(nt, emu_bf16x8{});
     blk.mb8.assign(nt, emu_bf16x8{});
-    std::vector<std::thread> th;
-    th.reserve(nt);
-    for (unsigned t = 0; t < nt; ++t)
-      th.emplace_back([&, t, b]() {
+    cur = &blk;
+    bIdx.x = b % grid.x;
+    bIdx.y = (b / grid.x) % grid.y;
+    bIdx.z = b / (grid.x * grid.y);
+    bDim = block;
+    gDim = grid;
+    for (unsigned t = 0; t < nt; ++t) {
+      Fiber& f = s->fibers[t];
+      f.tid = t;
+      f.done = false;
+      f.fake = nullptr;
+      getcontext(&f.ctx);
+      f.ctx.uc_stack.ss_sp = s->stacks + (size_t)t * kStack;
+      f.ctx.uc_stack.ss_size = kStack;
+      f.ctx.uc_link = &s->ctx;
+      makecontext(&f.ctx, fiberEntry, 0);
+    }
+    unsigned left = nt;
+    while (left > 0) {
+      for (unsigned t = 0; t < nt; ++t) {
+        Fiber& f = s->fibers[t];
+        if (f.done) continue;
+        s->cur = &f;
         tIdx.x = t;
-        bIdx.x = b % grid.x;
-        bIdx.y = (b / grid.x) % grid.y;
-        bIdx.z = b / (grid.x * grid.y);
-        bDim = block;
-        gDim = grid;
-        cur = &blk;
-        body();
-      });
-    for (auto& x : th) x.join();
+        tIdx.y = tIdx.z = 0;
+        __sanitizer_start_switch_fiber(&s->fake, s->stacks + (size_t)t * kStack, kStack);
+        swapcontext(&s->ctx, &f.ctx);
+        __sanitizer_finish_switch_fiber(s->fake, nullptr, nullptr);
+        if (f.done) --left;
+      }
+    }
+    cur = nullptr;
   }
 }
 }  // namespace emu

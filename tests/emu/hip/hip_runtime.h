@@ -2,7 +2,7 @@
 // libitsca_hip.so (TEST INFRASTRUCTURE ONLY).
 //
 // Compiling the product's .hip files for the host with clang++ against this header runs every
-// kernel on the CPU: one std::thread per GPU thread, a std::barrier per
+// kernel on the CPU: one fiber per GPU thread, a cooperative barrier per
 // workgroup for __syncthreads, a per-wavefront exchange for __shfl_up, and
 // plain host memory for device buffers.  Built with AddressSanitizer, an
 // out-of-bounds access in a kernel becomes an ASan report.  Used to check

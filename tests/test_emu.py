@@ -1,8 +1,9 @@
 """The gfx950 kernels run on the CPU: the product's kernel and runtime sources
-compiled for the host against the HIP shim of tests/emu/ (one thread per GPU
+compiled for the host against the HIP shim of tests/emu/ (one fiber per GPU
 thread, the wavefront exchanges and the MFMA instructions emulated), with
 AddressSanitizer.  Checks the kernels' logic and bounds without a GPU: golden
-fixtures bit for bit, and the learned filter's paths."""
+fixtures bit for bit under two tunings, the learned filter's paths, and an
+event of 1831 clusters per layer against the oracle."""
 import glob
 import os
 import subprocess
@@ -18,7 +19,7 @@ SCRIPT = r"""
 import ctypes, os, sys
 from collections import Counter
 import numpy as np
-sys.path[:0] = [os.path.join(ROOT, "tracking-itsu_amd"), os.path.join(ROOT, "tests")]
+sys.path[:0] = [os.path.join(ROOT, "tracking-itsu_amd"), os.path.join(ROOT, "tests"), os.path.join(ROOT, "oracle")]
 import itsca
 from golden_cases import fixture, small_cases
 lib = itsca.bind_hip(ctypes.CDLL(os.path.join(ROOT, "tests", "emu", "libitsca_emu.so")))
@@ -54,6 +55,28 @@ for name in ("small_s1", "sparse_s6"):
             assert np.array_equal(got, trk[l][keep]), l
     t.close()
     print(name, "ok", flush=True)
+# another launch shape and an odd number of z sub-bins of the search index
+p = cases["pileup2_s7"]
+ev = itsca.load_events(p)[0]
+t = itsca.GpuTracker(persistent_blocks=4, lib=lib)
+t.set_tuning(128, 4, 3)
+t.set_events([ev])
+t.run()
+for v in range(len(ev.vertices)):
+    assert itsca.dump_item(t, v, ev.vertices[v]) == fixture("pileup2_s7", "stable", v), v
+t.close()
+print("pileup2_s7 block 128, 3 z sub-bins ok", flush=True)
+# 1831 clusters per layer: two tiles of the stable sort, dense bins searched by z sub-bin, against the oracle
+import pyoracle
+ev = itsca.generate_events(3, 3006, 1)[0]
+assert int(np.diff(ev.layer_offsets).min()) > 1024
+t = itsca.GpuTracker(persistent_blocks=8, lib=lib)
+t.set_tuning(256, 8, 8)
+t.set_events([ev])
+t.run()
+assert itsca.dump_item(t, 0, ev.vertices[0]) == pyoracle.OracleVertex(ev, 0, stable=True).dump()
+t.close()
+print("mixed event 3006 vs oracle ok", flush=True)
 print("ALL OK")
 """
 

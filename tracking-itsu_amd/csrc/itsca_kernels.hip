@@ -1055,7 +1055,7 @@ __global__ void __launch_bounds__(B) kCellsCount(DevState s)
         for (int k = 0; k < 8; ++k) up[k] = s.tTP[c.tnBase + min(ib + k, uEnd - 1)];
         uint32_t mask = 0;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) mask |= (cellCheapCuts(c, up[k]) & (ib + k < uEnd) ? 1u : 0u) << k;
+        for (int k = 0; k < 8; ++k) mask |= ((cellCheapCuts(c, up[k]) & (ib + k < uEnd)) ? 1u : 0u) << k;
         while (mask != 0u) {
           const int k = __builtin_ctz(mask);
           mask &= mask - 1u;
