@@ -16,12 +16,28 @@ typedef struct itsca_events itsca_events;
 /* preset: 1 small (100 tracks), 2 central (8000 tracks), 3 mixed multiplicity
  * (log-uniform 50..8000), 5 pile-up (4 collisions); event i uses seed + i. */
 itsca_events* itsca_events_generate(int preset, uint64_t seed, int num_events);
-/* Reads the reference event format; NULL if the file cannot be opened. */
+/* Reads the reference event format, or the binary side-car below (recognised
+ * by its magic); NULL if the file cannot be opened or is malformed. */
 itsca_events* itsca_events_load(const char* path);
 /* Writes the reference event format (and the labels format if labels_path is
  * not NULL); returns 0 on success. */
 int itsca_events_write(const itsca_events* ev, const char* events_path, const char* labels_path);
 void itsca_events_free(itsca_events* ev);
+
+/* Binary side-car (no counterpart in the reference): the same content as the
+ * text format, little endian ("ITSCAEV1", event count; per event the vertex
+ * count, 7 layer sizes, the vertices, then per layer x, y, z, alpha, cluster
+ * id, mc), read back by itsca_events_load through mmap.  Returns 0 on success. */
+int itsca_events_write_binary(const itsca_events* ev, const char* path);
+/* Reads a labels file (IOUtils.cxx:80-123 format) into the batch, one list
+ * per event (written back by itsca_events_write); returns 0 on success. */
+int itsca_events_load_labels(itsca_events* ev, const char* labels_path);
+/* Pile-up merger, the counterpart of generatePileUpData.cxx:44-207
+ * (benchmarks/scripts/root): consecutive events are merged until a merged
+ * event holds pile_up vertices; the MC labels of clusters and label rows that
+ * follow the k-th vertex of a merged event are offset by (k - 1) * 100000.
+ * Returns a new batch (free it with itsca_events_free), NULL on error. */
+itsca_events* itsca_events_merge_pileup(const itsca_events* ev, int pile_up);
 
 int itsca_events_count(const itsca_events* ev);
 int itsca_events_num_vertices(const itsca_events* ev, int event);

@@ -7,6 +7,13 @@
 // yields the same float32 event on the same libm.
 #include "events.hpp"
 
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <cstring>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -250,13 +257,24 @@ bool writeEvents(const EventBatch& batch, const std::string& path, std::string& 
     err = "cannot open " + path;
     return false;
   }
+  std::vector<std::pair<int32_t, int32_t>> order;  // (cluster id, layer * 2^24 + index)
   for (const RawEvent& ev : batch.events) {
     for (int v = 0; v < ev.numVertices(); ++v)
       std::fprintf(f, "-1\t%.9g\t%.9g\t%.9g\n", ev.vertices[3 * v], ev.vertices[3 * v + 1], ev.vertices[3 * v + 2]);
+    // Clusters in cluster-id order, which is the order of the file they were
+    // read from (layer by layer for generated events).
+    order.clear();
     for (int l = 0; l < kNumLayers; ++l)
-      for (int i = 0; i < ev.layerSize(l); ++i)
-        std::fprintf(f, "%d\t%.9g\t%.9g\t%.9g\t0\t0\t0\t%.9g\t%d\n", l, ev.x[l][i], ev.y[l][i], ev.z[l][i],
-                     ev.alpha[l][i], ev.mc[l][i]);
+      for (int i = 0; i < ev.layerSize(l); ++i) order.emplace_back(ev.id[l][i], (l << 24) | i);
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::pair<int32_t, int32_t>& a, const std::pair<int32_t, int32_t>& b) {
+                       return a.first < b.first;
+                     });
+    for (const auto& o : order) {
+      const int l = o.second >> 24, i = o.second & 0xffffff;
+      std::fprintf(f, "%d\t%.9g\t%.9g\t%.9g\t0\t0\t0\t%.9g\t%d\n", l, ev.x[l][i], ev.y[l][i], ev.z[l][i],
+                   ev.alpha[l][i], ev.mc[l][i]);
+    }
   }
   std::fclose(f);
   return true;
@@ -281,6 +299,7 @@ bool writeLabels(const EventBatch& batch, const std::string& path, std::string& 
 
 bool loadEvents(const std::string& path, EventBatch& out, std::string& err)
 {
+  if (isEventsBinary(path)) return loadEventsBinary(path, out, err);
   std::ifstream in(path);
   if (!in) {
     err = "cannot open " + path;
@@ -313,6 +332,205 @@ bool loadEvents(const std::string& path, EventBatch& out, std::string& err)
     }
   }
   out.labels.assign(out.events.size(), {});
+  return true;
+}
+
+bool loadLabelRows(const std::string& path, EventBatch& batch, std::string& err)
+{
+  std::ifstream in(path);
+  if (!in) {
+    err = "cannot open " + path;
+    return false;
+  }
+  batch.labels.assign(batch.events.size(), {});
+  std::string line;
+  std::getline(in, line);  // header (IOUtils.cxx:92)
+  size_t e = 0;
+  while (std::getline(in, line)) {
+    std::istringstream ss(line);
+    McLabel m;
+    if (!(ss >> m.mc)) continue;
+    if (m.mc == -1) {
+      ++e;
+      continue;
+    }
+    if (!(ss >> m.pt >> m.phi >> m.eta >> m.pdg >> m.nClusters)) continue;
+    if (e < batch.labels.size()) batch.labels[e].push_back(m);
+  }
+  return true;
+}
+
+namespace {
+constexpr char kMagic[8] = {'I', 'T', 'S', 'C', 'A', 'E', 'V', '1'};
+
+template <typename T>
+void put(std::vector<char>& b, const T* p, size_t n)
+{
+  const char* c = reinterpret_cast<const char*>(p);
+  b.insert(b.end(), c, c + n * sizeof(T));
+}
+}  // namespace
+
+bool writeEventsBinary(const EventBatch& batch, const std::string& path, std::string& err)
+{
+  std::vector<char> b;
+  put(b, kMagic, 8);
+  const uint32_t ne = (uint32_t)batch.events.size();
+  put(b, &ne, 1);
+  for (const RawEvent& ev : batch.events) {
+    uint32_t head[1 + kNumLayers];
+    head[0] = (uint32_t)ev.numVertices();
+    for (int l = 0; l < kNumLayers; ++l) head[1 + l] = (uint32_t)ev.layerSize(l);
+    put(b, head, 1 + kNumLayers);
+    put(b, ev.vertices.data(), ev.vertices.size());
+    for (int l = 0; l < kNumLayers; ++l) {
+      const size_t n = (size_t)ev.layerSize(l);
+      put(b, ev.x[l].data(), n);
+      put(b, ev.y[l].data(), n);
+      put(b, ev.z[l].data(), n);
+      put(b, ev.alpha[l].data(), n);
+      put(b, ev.id[l].data(), n);
+      put(b, ev.mc[l].data(), n);
+    }
+  }
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) {
+    err = "cannot open " + path;
+    return false;
+  }
+  const bool ok = std::fwrite(b.data(), 1, b.size(), f) == b.size();
+  std::fclose(f);
+  if (!ok) err = "cannot write " + path;
+  return ok;
+}
+
+bool isEventsBinary(const std::string& path)
+{
+  char m[8];
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) return false;
+  const bool ok = std::fread(m, 1, 8, f) == 8 && std::memcmp(m, kMagic, 8) == 0;
+  std::fclose(f);
+  return ok;
+}
+
+bool loadEventsBinary(const std::string& path, EventBatch& out, std::string& err)
+{
+  const int fd = open(path.c_str(), O_RDONLY);
+  if (fd < 0) {
+    err = "cannot open " + path;
+    return false;
+  }
+  struct stat st;
+  if (fstat(fd, &st) != 0 || st.st_size < 12) {
+    close(fd);
+    err = "not an event file: " + path;
+    return false;
+  }
+  const size_t size = (size_t)st.st_size;
+  void* map = mmap(nullptr, size, PROT_READ, MAP_PRIVATE, fd, 0);
+  close(fd);
+  if (map == MAP_FAILED) {
+    err = "cannot map " + path;
+    return false;
+  }
+  const char* p = static_cast<const char*>(map);
+  size_t at = 0;
+  bool ok = std::memcmp(p, kMagic, 8) == 0;
+  at = 8;
+  auto take = [&](void* dst, size_t bytes) {
+    if (!ok || at + bytes > size) {
+      ok = false;
+      return;
+    }
+    if (bytes) std::memcpy(dst, p + at, bytes);
+    at += bytes;
+  };
+  uint32_t ne = 0;
+  take(&ne, 4);
+  out.events.clear();
+  out.labels.clear();
+  if (ok && ne > size / (4 * (1 + kNumLayers))) ok = false;  // more events than the file can hold
+  if (ok) out.events.resize(ne);
+  for (uint32_t e = 0; ok && e < ne; ++e) {
+    RawEvent& ev = out.events[e];
+    uint32_t head[1 + kNumLayers];
+    take(head, sizeof(head));
+    if (!ok) break;
+    for (int k = 0; k < 1 + kNumLayers; ++k)
+      if (head[k] > size) ok = false;
+    if (!ok) break;
+    ev.vertices.resize(3 * (size_t)head[0]);
+    take(ev.vertices.data(), ev.vertices.size() * 4);
+    for (int l = 0; ok && l < kNumLayers; ++l) {
+      const size_t n = head[1 + l];
+      ev.x[l].resize(n);
+      ev.y[l].resize(n);
+      ev.z[l].resize(n);
+      ev.alpha[l].resize(n);
+      ev.id[l].resize(n);
+      ev.mc[l].resize(n);
+      take(ev.x[l].data(), n * 4);
+      take(ev.y[l].data(), n * 4);
+      take(ev.z[l].data(), n * 4);
+      take(ev.alpha[l].data(), n * 4);
+      take(ev.id[l].data(), n * 4);
+      take(ev.mc[l].data(), n * 4);
+    }
+  }
+  munmap(map, size);
+  if (!ok) {
+    out.events.clear();
+    err = "truncated or malformed event file: " + path;
+    return false;
+  }
+  out.labels.assign(out.events.size(), {});
+  return true;
+}
+
+bool mergePileUp(const EventBatch& in, int pileUp, EventBatch& out, std::string& err)
+{
+  if (pileUp < 1) {
+    err = "pile-up must be at least 1";
+    return false;
+  }
+  out.events.clear();
+  out.labels.clear();
+  const bool withLabels = in.labels.size() == in.events.size();
+  int verticesNum = 0;  // vertices of the merged event so far
+  int32_t idOffset = 0;  // clusters of the merged event so far
+  for (size_t e = 0; e < in.events.size(); ++e) {
+    const RawEvent& src = in.events[e];
+    if (out.events.empty() || verticesNum >= pileUp) {
+      out.events.emplace_back();
+      out.labels.emplace_back();
+      verticesNum = 0;
+      idOffset = 0;
+    }
+    RawEvent& dst = out.events.back();
+    dst.vertices.insert(dst.vertices.end(), src.vertices.begin(), src.vertices.end());
+    verticesNum += src.numVertices();
+    // Clusters follow the event's last vertex line (generatePileUpData.cxx:88-104).
+    const int32_t offset = verticesNum > 1 ? (verticesNum - 1) * kPileUpMcOffset : 0;
+    for (int l = 0; l < kNumLayers; ++l) {
+      for (int i = 0; i < src.layerSize(l); ++i) {
+        dst.x[l].push_back(src.x[l][i]);
+        dst.y[l].push_back(src.y[l][i]);
+        dst.z[l].push_back(src.z[l][i]);
+        dst.alpha[l].push_back(src.alpha[l][i]);
+        dst.id[l].push_back(idOffset + src.id[l][i]);  // the source's file order, after the earlier collisions
+        dst.mc[l].push_back(src.mc[l][i] + offset);
+      }
+    }
+    int32_t n = 0;
+    for (int l = 0; l < kNumLayers; ++l) n += src.layerSize(l);
+    idOffset += n;
+    if (withLabels)
+      for (McLabel m : in.labels[e]) {
+        m.mc += offset;
+        out.labels.back().push_back(m);
+      }
+  }
   return true;
 }
 

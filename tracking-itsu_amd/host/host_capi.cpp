@@ -47,6 +47,33 @@ int itsca_events_write(const itsca_events* ev, const char* events_path, const ch
   return 0;
 }
 
+int itsca_events_write_binary(const itsca_events* ev, const char* path)
+{
+  if (!ev || !path) return -1;
+  std::string err;
+  return itsca::writeEventsBinary(ev->batch, path, err) ? 0 : -1;
+}
+
+int itsca_events_load_labels(itsca_events* ev, const char* labels_path)
+{
+  if (!ev || !labels_path) return -1;
+  std::string err;
+  return itsca::loadLabelRows(labels_path, ev->batch, err) ? 0 : -1;
+}
+
+itsca_events* itsca_events_merge_pileup(const itsca_events* ev, int pile_up)
+{
+  if (!ev || pile_up < 1) return nullptr;
+  itsca_events* out = new (std::nothrow) itsca_events();
+  if (!out) return nullptr;
+  std::string err;
+  if (!itsca::mergePileUp(ev->batch, pile_up, out->batch, err)) {
+    delete out;
+    return nullptr;
+  }
+  return out;
+}
+
 void itsca_events_free(itsca_events* ev) { delete ev; }
 
 int itsca_events_count(const itsca_events* ev) { return ev ? (int)ev->batch.events.size() : -1; }

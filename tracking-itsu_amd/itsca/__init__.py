@@ -68,6 +68,7 @@ MLP_FEATURES = 16
 HOST_SYMBOLS = [
     "itsca_events_generate", "itsca_events_load", "itsca_events_write", "itsca_events_free", "itsca_events_count",
     "itsca_events_num_vertices", "itsca_events_layer_offsets", "itsca_events_copy", "itsca_filter_load",
+    "itsca_events_write_binary", "itsca_events_load_labels", "itsca_events_merge_pileup",
 ]
 
 
@@ -147,6 +148,10 @@ def host_lib() -> ctypes.CDLL:
         F, I32 = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32)
         lib.itsca_events_copy.argtypes = [P, I, F, F, F, I32, I32, F]
         lib.itsca_filter_load.argtypes = [ctypes.c_char_p, F, I, F]
+        lib.itsca_events_write_binary.argtypes = [P, ctypes.c_char_p]
+        lib.itsca_events_load_labels.argtypes = [P, ctypes.c_char_p]
+        lib.itsca_events_merge_pileup.argtypes = [P, I]
+        lib.itsca_events_merge_pileup.restype = P
         _host = lib
     return _host
 
@@ -225,8 +230,34 @@ def write_events(preset: int, seed: int, num_events: int, path: str, labels_path
 
 
 def load_events(path: str) -> List[Event]:
-    """Reads the reference event text format (src/CA/IOUtils.cxx:38-78)."""
+    """Reads the reference event text format (src/CA/IOUtils.cxx:38-78) or its
+    binary side-car (written by convert_events_binary)."""
     return _read_events(_EventsHandle(host_lib().itsca_events_load(path.encode())))
+
+
+def convert_events_binary(src: str, dst: str) -> None:
+    """Writes the binary side-car of an event file (text or binary): the same
+    content, read back through mmap by load_events."""
+    h = _EventsHandle(host_lib().itsca_events_load(src.encode()))
+    if host_lib().itsca_events_write_binary(h.h, dst.encode()) != 0:
+        raise ItscaError(f"cannot write {dst}")
+
+
+def merge_pileup(events_path: str, pile_up: int, out_path: str, labels_path: Optional[str] = None,
+                 out_labels_path: Optional[str] = None) -> int:
+    """Pile-up merger (benchmarks/scripts/root/generatePileUpData.cxx:44-207):
+    consecutive events merged into events of `pile_up` vertices, MC labels of
+    the k-th collision offset by (k - 1) * 100000; writes the reference text
+    formats and returns the number of merged events."""
+    lib = host_lib()
+    h = _EventsHandle(lib.itsca_events_load(events_path.encode()))
+    if labels_path and lib.itsca_events_load_labels(h.h, labels_path.encode()) != 0:
+        raise ItscaError(f"cannot read {labels_path}")
+    m = _EventsHandle(lib.itsca_events_merge_pileup(h.h, int(pile_up)))
+    lp = out_labels_path.encode() if out_labels_path else None
+    if lib.itsca_events_write(m.h, out_path.encode(), lp) != 0:
+        raise ItscaError(f"cannot write {out_path}")
+    return lib.itsca_events_count(m.h)
 
 
 class GpuTracker:
