@@ -91,12 +91,18 @@ __device__ __forceinline__ int segmentOfChunk(const int* chunkStart, int S, int 
 // PrimaryVertexContext.cxx:31-142, Cluster.cxx:35-45.
 // One workgroup per (item, layer).
 // ---------------------------------------------------------------------------
-constexpr int kPrepLds = 16384;  // layers up to this many clusters are sorted through LDS
+// kPrepPer: clusters per thread on the LDS path.  Layers up to kPrepPer * 1024
+// clusters are sorted through an LDS array of that many ints: 64 KB for
+// kPrepPer = 16 (two workgroups per CU), 128 KB for kPrepPer = 32 (batches
+// with larger layers, such as pile-up events).  Positions are packed as 16
+// bits, so kPrepPer * 1024 <= 65536.
 constexpr int kPrepThreads = 1024;
-constexpr int kPrepPer = kPrepLds / kPrepThreads;  // clusters per thread on the LDS path
 
+template <int kPrepPer>
 __global__ void __launch_bounds__(1024) kPrep(DevState s)
 {
+  constexpr int kPrepLds = kPrepPer * kPrepThreads;
+  static_assert(kPrepLds <= 65536, "sorted positions are packed as 16 bits");
   static_assert(kBins * kMaxZSub <= kPrepLds, "sub-bin counters live in the bucket array");
   __shared__ int lBucket[kPrepLds];
   __shared__ int hist[kTableSize];
@@ -210,7 +216,7 @@ __global__ void __launch_bounds__(1024) kPrep(DevState s)
   // position goes to global scratch and the records are gathered.
   if (n <= kPrepLds) {
     // The sorted position of each of this thread's clusters, i = threadIdx.x + k * 1024,
-    // two 16-bit positions per register (n <= 16384).
+    // two 16-bit positions per register.
     uint32_t pos2[kPrepPer / 2];
 #pragma unroll
     for (int k = 0; k < kPrepPer / 2; ++k) pos2[k] = 0;
@@ -1482,7 +1488,10 @@ void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t str
     if (timer) timer->after(name, stage, stream);
   };
   if (timer) timer->start(stream);
-  hipLaunchKernelGGL(kPrep, dim3(s.W * kLayers), dim3(1024), 0, stream, s);
+  if (cfg.maxLayerClusters <= 16 * kPrepThreads)
+    hipLaunchKernelGGL(kPrep<16>, dim3(s.W * kLayers), dim3(kPrepThreads), 0, stream, s);
+  else
+    hipLaunchKernelGGL(kPrep<32>, dim3(s.W * kLayers), dim3(kPrepThreads), 0, stream, s);
   done("kPrep", 0);
   hipLaunchKernelGGL(kPlanTracklets, dim3(1), dim3(1024), 0, stream, s, counts);
   done("kPlanTracklets", 1);
