@@ -66,8 +66,14 @@ struct DevState {
   const float *rx, *ry, *rz;
   const int *rid, *rmc;
   const int* evLayerOff;  // 8 per event, absolute into the raw arrays
-  // sorted clusters
-  const Seg* clSeg;  // W*7
+  // sorted clusters.  clSeg indexes the per-item arrays aligned with the
+  // sorted clusters (trkCnt, trkOff, trkKept, trkCand); the records
+  // themselves (cx..cbin, crzp, crzpS, table, subTab, segGeo) are those of
+  // item recItem[w]: the item itself, or an earlier vertex of the same event
+  // with the same x and y, whose preparation is identical (phi and r are
+  // taken about (x, y), the bins use the absolute z).
+  const Seg* clSeg;     // W*7
+  const int* recItem;   // W
   float *cx, *cy, *cz, *cphi, *cr;
   float4* crzp;  // (r, z, phi, 0) per sorted cluster: one load per tracklet candidate
   float4* crzpS;  // the same records, z-sub-bin order inside each bin; w = offset in the bin (int bits)
@@ -114,5 +120,8 @@ struct DevState {
   int maxChunks;
   int* status;  // kNumStatus
 };
+
+// Segment of the sorted-cluster records that work item w reads for layer l.
+__device__ __forceinline__ Seg recSeg(const DevState& s, int w, int l) { return s.clSeg[s.recItem[w] * kLayers + l]; }
 
 }  // namespace itsca

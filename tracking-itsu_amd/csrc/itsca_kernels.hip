@@ -112,6 +112,10 @@ __global__ void __launch_bounds__(1024) kPrep(DevState s)
   const int w = blockIdx.x / kLayers, l = blockIdx.x % kLayers;
   if (blockIdx.x == 0 && threadIdx.x < kNumStatus) s.status[threadIdx.x] = 0;
   const Seg seg = s.clSeg[w * kLayers + l];
+  if (s.recItem[w] != w) {  // reads the records of an earlier vertex: only its own counts are reset
+    for (int i = threadIdx.x; i < seg.count; i += blockDim.x) s.trkCnt[seg.base + i] = 0;
+    return;
+  }
   const int e = s.itemEvent[w];
   const int rawBase = s.evLayerOff[e * 8 + l];
   const float vx = s.itemVertex[3 * w], vy = s.itemVertex[3 * w + 1];
@@ -665,15 +669,16 @@ __device__ __forceinline__ void searchRows(const TrackletSearch& t, const int* t
 
 // The search of one cluster with every candidate read from global memory
 // (the fill's path for clusters that overflowed their slots).
+// rw: the item whose records are read; baseNext: their next-layer segment.
 template <typename F>
-__device__ __forceinline__ void forEachTracklet(const DevState& s, int w, int l, const int* table, int baseNext,
+__device__ __forceinline__ void forEachTracklet(const DevState& s, int rw, int l, const int* table, int baseNext,
                                                 int nNext, float4 geoNext, float z, float r, float phi, float vz,
                                                 F&& emit)
 {
   const TrackletSearch t = trackletSearch(geoNext, z, r, phi, vz, l);
   const CandidateSource src{reinterpret_cast<const char*>(s.crzp + baseNext),
                             reinterpret_cast<const char*>(s.crzpS + baseNext),
-                            s.subTab + (size_t)(w * kLayers + l + 1) * kBins,
+                            s.subTab + (size_t)(rw * kLayers + l + 1) * kBins,
                             s.zSub,
                             nullptr,
                             nullptr,
@@ -705,23 +710,25 @@ __global__ void __launch_bounds__(B) kTrackletsCount(DevState s)
     const int sg = segmentOfChunk(chunkStart, S, chunk);
     const int w = sg / kTrackletLayers, l = sg % kTrackletLayers;
     const Seg cur = s.clSeg[w * kLayers + l], next = s.clSeg[w * kLayers + l + 1];
+    const int rw = s.recItem[w];
+    const Seg rcur = s.clSeg[rw * kLayers + l], rnext = s.clSeg[rw * kLayers + l + 1];
     const int first = (chunk - chunkStart[sg]) * B;
     const int i = first + threadIdx.x;
     const int p = cur.base + i;
-    const int* table = s.table + (size_t)(w * kLayers + l + 1) * kTableSize;
+    const int* table = s.table + (size_t)(rw * kLayers + l + 1) * kTableSize;
     TrackletSearch t;
     float z = 0.0f, r = 0.0f, phi = 0.0f;
     t.win.empty = true;
     if (i < cur.count) {
-      z = s.cz[p];
-      r = s.cr[p];
-      phi = s.cphi[p];
-      t = trackletSearch(s.segGeo[w * kLayers + l + 1], z, r, phi, s.itemVertex[3 * w + 2], l);
+      z = s.cz[rcur.base + i];
+      r = s.cr[rcur.base + i];
+      phi = s.cphi[rcur.base + i];
+      t = trackletSearch(s.segGeo[rw * kLayers + l + 1], z, r, phi, s.itemVertex[3 * w + 2], l);
     }
     // Union of the windows.  Rows are taken relative to the row before the
     // chunk's first cluster: the chunk's clusters sit in consecutive rows, and
     // a window starts at most one row before its cluster's row.
-    const int rowRef = (s.cbin[cur.base + first] / kPhiBins + kPhiBins - 1) % kPhiBins;
+    const int rowRef = (s.cbin[rcur.base + first] / kPhiBins + kPhiBins - 1) % kPhiBins;
     if (threadIdx.x == 0) {
       uni[0] = kPhiBins;
       uni[1] = -kPhiBins;
@@ -791,7 +798,7 @@ __global__ void __launch_bounds__(B) kTrackletsCount(DevState s)
       for (int k = threadIdx.x; k < total; k += B) {
         int ri = 0;
         while (ri + 1 < nrows && rowStart[ri + 1] <= k) ++ri;
-        stage[k] = copySrc[next.base + rowLo[ri] + (k - rowStart[ri])];
+        stage[k] = copySrc[rnext.base + rowLo[ri] + (k - rowStart[ri])];
       }
       __syncthreads();
     }
@@ -802,9 +809,9 @@ __global__ void __launch_bounds__(B) kTrackletsCount(DevState s)
         if (cnt < kTrkSlots) slots[cnt] = j;
         ++cnt;
       };
-      const CandidateSource src{reinterpret_cast<const char*>(s.crzp + next.base),
-                                reinterpret_cast<const char*>(s.crzpS + next.base),
-                                s.subTab + (size_t)(w * kLayers + l + 1) * kBins,
+      const CandidateSource src{reinterpret_cast<const char*>(s.crzp + rnext.base),
+                                reinterpret_cast<const char*>(s.crzpS + rnext.base),
+                                s.subTab + (size_t)(rw * kLayers + l + 1) * kBins,
                                 s.zSub,
                                 stage,
                                 rowOff,
@@ -886,16 +893,19 @@ __global__ void __launch_bounds__(B) kTrackletsFill(DevState s)
     exCap[threadIdx.x] = exc;
     const int chunkBase = s.chunkOff[(size_t)kStTracklets * s.maxChunks + chunk];
     if (i < cur.count) s.trkOff[p] = chunkBase + ex - s.trkSeg[sg].base;
+    const int rw = s.recItem[w];
+    const Seg rcur = s.clSeg[rw * kLayers + l], rnext = s.clSeg[rw * kLayers + l + 1];
     forEachSlotOutput<B>(capped, exc, totalCap, exCap, owner, [&](int c, int k) {
       const int j = s.trkCand[(size_t)(cur.base + first + c) * kTrkSlots + k];
-      writeTracklet(s, chunkBase + exFull[c] + k, first + c, j, cur.base, next.base);
+      writeTracklet(s, chunkBase + exFull[c] + k, first + c, j, rcur.base, rnext.base);
     });
     if (cnt > kTrkSlots) {
-      const int* table = s.table + (size_t)(w * kLayers + l + 1) * kTableSize;
+      const int* table = s.table + (size_t)(rw * kLayers + l + 1) * kTableSize;
+      const int q = rcur.base + i;
       int k = chunkBase + ex;
-      forEachTracklet(s, w, l, table, next.base, next.count, s.segGeo[w * kLayers + l + 1], s.cz[p], s.cr[p],
-                      s.cphi[p], s.itemVertex[3 * w + 2],
-                      [&](int j) { writeTracklet(s, k++, i, j, cur.base, next.base); });
+      forEachTracklet(s, rw, l, table, rnext.base, next.count, s.segGeo[rw * kLayers + l + 1], s.cz[q], s.cr[q],
+                      s.cphi[q], s.itemVertex[3 * w + 2],
+                      [&](int j) { writeTracklet(s, k++, i, j, rcur.base, rnext.base); });
     }
   }
 }
@@ -936,11 +946,12 @@ __device__ __forceinline__ CellSeed cellSeed(const DevState& s, int w, int l, in
   CellSeed c;
   c.l = l;
   c.tnBase = s.trkSeg[w * kTrackletLayers + l + 1].base;
-  const Seg cl0 = s.clSeg[w * kLayers + l], cl1 = s.clSeg[w * kLayers + l + 1];
-  c.cl2Base = s.clSeg[w * kLayers + l + 2].base;
+  const Seg cl0 = recSeg(s, w, l), cl1 = recSeg(s, w, l + 1);  // records
+  c.cl2Base = recSeg(s, w, l + 2).base;
+  const int item1 = s.clSeg[w * kLayers + l + 1].base;  // the item's own counts and offsets
   const int k = s.tSecond[t];
-  c.nCand = s.trkCnt[cl1.base + k];  // tracklets of layer l+1 starting at this cluster
-  c.uFirst = c.nCand > 0 ? s.trkOff[cl1.base + k] : 0;
+  c.nCand = s.trkCnt[item1 + k];  // tracklets of layer l+1 starting at this cluster
+  c.uFirst = c.nCand > 0 ? s.trkOff[item1 + k] : 0;
   c.vx = s.itemVertex[3 * w];
   c.vy = s.itemVertex[3 * w + 1];
   c.vz = s.itemVertex[3 * w + 2];
@@ -1407,9 +1418,10 @@ __global__ void __launch_bounds__(B) kRoadsFill(DevState s)
     s.roadOff[c4.base + in] = out - s.roadSeg[w].base;
     Seg cs[kCellLayers];
     for (int k = 0; k < kCellLayers; ++k) cs[k] = s.celSeg[w * kCellLayers + k];
-    const Seg cl[kLayers] = {s.clSeg[w * kLayers + 0], s.clSeg[w * kLayers + 1], s.clSeg[w * kLayers + 2],
-                             s.clSeg[w * kLayers + 3], s.clSeg[w * kLayers + 4], s.clSeg[w * kLayers + 5],
-                             s.clSeg[w * kLayers + 6]};
+    const int rw = s.recItem[w];
+    const Seg cl[kLayers] = {s.clSeg[rw * kLayers + 0], s.clSeg[rw * kLayers + 1], s.clSeg[rw * kLayers + 2],
+                             s.clSeg[rw * kLayers + 3], s.clSeg[rw * kLayers + 4], s.clSeg[rw * kLayers + 5],
+                             s.clSeg[rw * kLayers + 6]};
     // Iterative depth-first walk; pos[k] indexes the predecessor list of the
     // cell chosen at layer k+1.
     int cell[kCellLayers];

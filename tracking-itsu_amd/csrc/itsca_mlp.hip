@@ -153,20 +153,22 @@ __device__ __forceinline__ void packFeatures(const float f[kMlpIn], uint16_t* ro
   r[1] = make_uint4(packBf16x2(f[8], f[9]), 0u, 0u, 0u);
 }
 
-// The cluster-i part of the features (pi: sorted cluster of layer l).
-__device__ __forceinline__ TrackletSeed seedOf(const DevState& s, float vx, float vy, float vz, int pi)
+// The cluster-i part of the features: sorted cluster i of layer l, whose
+// record is at ri and whose tracklet count (of this work item) is at ci.
+__device__ __forceinline__ TrackletSeed seedOf(const DevState& s, float vx, float vy, float vz, int ri, int ci)
 {
-  const float4 a = s.crzp[pi];  // (r, z, phi, 0)
-  return trackletSeed(s.cx[pi], s.cy[pi], a.y, a.x, a.z, vx, vy, vz, s.trkCnt[pi]);
+  const float4 a = s.crzp[ri];  // (r, z, phi, 0)
+  return trackletSeed(s.cx[ri], s.cy[ri], a.y, a.x, a.z, vx, vy, vz, s.trkCnt[ci]);
 }
 
-// Features of the candidate pj (sorted cluster of layer l+1) of seed t.
+// Features of candidate j (sorted cluster of layer l+1; record at rj, count at
+// cj) of seed t.
 __device__ __forceinline__ void featuresOf(const DevState& s, const TrackletSeed& t, float vx, float vy, float vz, int l,
-                                           int pj, float f[kMlpIn])
+                                           int rj, int cj, float f[kMlpIn])
 {
-  const float4 b = s.crzp[pj];
-  candidateFeatures(t, l, s.cx[pj] - vx, s.cy[pj] - vy, b.y, b.x, b.z, vz,
-                    l + 1 < kTrackletLayers ? s.trkCnt[pj] : 0, f);
+  const float4 b = s.crzp[rj];
+  candidateFeatures(t, l, s.cx[rj] - vx, s.cy[rj] - vy, b.y, b.x, b.z, vz,
+                    l + 1 < kTrackletLayers ? s.trkCnt[cj] : 0, f);
 }
 
 // Segment (item, layer) of a chunk: the largest sg with chunkStart[sg] <= chunk.
@@ -219,6 +221,7 @@ __global__ void __launch_bounds__(B) kTrackletsScore(DevState s, float threshold
     const int sg = segmentOf(chunkStart, S, chunk);
     const int w = sg / kTrackletLayers, l = sg % kTrackletLayers;
     const Seg cur = s.clSeg[w * kLayers + l], next = s.clSeg[w * kLayers + l + 1];
+    const int recCur = recSeg(s, w, l).base, recNext = recSeg(s, w, l + 1).base;
     const int first = (chunk - chunkStart[sg]) * B;
     const int i = first + threadIdx.x;
     const int p = cur.base + i;
@@ -235,7 +238,7 @@ __global__ void __launch_bounds__(B) kTrackletsScore(DevState s, float threshold
     const int rowBase = cur.base + first + wave * 64;  // slot row of the cluster of lane c: rowBase + c
     if (scored > 0) {  // the cluster-i features, once per cluster
       static_assert(sizeof(TrackletSeed) == kSeedFloats * sizeof(float), "seed layout");
-      const TrackletSeed t = seedOf(s, vx, vy, vz, p);
+      const TrackletSeed t = seedOf(s, vx, vy, vz, recCur + i, p);
       const float* tv = reinterpret_cast<const float*>(&t);
       for (int k = 0; k < kSeedFloats; ++k) seeds[k][threadIdx.x] = tv[k];
     }
@@ -256,7 +259,7 @@ __global__ void __launch_bounds__(B) kTrackletsScore(DevState s, float threshold
           TrackletSeed t;
           float* tv = reinterpret_cast<float*>(&t);
           for (int k = 0; k < kSeedFloats; ++k) tv[k] = seeds[k][wave * 64 + c];
-          featuresOf(s, t, vx, vy, vz, l, next.base + j, f);
+          featuresOf(s, t, vx, vy, vz, l, recNext + j, next.base + j, f);
         } else {
           for (int k = 0; k < kMlpIn; ++k) f[k] = 0.0f;
         }
@@ -356,11 +359,11 @@ __global__ void kCollect(DevState s, const uint8_t* label, int total, int stride
         hi = mid;
     }
     const int w = lo / kTrackletLayers, l = lo % kTrackletLayers;
-    const int pi = s.clSeg[w * kLayers + l].base + s.tFirst[t];
-    const int pj = s.clSeg[w * kLayers + l + 1].base + s.tSecond[t];
+    const int i = s.tFirst[t], j = s.tSecond[t];
     float f[kMlpIn];
     const float vx = s.itemVertex[3 * w], vy = s.itemVertex[3 * w + 1], vz = s.itemVertex[3 * w + 2];
-    featuresOf(s, seedOf(s, vx, vy, vz, pi), vx, vy, vz, l, pj, f);
+    featuresOf(s, seedOf(s, vx, vy, vz, recSeg(s, w, l).base + i, s.clSeg[w * kLayers + l].base + i), vx, vy, vz,
+               l, recSeg(s, w, l + 1).base + j, s.clSeg[w * kLayers + l + 1].base + j, f);
     packFeatures(f, outX + (size_t)(base + row) * kMlpInPad);
     outY[base + row] = label[t];
   }

@@ -96,6 +96,7 @@ struct itsca_ctx {
   // batch shape
   int W = 0, E = 0;
   std::vector<int> itemEvent;
+  std::vector<int> recItem;  // the item whose cluster records each item reads
   std::vector<Seg> clSeg;  // 7 per item
   long sortedClusters = 0;
   int tCap = 0, cCap = 0, eCap = 0, rCap = 0;
@@ -107,7 +108,7 @@ struct itsca_ctx {
 
   // device buffers
   DBuf<float> rx, ry, rz, itemVertex;
-  DBuf<int> rid, rmc, evLayerOff, dItemEvent;
+  DBuf<int> rid, rmc, evLayerOff, dItemEvent, dRecItem;
   DBuf<Seg> dClSeg, dTrkSeg, dCelSeg, dEdgSeg, dRoadSeg;
   DBuf<float> cx, cy, cz, cphi, cr, sPhi, sR;
   DBuf<int> cid, cmc, cbin, sBin, sInv, trkCnt, trkOff, trkKept, table;
@@ -285,6 +286,7 @@ struct itsca_ctx {
     s.maxSegs = maxSegs;
     s.chunk = blockSize;
     s.itemEvent = dItemEvent.p;
+    s.recItem = dRecItem.p;
     s.itemVertex = itemVertex.p;
     s.rx = rx.p;
     s.ry = ry.p;
@@ -510,8 +512,12 @@ itsca_status itsca_upload(itsca_ctx* ctx)
 {
   return guarded(ctx, "itsca_upload", [&]() -> itsca_status {
     HIP_TRY(hipSetDevice(ctx->device));
-    // Work items: every vertex of every event, in event order.
+    // Work items: every vertex of every event, in event order.  A vertex
+    // with the same x and y (bit for bit) as an earlier vertex of its event
+    // reads that vertex's prepared clusters: phi and r are taken about
+    // (x, y) and the bins use the absolute z, so they are identical.
     ctx->itemEvent.clear();
+    ctx->recItem.clear();
     ctx->clSeg.clear();
     std::vector<float> itemVtx;
     long sc = 0;
@@ -519,7 +525,15 @@ itsca_status itsca_upload(itsca_ctx* ctx)
     for (int e = 0; e < ctx->E; ++e) {
       const int v0 = ctx->hEvVert[e];
       const int v1 = e + 1 < ctx->E ? ctx->hEvVert[e + 1] : nv;
+      const int firstItem = (int)ctx->itemEvent.size();
       for (int v = v0; v < v1; ++v) {
+        int rec = (int)ctx->itemEvent.size();
+        for (int u = v0; u < v; ++u)
+          if (std::memcmp(&ctx->hvert[3 * u], &ctx->hvert[3 * v], 2 * sizeof(float)) == 0) {
+            rec = ctx->recItem[firstItem + (u - v0)];
+            break;
+          }
+        ctx->recItem.push_back(rec);
         ctx->itemEvent.push_back(e);
         itemVtx.insert(itemVtx.end(), {ctx->hvert[3 * v], ctx->hvert[3 * v + 1], ctx->hvert[3 * v + 2]});
         for (int l = 0; l < kLayers; ++l) {
@@ -540,6 +554,7 @@ itsca_status itsca_upload(itsca_ctx* ctx)
     HIP_TRY(ctx->rmc.reserve(NR));
     HIP_TRY(ctx->evLayerOff.reserve(std::max<size_t>(ctx->hLayerOff.size(), 1)));
     HIP_TRY(ctx->dItemEvent.reserve(std::max(ctx->W, 1)));
+    HIP_TRY(ctx->dRecItem.reserve(std::max(ctx->W, 1)));
     HIP_TRY(ctx->itemVertex.reserve(std::max<size_t>(itemVtx.size(), 1)));
     HIP_TRY(ctx->dClSeg.reserve(std::max<size_t>(ctx->clSeg.size(), 1)));
     auto put = [&](void* dst, const void* src, size_t bytes) {
@@ -552,6 +567,7 @@ itsca_status itsca_upload(itsca_ctx* ctx)
     put(ctx->rmc.p, ctx->hmc.data(), ctx->hmc.size() * 4);
     put(ctx->evLayerOff.p, ctx->hLayerOff.data(), ctx->hLayerOff.size() * 4);
     put(ctx->dItemEvent.p, ctx->itemEvent.data(), ctx->itemEvent.size() * 4);
+    put(ctx->dRecItem.p, ctx->recItem.data(), ctx->recItem.size() * 4);
     put(ctx->itemVertex.p, itemVtx.data(), itemVtx.size() * 4);
     put(ctx->dClSeg.p, ctx->clSeg.data(), ctx->clSeg.size() * sizeof(Seg));
     ctx->estimateCapacities();
@@ -661,7 +677,7 @@ itsca_status itsca_download(itsca_ctx* ctx, int item, int what, int layer, void*
     switch (what) {
       case ITSCA_CLUSTERS: {
         if (!layerOk(0, 6)) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_download: bad layer");
-        const Seg g = ctx->clSeg[item * kLayers + layer];
+        const Seg g = ctx->clSeg[ctx->recItem[item] * kLayers + layer];
         auto id = fetch(ctx->cid.p + g.base, g.count), bin = fetch(ctx->cbin.p + g.base, g.count),
              mc = fetch(ctx->cmc.p + g.base, g.count);
         auto phi = asInt(fetch(ctx->cphi.p + g.base, g.count)), r = asInt(fetch(ctx->cr.p + g.base, g.count)),
@@ -673,7 +689,7 @@ itsca_status itsca_download(itsca_ctx* ctx, int item, int what, int layer, void*
       }
       case ITSCA_INDEX_TABLE: {
         if (!layerOk(1, 6)) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_download: bad layer");
-        rows = fetch(ctx->table.p + (size_t)(item * kLayers + layer) * kTableSize, kTableSize);
+        rows = fetch(ctx->table.p + (size_t)(ctx->recItem[item] * kLayers + layer) * kTableSize, kTableSize);
         break;
       }
       case ITSCA_TRACKLETS: {
