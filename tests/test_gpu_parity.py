@@ -94,6 +94,35 @@ def test_pileup_event_with_large_layers_matches_oracle(tracker):
         assert itsca.dump_item(tracker, v, ev.vertices[v]) == pyoracle.OracleVertex(ev, v, stable=True).dump(), v
 
 
+def test_cli_on_merged_pileup_in_binary_format(tracker):
+    """bin/itsca-events merges small events into pile-up events of 3 vertices
+    (vertices sharing x and y share their prepared clusters) and writes the
+    binary side-car; bin/itsca-track reads it with the merged labels, writes
+    the road reports, and finds per vertex the road counts of the same events
+    run through the C ABI."""
+    import re
+    import subprocess
+    say("gpu cli on merged pile-up, binary events")
+    with tempfile.TemporaryDirectory() as tmp:
+        src, lab = os.path.join(tmp, "e.txt"), os.path.join(tmp, "l.txt")
+        merged, mlab, binp = os.path.join(tmp, "m.txt"), os.path.join(tmp, "ml.txt"), os.path.join(tmp, "m.itsb")
+        itsca.write_events(1, 21, 6, src, lab)
+        tool = os.path.join(itsca.BIN_DIR, "itsca-events")
+        subprocess.run([tool, "pileup", "3", src, merged, lab, mlab], check=True, capture_output=True)
+        subprocess.run([tool, "binary", merged, binp], check=True, capture_output=True)
+        out = subprocess.run([os.path.join(itsca.BIN_DIR, "itsca-track"), binp, mlab], capture_output=True, text=True,
+                             timeout=120)
+        assert out.returncode == 0, out.stdout + out.stderr
+        got = [int(n) for n in re.findall(r"Found (\d+) roads for vertex", out.stdout)]
+        evs = itsca.load_events(binp)
+        assert [len(e.vertices) for e in evs] == [3, 3]
+        tracker.set_events(evs)
+        tracker.run()
+        assert got == [tracker.counts(i).roads for i in range(tracker.num_items())] and sum(got) > 0
+        for name in ("CorrectRoads.txt", "DuplicateRoads.txt", "FakeRoads.txt"):
+            assert os.path.exists(os.path.join(tmp, name)), name
+
+
 def test_runs_are_deterministic(tracker):
     say("gpu determinism")
     evs = itsca.generate_events(2, 5, 2)

@@ -1276,8 +1276,9 @@ __global__ void __launch_bounds__(B) kEdgesFill(DevState s)
 // Predecessor lists: a stable counting sort of the (c, n) edges of one
 // (item, layer) by n.  One workgroup per (item, layer); keys in passes of
 // kTransposeKeys.  Bucket placement by LDS atomics, then each edge is written
-// at bucket start + (number of bucket members with a smaller c), which keeps
-// the predecessors ascending whatever order the atomics took.
+// at bucket start + (number of bucket members with a smaller edge index).  The
+// fill writes the edges in ascending c, so that rank keeps the predecessors
+// ascending whatever order the atomics took.
 constexpr int kTransposeKeys = 8192;
 
 __global__ void __launch_bounds__(1024) kEdgesTranspose(DevState s)
@@ -1325,10 +1326,9 @@ __global__ void __launch_bounds__(1024) kEdgesTranspose(DevState s)
       const int kk = to[a] - k0;
       if (kk < 0 || kk >= nk) continue;
       const int lo = start[kk], hi = hist[kk];  // the cursor ended at the bucket end
-      const int mine = from[a];
       int rank = 0;
-      for (int b = lo; b < hi; ++b) rank += from[tmp[b]] < mine ? 1 : 0;
-      s.pred[e.base + lo + rank] = mine;
+      for (int b = lo; b < hi; ++b) rank += tmp[b] < a ? 1 : 0;
+      s.pred[e.base + lo + rank] = from[a];
     }
     __syncthreads();
   }

@@ -8,6 +8,7 @@
 #include <tuple>
 #include <unordered_set>
 
+#include "events.hpp"
 #include "ITSReconstruction/CA/IOUtils.h"
 #include "ITSReconstruction/CA/Tracker.h"
 #include "itsca.h"
@@ -229,10 +230,25 @@ PrimaryVertexContext Tracker<true>::getPrimaryVertexContext(int item) const
 
 namespace IOUtils {
 
-// IOUtils.cxx:38-78
+// IOUtils.cxx:38-78; also reads the binary side-car (itsca::writeEventsBinary).
 std::vector<Event> loadEventData(const std::string& fileName)
 {
   std::vector<Event> events;
+  if (itsca::isEventsBinary(fileName)) {
+    itsca::EventBatch batch;
+    std::string err;
+    if (!itsca::loadEventsBinary(fileName, batch, err)) return events;
+    for (const itsca::RawEvent& raw : batch.events) {
+      events.emplace_back((int)events.size());
+      Event& ev = events.back();
+      for (int v = 0; v < raw.numVertices(); ++v)
+        ev.addPrimaryVertex(raw.vertices[3 * v], raw.vertices[3 * v + 1], raw.vertices[3 * v + 2]);
+      for (int l = 0; l < Constants::ITS::LayersNumber; ++l)
+        for (int i = 0; i < raw.layerSize(l); ++i)
+          ev.pushClusterToLayer(l, raw.id[l][i], raw.x[l][i], raw.y[l][i], raw.z[l][i], raw.alpha[l][i], raw.mc[l][i]);
+    }
+    return events;
+  }
   std::ifstream in(fileName);
   std::string line, unused;
   int clusterId = -1;
