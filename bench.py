@@ -162,16 +162,19 @@ def road_quality(tracker):
 
 
 def single_events(itsca, steps=50):
-    """One event per launch, plain form, graph replay: the latency of a small
-    event (100 tracks) and of one central event (8000 tracks)."""
+    """One event per launch, plain form, graph replay, autotuned: the latency
+    of a small event (100 tracks) and of one central event (8000 tracks)."""
     out = {}
     for key, preset, seed in (("small_100_tracks", 1, 1), ("central_8000_tracks", 2, HELD_OUT_SEED)):
         t = itsca.GpuTracker(device=0)
         t.set_events(itsca.generate_events(preset, seed, 1))
+        tuning, _ = t.autotune(5)
         s = time_form(t, steps, 3)
         t.close()
         out[key] = {"ms_per_event": round(1e3 * s / steps, 4), "events_per_s": round(steps / s, 1),
-                    "form": "plain, one event per launch, hipGraph replay", "seed": seed}
+                    "form": "plain, one event per launch, hipGraph replay", "seed": seed,
+                    "tuning": {"block_size": tuning.block_size, "persistent_blocks": tuning.persistent_blocks,
+                               "z_subbins": tuning.z_subbins}}
     return out
 
 
