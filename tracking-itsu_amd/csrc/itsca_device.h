@@ -74,7 +74,8 @@ struct DevState {
   // taken about (x, y), the bins use the absolute z).
   const Seg* clSeg;     // W*7
   const int* recItem;   // W
-  float *cx, *cy, *cz, *cphi, *cr;
+  float2* cxy;  // (x, y) per sorted cluster
+  float *cz, *cphi, *cr;
   float4* crzp;  // (r, z, phi, 0) per sorted cluster: one load per tracklet candidate
   float4* crzpS;  // the same records, z-sub-bin order inside each bin; w = offset in the bin (int bits)
   uint4* subTab;  // per (item, layer) and bin: byte t = offset of sub-bin t in crzpS (t <= zSub)

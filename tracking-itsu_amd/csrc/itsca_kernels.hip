@@ -251,8 +251,15 @@ __global__ void __launch_bounds__(1024) kPrep(DevState s)
       for (int p = threadIdx.x; p < n; p += kPrepThreads) st(dst, p, lBucket[p]);
       __syncthreads();
     };
-    field(s.rx + rawBase, s.cx + base);
-    field(s.ry + rawBase, s.cy + base);
+    // x and y go to the two halves of the packed (x, y) records.
+    auto fieldHalf = [&](const void* src, int half) {
+      stageField(src);
+      char* dst = reinterpret_cast<char*>(s.cxy + base) + 4 * half;
+      for (int p = threadIdx.x; p < n; p += kPrepThreads) *reinterpret_cast<int*>(dst + 8u * (unsigned)p) = lBucket[p];
+      __syncthreads();
+    };
+    fieldHalf(s.rx + rawBase, 0);
+    fieldHalf(s.ry + rawBase, 1);
     field(s.rid + rawBase, s.cid + base);
     field(s.rmc + rawBase, s.cmc + base);
     field(s.sBin + base, s.cbin + base);
@@ -279,8 +286,7 @@ __global__ void __launch_bounds__(1024) kPrep(DevState s)
       const int i = s.sInv[base + p];
       const int q = base + p;
       const float z = s.rz[rawBase + i], phi = s.sPhi[base + i], r = s.sR[base + i];
-      s.cx[q] = s.rx[rawBase + i];
-      s.cy[q] = s.ry[rawBase + i];
+      s.cxy[q] = make_float2(s.rx[rawBase + i], s.ry[rawBase + i]);
       s.cz[q] = z;
       s.cphi[q] = phi;
       s.cr[q] = r;
@@ -835,7 +841,8 @@ __device__ __forceinline__ void writeTracklet(const DevState& s, int k, int i, i
   const float zi = s.cz[pi], ri = s.cr[pi];
   s.tFirst[k] = i;
   s.tSecond[k] = j;
-  s.tTP[k] = make_float2((zi - s.cz[q]) / (ri - s.cr[q]), fdlibmAtan2f(s.cy[pi] - s.cy[q], s.cx[pi] - s.cx[q]));
+  const float2 a = s.cxy[pi], b = s.cxy[q];
+  s.tTP[k] = make_float2((zi - s.cz[q]) / (ri - s.cr[q]), fdlibmAtan2f(a.y - b.y, a.x - b.x));
   s.cellCnt[k] = 0;  // stays 0 where the cell stage does not run
 }
 
@@ -956,12 +963,13 @@ __device__ __forceinline__ CellSeed cellSeed(const DevState& s, int w, int l, in
   c.vy = s.itemVertex[3 * w + 1];
   c.vz = s.itemVertex[3 * w + 2];
   const int p1 = cl0.base + s.tFirst[t], p2 = cl1.base + k;
-  c.x1 = s.cx[p1];
-  c.y1 = s.cy[p1];
+  const float2 xy1 = s.cxy[p1], xy2 = s.cxy[p2];
+  c.x1 = xy1.x;
+  c.y1 = xy1.y;
   c.z1 = s.cz[p1];
   c.r1 = s.cr[p1];
-  c.x2 = s.cx[p2];
-  c.y2 = s.cy[p2];
+  c.x2 = xy2.x;
+  c.y2 = xy2.y;
   const float r2 = s.cr[p2];
   c.r1sq = c.r1 * c.r1;
   c.r2sq = r2 * r2;
@@ -997,7 +1005,8 @@ __device__ __forceinline__ bool cellFit(const DevState& s, const CellSeed& c, in
 {
   const int u = c.tnBase + iu;
   const int p3 = c.cl2Base + s.tSecond[u];
-  const float x3 = s.cx[p3], y3 = s.cy[p3], r3 = s.cr[p3];
+  const float2 xy3 = s.cxy[p3];
+  const float x3 = xy3.x, y3 = xy3.y, r3 = s.cr[p3];
   const float r3sq = r3 * r3;
   const float d2x = x3 - c.x1, d2y = y3 - c.y1, d2z = r3sq - c.r1sq;
   const float nx = (c.d1y * d2z) - (c.d1z * d2y);

@@ -158,7 +158,8 @@ __device__ __forceinline__ void packFeatures(const float f[kMlpIn], uint16_t* ro
 __device__ __forceinline__ TrackletSeed seedOf(const DevState& s, float vx, float vy, float vz, int ri, int ci)
 {
   const float4 a = s.crzp[ri];  // (r, z, phi, 0)
-  return trackletSeed(s.cx[ri], s.cy[ri], a.y, a.x, a.z, vx, vy, vz, s.trkCnt[ci]);
+  const float2 xy = s.cxy[ri];
+  return trackletSeed(xy.x, xy.y, a.y, a.x, a.z, vx, vy, vz, s.trkCnt[ci]);
 }
 
 // Features of candidate j (sorted cluster of layer l+1; record at rj, count at
@@ -167,7 +168,8 @@ __device__ __forceinline__ void featuresOf(const DevState& s, const TrackletSeed
                                            int rj, int cj, float f[kMlpIn])
 {
   const float4 b = s.crzp[rj];
-  candidateFeatures(t, l, s.cx[rj] - vx, s.cy[rj] - vy, b.y, b.x, b.z, vz,
+  const float2 xy = s.cxy[rj];
+  candidateFeatures(t, l, xy.x - vx, xy.y - vy, b.y, b.x, b.z, vz,
                     l + 1 < kTrackletLayers ? s.trkCnt[cj] : 0, f);
 }
 
