@@ -74,7 +74,7 @@ struct DevState {
   // taken about (x, y), the bins use the absolute z).
   const Seg* clSeg;     // W*7
   const int* recItem;   // W
-  float2* cxy;  // (x, y) per sorted cluster
+  float4* cxyrz;  // (x, y, r, z) per sorted cluster: one load for the tracklet fill and the cell fit
   float *cz, *cphi, *cr;
   float4* crzp;  // (r, z, phi, 0) per sorted cluster: one load per tracklet candidate
   float4* crzpS;  // the same records, z-sub-bin order inside each bin; w = offset in the bin (int bits)

@@ -251,27 +251,24 @@ __global__ void __launch_bounds__(1024) kPrep(DevState s)
       for (int p = threadIdx.x; p < n; p += kPrepThreads) st(dst, p, lBucket[p]);
       __syncthreads();
     };
-    // x and y go to the two halves of the packed (x, y) records.
-    auto fieldHalf = [&](const void* src, int half) {
-      stageField(src);
-      char* dst = reinterpret_cast<char*>(s.cxy + base) + 4 * half;
-      for (int p = threadIdx.x; p < n; p += kPrepThreads) *reinterpret_cast<int*>(dst + 8u * (unsigned)p) = lBucket[p];
-      __syncthreads();
-    };
-    fieldHalf(s.rx + rawBase, 0);
-    fieldHalf(s.ry + rawBase, 1);
     field(s.rid + rawBase, s.cid + base);
     field(s.rmc + rawBase, s.cmc + base);
     field(s.sBin + base, s.cbin + base);
     field(s.sR + base, s.cr + base);
     field(s.rz + rawBase, s.cz + base);
+    // x and y, in sorted order, go to the bin and r scratch (read above, free
+    // now) until the packed records are written.
+    field(s.rx + rawBase, s.sBin + base);
+    field(s.ry + rawBase, s.sR + base);
     // A sorted position is always written by the same thread, which reads back
-    // its own r and z for the packed record.
+    // its own x, y, r and z for the packed records.
     stageField(s.sPhi + base);
     for (int p = threadIdx.x; p < n; p += kPrepThreads) {
       const float phi = __int_as_float(lBucket[p]);
+      const float r = __int_as_float(ld(s.cr + base, p)), z = __int_as_float(ld(s.cz + base, p));
       st(s.cphi + base, p, __float_as_int(phi));
-      s.crzp[base + p] = make_float4(__int_as_float(ld(s.cr + base, p)), __int_as_float(ld(s.cz + base, p)), phi, 0.0f);
+      s.crzp[base + p] = make_float4(r, z, phi, 0.0f);
+      s.cxyrz[base + p] = make_float4(__int_as_float(ld(s.sBin + base, p)), __int_as_float(ld(s.sR + base, p)), r, z);
     }
   } else {
     for (int t0 = 0; t0 < n; t0 += blockDim.x) {
@@ -286,7 +283,7 @@ __global__ void __launch_bounds__(1024) kPrep(DevState s)
       const int i = s.sInv[base + p];
       const int q = base + p;
       const float z = s.rz[rawBase + i], phi = s.sPhi[base + i], r = s.sR[base + i];
-      s.cxy[q] = make_float2(s.rx[rawBase + i], s.ry[rawBase + i]);
+      s.cxyrz[q] = make_float4(s.rx[rawBase + i], s.ry[rawBase + i], r, z);
       s.cz[q] = z;
       s.cphi[q] = phi;
       s.cr[q] = r;
@@ -838,11 +835,10 @@ __device__ __forceinline__ void writeTracklet(const DevState& s, int k, int i, i
 {
   if (k >= s.tCap) return;
   const int pi = curBase + i, q = nextBase + j;
-  const float zi = s.cz[pi], ri = s.cr[pi];
   s.tFirst[k] = i;
   s.tSecond[k] = j;
-  const float2 a = s.cxy[pi], b = s.cxy[q];
-  s.tTP[k] = make_float2((zi - s.cz[q]) / (ri - s.cr[q]), fdlibmAtan2f(a.y - b.y, a.x - b.x));
+  const float4 a = s.cxyrz[pi], b = s.cxyrz[q];  // (x, y, r, z)
+  s.tTP[k] = make_float2((a.w - b.w) / (a.z - b.z), fdlibmAtan2f(a.y - b.y, a.x - b.x));
   s.cellCnt[k] = 0;  // stays 0 where the cell stage does not run
 }
 
@@ -963,14 +959,14 @@ __device__ __forceinline__ CellSeed cellSeed(const DevState& s, int w, int l, in
   c.vy = s.itemVertex[3 * w + 1];
   c.vz = s.itemVertex[3 * w + 2];
   const int p1 = cl0.base + s.tFirst[t], p2 = cl1.base + k;
-  const float2 xy1 = s.cxy[p1], xy2 = s.cxy[p2];
-  c.x1 = xy1.x;
-  c.y1 = xy1.y;
-  c.z1 = s.cz[p1];
-  c.r1 = s.cr[p1];
-  c.x2 = xy2.x;
-  c.y2 = xy2.y;
-  const float r2 = s.cr[p2];
+  const float4 q1 = s.cxyrz[p1], q2 = s.cxyrz[p2];  // (x, y, r, z)
+  c.x1 = q1.x;
+  c.y1 = q1.y;
+  c.z1 = q1.w;
+  c.r1 = q1.z;
+  c.x2 = q2.x;
+  c.y2 = q2.y;
+  const float r2 = q2.z;
   c.r1sq = c.r1 * c.r1;
   c.r2sq = r2 * r2;
   c.d1x = c.x2 - c.x1;
@@ -1005,8 +1001,8 @@ __device__ __forceinline__ bool cellFit(const DevState& s, const CellSeed& c, in
 {
   const int u = c.tnBase + iu;
   const int p3 = c.cl2Base + s.tSecond[u];
-  const float2 xy3 = s.cxy[p3];
-  const float x3 = xy3.x, y3 = xy3.y, r3 = s.cr[p3];
+  const float4 q3 = s.cxyrz[p3];  // (x, y, r, z)
+  const float x3 = q3.x, y3 = q3.y, r3 = q3.z;
   const float r3sq = r3 * r3;
   const float d2x = x3 - c.x1, d2y = y3 - c.y1, d2z = r3sq - c.r1sq;
   const float nx = (c.d1y * d2z) - (c.d1z * d2y);

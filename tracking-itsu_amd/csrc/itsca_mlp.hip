@@ -157,9 +157,8 @@ __device__ __forceinline__ void packFeatures(const float f[kMlpIn], uint16_t* ro
 // record is at ri and whose tracklet count (of this work item) is at ci.
 __device__ __forceinline__ TrackletSeed seedOf(const DevState& s, float vx, float vy, float vz, int ri, int ci)
 {
-  const float4 a = s.crzp[ri];  // (r, z, phi, 0)
-  const float2 xy = s.cxy[ri];
-  return trackletSeed(xy.x, xy.y, a.y, a.x, a.z, vx, vy, vz, s.trkCnt[ci]);
+  const float4 a = s.cxyrz[ri];  // (x, y, r, z)
+  return trackletSeed(a.x, a.y, a.w, a.z, s.cphi[ri], vx, vy, vz, s.trkCnt[ci]);
 }
 
 // Features of candidate j (sorted cluster of layer l+1; record at rj, count at
@@ -167,9 +166,8 @@ __device__ __forceinline__ TrackletSeed seedOf(const DevState& s, float vx, floa
 __device__ __forceinline__ void featuresOf(const DevState& s, const TrackletSeed& t, float vx, float vy, float vz, int l,
                                            int rj, int cj, float f[kMlpIn])
 {
-  const float4 b = s.crzp[rj];
-  const float2 xy = s.cxy[rj];
-  candidateFeatures(t, l, xy.x - vx, xy.y - vy, b.y, b.x, b.z, vz,
+  const float4 b = s.cxyrz[rj];  // (x, y, r, z)
+  candidateFeatures(t, l, b.x - vx, b.y - vy, b.w, b.z, s.cphi[rj], vz,
                     l + 1 < kTrackletLayers ? s.trkCnt[cj] : 0, f);
 }
 

@@ -111,7 +111,7 @@ struct itsca_ctx {
   DBuf<int> rid, rmc, evLayerOff, dItemEvent, dRecItem;
   DBuf<Seg> dClSeg, dTrkSeg, dCelSeg, dEdgSeg, dRoadSeg;
   DBuf<float> cz, cphi, cr, sPhi, sR;
-  DBuf<float2> cxy;
+  DBuf<float4> cxyrz;
   DBuf<int> cid, cmc, cbin, sBin, sInv, trkCnt, trkOff, trkKept, table;
   DBuf<float4> crzp, crzpS, segGeo;
   DBuf<uint4> subTab;
@@ -173,7 +173,7 @@ struct itsca_ctx {
     const size_t SC = (size_t)std::max<long>(sortedClusters, 1);
     maxSegs = kLayers * std::max(W, 1);
     maxChunks = (int)((std::max<size_t>({SC, (size_t)tCap, (size_t)cCap}) + kMinChunk - 1) / kMinChunk) + 8 * W + 1;
-    HIP_TRY(cxy.reserve(SC));
+    HIP_TRY(cxyrz.reserve(SC));
     HIP_TRY(cz.reserve(SC));
     HIP_TRY(cphi.reserve(SC));
     HIP_TRY(cr.reserve(SC));
@@ -295,7 +295,7 @@ struct itsca_ctx {
     s.rmc = rmc.p;
     s.evLayerOff = evLayerOff.p;
     s.clSeg = dClSeg.p;
-    s.cxy = cxy.p;
+    s.cxyrz = cxyrz.p;
     s.cz = cz.p;
     s.cphi = cphi.p;
     s.cr = cr.p;
@@ -681,7 +681,7 @@ itsca_status itsca_download(itsca_ctx* ctx, int item, int what, int layer, void*
              mc = fetch(ctx->cmc.p + g.base, g.count);
         auto phi = asInt(fetch(ctx->cphi.p + g.base, g.count)), r = asInt(fetch(ctx->cr.p + g.base, g.count)),
              z = asInt(fetch(ctx->cz.p + g.base, g.count));
-        auto xy = fetch(ctx->cxy.p + g.base, g.count);
+        auto xy = fetch(ctx->cxyrz.p + g.base, g.count);
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         std::vector<int32_t> x(g.count), y(g.count);
         for (int i = 0; i < g.count; ++i) {
