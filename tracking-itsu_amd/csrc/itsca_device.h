@@ -91,7 +91,7 @@ struct DevState {
   const uint16_t* mlpW;  // learned filter parameters, BF16 bits
   int* trkCand;          // kTrkSlots per sorted cluster
   Seg* trkSeg;           // W*6
-  int *tFirst, *tSecond;
+  int2* tFS;    // (first cluster, second cluster) positions
   float2* tTP;  // (tanLambda, phi): one load per cell candidate
   int tCap;
   // cells

@@ -835,8 +835,7 @@ __device__ __forceinline__ void writeTracklet(const DevState& s, int k, int i, i
 {
   if (k >= s.tCap) return;
   const int pi = curBase + i, q = nextBase + j;
-  s.tFirst[k] = i;
-  s.tSecond[k] = j;
+  s.tFS[k] = make_int2(i, j);
   const float4 a = s.cxyrz[pi], b = s.cxyrz[q];  // (x, y, r, z)
   s.tTP[k] = make_float2((a.w - b.w) / (a.z - b.z), fdlibmAtan2f(a.y - b.y, a.x - b.x));
   s.cellCnt[k] = 0;  // stays 0 where the cell stage does not run
@@ -952,13 +951,14 @@ __device__ __forceinline__ CellSeed cellSeed(const DevState& s, int w, int l, in
   const Seg cl0 = recSeg(s, w, l), cl1 = recSeg(s, w, l + 1);  // records
   c.cl2Base = recSeg(s, w, l + 2).base;
   const int item1 = s.clSeg[w * kLayers + l + 1].base;  // the item's own counts and offsets
-  const int k = s.tSecond[t];
+  const int2 fs = s.tFS[t];
+  const int k = fs.y;
   c.nCand = s.trkCnt[item1 + k];  // tracklets of layer l+1 starting at this cluster
   c.uFirst = c.nCand > 0 ? s.trkOff[item1 + k] : 0;
   c.vx = s.itemVertex[3 * w];
   c.vy = s.itemVertex[3 * w + 1];
   c.vz = s.itemVertex[3 * w + 2];
-  const int p1 = cl0.base + s.tFirst[t], p2 = cl1.base + k;
+  const int p1 = cl0.base + fs.x, p2 = cl1.base + k;
   const float4 q1 = s.cxyrz[p1], q2 = s.cxyrz[p2];  // (x, y, r, z)
   c.x1 = q1.x;
   c.y1 = q1.y;
@@ -1000,7 +1000,7 @@ __device__ __forceinline__ bool cellCheapCuts(const CellSeed& c, float2 up)
 __device__ __forceinline__ bool cellFit(const DevState& s, const CellSeed& c, int iu, CellOut& o)
 {
   const int u = c.tnBase + iu;
-  const int p3 = c.cl2Base + s.tSecond[u];
+  const int p3 = c.cl2Base + s.tFS[u].y;
   const float4 q3 = s.cxyrz[p3];  // (x, y, r, z)
   const float x3 = q3.x, y3 = q3.y, r3 = q3.z;
   const float r3sq = r3 * r3;
@@ -1101,13 +1101,13 @@ __global__ void __launch_bounds__(B) kCellsCount(DevState s)
   }
 }
 
-__device__ __forceinline__ void writeCell(const DevState& s, const CellSeed& c, int k, int it, int iu,
-                                          const CellOut& o, int first, int second)
+__device__ __forceinline__ void writeCell(const DevState& s, int tnBase, int k, int it, int iu, const CellOut& o,
+                                          int first, int second)
 {
   if (k >= s.cCap) return;
   s.c1[k] = first;
   s.c2[k] = second;
-  s.c3[k] = s.tSecond[c.tnBase + iu];
+  s.c3[k] = s.tFS[tnBase + iu].y;
   s.ct1[k] = it;
   s.ct2[k] = iu;
   s.cnx[k] = o.nx;
@@ -1152,7 +1152,8 @@ __global__ void __launch_bounds__(B) kCellsFill(DevState s)
       const CellSeed cs = cellSeed(s, w, l, tt);
       CellOut out;
       cellFit(s, cs, iu, out);  // accepted in the count pass; same inputs, same result
-      writeCell(s, cs, chunkBase + exFull[c] + n, firstT + c, iu, out, s.tFirst[tt], s.tSecond[tt]);
+      const int2 fs = s.tFS[tt];
+      writeCell(s, cs.tnBase, chunkBase + exFull[c] + n, firstT + c, iu, out, fs.x, fs.y);
     });
     if (cnt > kCellSlots) {
       const CellSeed cs = cellSeed(s, w, l, t);
@@ -1160,7 +1161,7 @@ __global__ void __launch_bounds__(B) kCellsFill(DevState s)
       for (int iu = cs.uFirst; iu < cs.uFirst + cs.nCand; ++iu) {
         CellOut out;
         if (cellCheapCuts(cs, s.tTP[cs.tnBase + iu]) && cellFit(s, cs, iu, out))
-          writeCell(s, cs, k++, it, iu, out, s.tFirst[t], s.tSecond[t]);
+          writeCell(s, cs.tnBase, k++, it, iu, out, s.tFS[t].x, s.tFS[t].y);
       }
     }
   }

@@ -359,7 +359,7 @@ __global__ void kCollect(DevState s, const uint8_t* label, int total, int stride
         hi = mid;
     }
     const int w = lo / kTrackletLayers, l = lo % kTrackletLayers;
-    const int i = s.tFirst[t], j = s.tSecond[t];
+    const int i = s.tFS[t].x, j = s.tFS[t].y;
     float f[kMlpIn];
     const float vx = s.itemVertex[3 * w], vy = s.itemVertex[3 * w + 1], vz = s.itemVertex[3 * w + 2];
     featuresOf(s, seedOf(s, vx, vy, vz, recSeg(s, w, l).base + i, s.clSeg[w * kLayers + l].base + i), vx, vy, vz,

@@ -115,7 +115,8 @@ struct itsca_ctx {
   DBuf<int> cid, cmc, cbin, sBin, sInv, trkCnt, trkOff, trkKept, table;
   DBuf<float4> crzp, crzpS, segGeo;
   DBuf<uint4> subTab;
-  DBuf<int> tFirst, tSecond, cellCnt, cellOff, trkCand, cellCand;
+  DBuf<int> cellCnt, cellOff, trkCand, cellCand;
+  DBuf<int2> tFS;
   DBuf<float2> tTP;
   DBuf<int> succCnt, succCand, eFrom, eTo, eTmp;
   DBuf<int> c1, c2, c3, ct1, ct2, clevel, edgeCnt, edgeOff, roadOff;
@@ -193,8 +194,7 @@ struct itsca_ctx {
     HIP_TRY(trkCand.reserve(SC * kTrkSlots));
     HIP_TRY(table.reserve((size_t)W * kLayers * kTableSize));
     HIP_TRY(segGeo.reserve((size_t)std::max(W, 1) * kLayers));
-    HIP_TRY(tFirst.reserve(tCap));
-    HIP_TRY(tSecond.reserve(tCap));
+    HIP_TRY(tFS.reserve(tCap));
     HIP_TRY(tTP.reserve(tCap));
     HIP_TRY(cellCnt.reserve(tCap));
     HIP_TRY(cellOff.reserve(tCap));
@@ -318,8 +318,7 @@ struct itsca_ctx {
     s.mlpW = mlpWbf.p;
     s.trkCand = trkCand.p;
     s.trkSeg = dTrkSeg.p;
-    s.tFirst = tFirst.p;
-    s.tSecond = tSecond.p;
+    s.tFS = tFS.p;
     s.tTP = tTP.p;
     s.tCap = tCap;
     s.cellCnt = cellCnt.p;
@@ -699,11 +698,13 @@ itsca_status itsca_download(itsca_ctx* ctx, int item, int what, int layer, void*
       case ITSCA_TRACKLETS: {
         if (!layerOk(0, 5)) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_download: bad layer");
         const Seg g = ctx->trkSeg[item * kTrackletLayers + layer];
-        auto a = fetch(ctx->tFirst.p + g.base, g.count), b = fetch(ctx->tSecond.p + g.base, g.count);
+        auto fs = fetch(ctx->tFS.p + g.base, g.count);
         auto tp = fetch(ctx->tTP.p + g.base, g.count);
         HIP_TRY(hipStreamSynchronize(ctx->stream));
-        std::vector<int32_t> t(g.count), p(g.count);
+        std::vector<int32_t> a(g.count), b(g.count), t(g.count), p(g.count);
         for (int i = 0; i < g.count; ++i) {
+          a[i] = fs[i].x;
+          b[i] = fs[i].y;
           std::memcpy(&t[i], &tp[i].x, 4);
           std::memcpy(&p[i], &tp[i].y, 4);
         }
