@@ -21,6 +21,7 @@
 #define __host__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+#define amdgpu_waves_per_eu(...)
 #define __shared__ static
 
 struct dim3 {

@@ -197,13 +197,22 @@ __device__ __forceinline__ int segmentOf(const int* chunkStart, int S, int chunk
 // own position, and the round's slots were all read before, so nothing still
 // to be read is overwritten).  Clusters whose candidates overflowed their
 // slots are kept whole.  Kept counts go to trkKept (trkCnt stays the raw count
-// that the features read) and the chunk sums are rewritten.
+// that the features read) and the chunk sums are rewritten.  The LDS arrays
+// allow four workgroups of 256 per CU; the waves-per-SIMD hint holds the
+// registers to that occupancy.
 // ---------------------------------------------------------------------------
 constexpr int kWaveRound = 1024;  // candidate slots per wavefront round
+
+// Record of one candidate cluster j: (x, y, r, z), phi and its tracklet count.
+struct CandRec {
+  float4 b;
+  float phi;
+  int count;
+};
 constexpr int kSeedFloats = 8;    // TrackletSeed
 
 template <int B>
-__global__ void __launch_bounds__(B) kTrackletsScore(DevState s, float threshold)
+__global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) kTrackletsScore(DevState s, float threshold)
 {
   constexpr int kWaves = B / 64;
   __shared__ uint8_t owner[kWaves][kWaveRound];
@@ -248,18 +257,40 @@ __global__ void __launch_bounds__(B) kTrackletsScore(DevState s, float threshold
       const int rn = min(kWaveRound, total - r0);
       for (int k = max(0, r0 - ex); k < scored && ex + k < r0 + rn; ++k) owner[wave][ex + k - r0] = (uint8_t)lane;
       waveSync();  // owners and seeds visible to the wavefront
-      for (int tile = 0; tile * 64 < rn; ++tile) {
-        const int o = tile * 64 + lane;  // slot index within the round
+      // Loads run two tiles ahead: the slot of tile t+2 and the records of
+      // tile t+1 are in flight while tile t is scored.  All of them lie in
+      // this round's slots, which the compaction below rewrites only after
+      // the last tile.
+      const int nTiles = (rn + 63) >> 6;
+      auto slotOf = [&](int tile) {  // candidate j of this lane's slot in the tile, -1 past the round
+        const int o = tile * 64 + lane;
         const int c = o < rn ? owner[wave][o] : 0;
         const int exc = __shfl(ex, c, 64);
+        return o < rn ? s.trkCand[(size_t)(rowBase + c) * kTrkSlots + (r0 + o - exc)] : -1;
+      };
+      auto recOf = [&](int tile, int j) {  // also keeps j for the compaction
+        CandRec q{make_float4(0.0f, 0.0f, 0.0f, 0.0f), 0.0f, 0};
+        if (j >= 0) {
+          candv[wave][tile * 64 + lane] = j;
+          q.b = s.cxyrz[recNext + j];
+          q.phi = s.cphi[recNext + j];
+          q.count = l + 1 < kTrackletLayers ? s.trkCnt[next.base + j] : 0;
+        }
+        return q;
+      };
+      CandRec qCur = recOf(0, slotOf(0));
+      int jNext = nTiles > 1 ? slotOf(1) : -1;
+      for (int tile = 0; tile < nTiles; ++tile) {
+        const int jAfter = tile + 2 < nTiles ? slotOf(tile + 2) : -1;
+        const CandRec qNext = recOf(tile + 1, jNext);
+        const int o = tile * 64 + lane;  // slot index within the round
         float f[kMlpIn];
         if (o < rn) {
-          const int j = s.trkCand[(size_t)(rowBase + c) * kTrkSlots + (r0 + o - exc)];
-          candv[wave][o] = j;
+          const int c = owner[wave][o];
           TrackletSeed t;
           float* tv = reinterpret_cast<float*>(&t);
           for (int k = 0; k < kSeedFloats; ++k) tv[k] = seeds[k][wave * 64 + c];
-          featuresOf(s, t, vx, vy, vz, l, recNext + j, next.base + j, f);
+          candidateFeatures(t, l, qCur.b.x - vx, qCur.b.y - vy, qCur.b.w, qCur.b.z, qCur.phi, vz, qCur.count, f);
         } else {
           for (int k = 0; k < kMlpIn; ++k) f[k] = 0.0f;
         }
@@ -272,6 +303,8 @@ __global__ void __launch_bounds__(B) kTrackletsScore(DevState s, float threshold
           keep[wave][2 * tile + 1] = (uint32_t)(kb >> 32);
         }
         waveSync();
+        qCur = qNext;
+        jNext = jAfter;
       }
       // this lane's candidates in the round: [a, b); kept ones go to the row in order
       const int a = max(ex - r0, 0), b = min(ex + scored - r0, rn);
