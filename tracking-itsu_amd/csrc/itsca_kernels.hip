@@ -996,12 +996,10 @@ __device__ __forceinline__ bool cellCheapCuts(const CellSeed& c, float2 up)
   return continuity & (deltaZ < cellMaxDeltaZ(c.l));
 }
 
-// Conformal-plane fit and DCA cut (Tracker.cxx:175-218, MathUtils.h:53-60).
-__device__ __forceinline__ bool cellFit(const DevState& s, const CellSeed& c, int iu, CellOut& o)
+// Conformal-plane fit and DCA cut (Tracker.cxx:175-218, MathUtils.h:53-60)
+// of the candidate whose third cluster has the record q3 = (x, y, r, z).
+__device__ __forceinline__ bool cellFitAt(const CellSeed& c, float4 q3, CellOut& o)
 {
-  const int u = c.tnBase + iu;
-  const int p3 = c.cl2Base + s.tFS[u].y;
-  const float4 q3 = s.cxyrz[p3];  // (x, y, r, z)
   const float x3 = q3.x, y3 = q3.y, r3 = q3.z;
   const float r3sq = r3 * r3;
   const float d2x = x3 - c.x1, d2y = y3 - c.y1, d2z = r3sq - c.r1sq;
@@ -1025,7 +1023,18 @@ __device__ __forceinline__ bool cellFit(const DevState& s, const CellSeed& c, in
   return true;
 }
 
+__device__ __forceinline__ float4 cellThird(const DevState& s, const CellSeed& c, int iu)
+{
+  return s.cxyrz[c.cl2Base + s.tFS[c.tnBase + iu].y];
+}
+
+__device__ __forceinline__ bool cellFit(const DevState& s, const CellSeed& c, int iu, CellOut& o)
+{
+  return cellFitAt(c, cellThird(s, c, iu), o);
+}
+
 constexpr int kCellList = 8;  // cheap-cut survivors kept per thread before the fit
+constexpr int kFitBatch = 4;  // survivors whose fit loads are issued together
 
 // Count: the cheap cuts run over every candidate; survivors (usually 0-2) are
 // fitted in a second loop, so the expensive fit does not sit inside the
@@ -1084,13 +1093,21 @@ __global__ void __launch_bounds__(B) kCellsCount(DevState s)
           survivor(ib + k);
         }
       }
-      if (m <= kCellList) {
-        for (int q = 0; q < m; ++q) {
-          CellOut o;
-          const int iu = list[q][threadIdx.x];
-          if (cellFit(s, c, iu, o)) {
-            if (cnt < kCellSlots) slots[cnt] = iu;
-            ++cnt;
+      if (m <= kCellList) {  // survivors fitted kFitBatch at a time, their loads issued together
+        for (int q0 = 0; q0 < m; q0 += kFitBatch) {
+          int iu[kFitBatch];
+          float4 q3[kFitBatch];
+#pragma unroll
+          for (int b = 0; b < kFitBatch; ++b) iu[b] = list[min(q0 + b, m - 1)][threadIdx.x];
+#pragma unroll
+          for (int b = 0; b < kFitBatch; ++b) q3[b] = cellThird(s, c, iu[b]);
+#pragma unroll
+          for (int b = 0; b < kFitBatch; ++b) {
+            CellOut o;
+            if (q0 + b < m && cellFitAt(c, q3[b], o)) {
+              if (cnt < kCellSlots) slots[cnt] = iu[b];
+              ++cnt;
+            }
           }
         }
       }
