@@ -491,6 +491,7 @@ __device__ __forceinline__ TrackletWindow trackletWindow(float z, float r, float
 // candidates and their order are unchanged.
 constexpr float kZSkipMargin = 0.01f;
 
+
 // The search of one cluster: its window, and the z bins of each window row
 // whose clusters are tested.
 struct TrackletSearch {
@@ -563,8 +564,10 @@ __device__ __forceinline__ int subStart(uint4 e, int t)
 // sorted order, as one range.  The reference's extra candidate (the inclusive
 // upper bound) is tested last, from the sorted records in global memory.
 // kStaged: the records of the tested bins are in the LDS copy (search-index
-// order with more than one z sub-bin, else sorted order).
-template <bool kStaged, typename F>
+// order with more than one z sub-bin, else sorted order).  kScanBlock: window
+// candidates whose loads are issued together (more registers, fewer partial
+// blocks).
+template <bool kStaged, int kScanBlock, typename F>
 __device__ __forceinline__ void searchRows(const TrackletSearch& t, const int* table, int nNext, int l, float z,
                                            float r, float phi, const CandidateSource& src, F&& emit)
 {
@@ -578,21 +581,21 @@ __device__ __forceinline__ void searchRows(const TrackletSearch& t, const int* t
   };
   auto loadGlobal = [](const char* g, int j) { return *reinterpret_cast<const float4*>(g + 16u * (unsigned)j); };
   uint64_t bits0 = 0, bits1 = 0;  // accepted offsets 0..63 and 64..127 of the current bin
-  // Tests the candidates [j0, j1) 8 at a time: all 8 loads are issued first,
-  // the cuts are combined without short-circuit branches, and in the last,
-  // partial block the indices past the range are clamped to its last element
-  // and masked out.  kBits: accepted candidates set the bit of their offset in
+  // Tests the candidates [j0, j1) kScanBlock at a time: the block's loads are
+  // issued first, the cuts are combined without short-circuit branches, and in
+  // the last, partial block the indices past the range are clamped to its last
+  // element and masked out.  kBits: accepted candidates set the bit of their offset in
   // the bin (record w); else they are emitted in ascending order.
   auto scan = [&](int j0, int j1, auto load, auto bitsMode) {
     constexpr bool kBits = decltype(bitsMode)::value;
     auto block = [&](int jb, auto full) {
       constexpr bool kFull = decltype(full)::value;
-      float4 q[8];
+      float4 q[kScanBlock];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) q[k] = load(kFull ? jb + k : min(jb + k, j1 - 1));
+      for (int k = 0; k < kScanBlock; ++k) q[k] = load(kFull ? jb + k : min(jb + k, j1 - 1));
       uint32_t mask = 0;
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
+      for (int k = 0; k < kScanBlock; ++k) {
         const bool ok = passes(q[k]) & (kFull || jb + k < j1);
         if (kBits) {
           const int o = __float_as_int(q[k].w);
@@ -611,7 +614,7 @@ __device__ __forceinline__ void searchRows(const TrackletSearch& t, const int* t
       }
     };
     int jb = j0;
-    for (; jb + 8 <= j1; jb += 8) block(jb, std::true_type{});
+    for (; jb + kScanBlock <= j1; jb += kScanBlock) block(jb, std::true_type{});
     if (jb < j1) block(jb, std::false_type{});
   };
   int row = win.y;
@@ -686,7 +689,7 @@ __device__ __forceinline__ void forEachTracklet(const DevState& s, int rw, int l
                             nullptr,
                             nullptr,
                             0};
-  searchRows<false>(t, table, nNext, l, z, r, phi, src, emit);
+  searchRows<false, 2>(t, table, nNext, l, z, r, phi, src, emit);  // rare path: few registers
 }
 
 // LDS copy of the next layer for the tracklet count.  A chunk's clusters are
@@ -820,9 +823,9 @@ __global__ void __launch_bounds__(B) kTrackletsCount(DevState s)
                                 rowOff,
                                 copyRef};
       if (nrows > 0 && (t.sub || s.zSub == 1))  // with sub-bins the copy serves only the bin-by-bin search
-        searchRows<true>(t, table, next.count, l, z, r, phi, src, emit);
+        searchRows<true, 8>(t, table, next.count, l, z, r, phi, src, emit);
       else
-        searchRows<false>(t, table, next.count, l, z, r, phi, src, emit);
+        searchRows<false, 8>(t, table, next.count, l, z, r, phi, src, emit);
       s.trkCnt[p] = cnt;
     }
     const int total = blockSum(cnt, scratch);  // its barriers also end this chunk's use of the copy
@@ -869,9 +872,11 @@ __device__ __forceinline__ void forEachSlotOutput(int capped, int exc, int total
 
 // Fill: one output record per thread of the chunk (coalesced writes, no
 // divergent window search); clusters with more than kTrkSlots tracklets are
-// recomputed by their own thread.
+// recomputed by their own thread.  Six waves per SIMD: the overflow search
+// tests its candidates two at a time so that the kernel fits that many
+// without spills.
 template <int B>
-__global__ void __launch_bounds__(B) kTrackletsFill(DevState s)
+__global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) kTrackletsFill(DevState s)
 {
   __shared__ int scratch[B / 64 + 1];
   __shared__ int exFull[B], exCap[B];
@@ -1035,6 +1040,7 @@ __device__ __forceinline__ bool cellFit(const DevState& s, const CellSeed& c, in
 
 constexpr int kCellList = 8;  // cheap-cut survivors kept per thread before the fit
 constexpr int kFitBatch = 4;  // survivors whose fit loads are issued together
+constexpr int kCandBlock = 4;  // candidates whose loads are issued together
 
 // Count: the cheap cuts run over every candidate; survivors (usually 0-2) are
 // fitted in a second loop, so the expensive fit does not sit inside the
@@ -1078,15 +1084,16 @@ __global__ void __launch_bounds__(B) kCellsCount(DevState s)
         }
         ++m;
       };
-      // Blocks of 8 candidates: loads first (indices past the range clamped
-      // and masked out), cuts without branches, survivors in ascending order.
-      for (int ib = c.uFirst; ib < uEnd; ib += 8) {
-        float2 up[8];
+      // Blocks of kCandBlock candidates: loads first (indices past the range
+      // clamped and masked out), cuts without branches, survivors in
+      // ascending order.
+      for (int ib = c.uFirst; ib < uEnd; ib += kCandBlock) {
+        float2 up[kCandBlock];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) up[k] = s.tTP[c.tnBase + min(ib + k, uEnd - 1)];
+        for (int k = 0; k < kCandBlock; ++k) up[k] = s.tTP[c.tnBase + min(ib + k, uEnd - 1)];
         uint32_t mask = 0;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) mask |= ((cellCheapCuts(c, up[k]) & (ib + k < uEnd)) ? 1u : 0u) << k;
+        for (int k = 0; k < kCandBlock; ++k) mask |= ((cellCheapCuts(c, up[k]) & (ib + k < uEnd)) ? 1u : 0u) << k;
         while (mask != 0u) {
           const int k = __builtin_ctz(mask);
           mask &= mask - 1u;
@@ -1138,8 +1145,9 @@ __device__ __forceinline__ void writeCell(const DevState& s, int tnBase, int k, 
 
 // Fill: one cell per thread of the chunk, refitted from the kept candidate;
 // tracklets with more than kCellSlots cells are redone by their own thread.
+// Eight waves per SIMD (the registers fit without spills).
 template <int B>
-__global__ void __launch_bounds__(B) kCellsFill(DevState s)
+__global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(8))) kCellsFill(DevState s)
 {
   __shared__ int scratch[B / 64 + 1];
   __shared__ int exFull[B], exCap[B];
