@@ -696,13 +696,14 @@ __device__ __forceinline__ void forEachTracklet(const DevState& s, int rw, int l
 // consecutive in (phi row, z bin) order, so the union of their windows is a
 // few phi rows times a range of z bins; those clusters of the next layer are
 // copied once per chunk and read from LDS by every cluster of the chunk.
-constexpr int kStageCap = 2048;  // next-layer clusters per chunk (32 KB)
+constexpr int kStageCap = 1984;  // next-layer clusters per chunk (31 KB: five workgroups of 256 per CU)
 constexpr int kStageRows = 8;    // phi rows of the copy
 
 // Count: each cluster's accepted candidates, in emission order, are kept in
 // kTrkSlots scratch slots so that the fill does not repeat the window search.
+// Five waves per SIMD (the LDS copy allows five workgroups of 256 per CU).
 template <int B>
-__global__ void __launch_bounds__(B) kTrackletsCount(DevState s)
+__global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) kTrackletsCount(DevState s)
 {
   __shared__ int scratch[B / 64 + 1];
   __shared__ float4 stage[kStageCap];
