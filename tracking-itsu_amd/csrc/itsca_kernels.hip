@@ -95,11 +95,13 @@ __device__ __forceinline__ int segmentOfChunk(const int* chunkStart, int S, int 
 // clusters are sorted through an LDS array of that many ints: 64 KB for
 // kPrepPer = 16 (two workgroups per CU), 128 KB for kPrepPer = 32 (batches
 // with larger layers, such as pile-up events).  Positions are packed as 16
-// bits, so kPrepPer * 1024 <= 65536.
+// bits, so kPrepPer * 1024 <= 65536.  The waves-per-SIMD hint keeps the
+// scalar registers to what eight waves allow: at seven, only one workgroup of
+// 16 waves fits a CU.
 constexpr int kPrepThreads = 1024;
 
 template <int kPrepPer>
-__global__ void __launch_bounds__(1024) kPrep(DevState s)
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) kPrep(DevState s)
 {
   constexpr int kPrepLds = kPrepPer * kPrepThreads;
   static_assert(kPrepLds <= 65536, "sorted positions are packed as 16 bits");
