@@ -99,7 +99,7 @@ struct DevState {
   int* cellCand;           // kCellSlots per tracklet
   Seg* celSeg;             // W*5
   int *c1, *c2, *c3, *ct1, *ct2, *clevel;
-  float *cnx, *cny, *cnz, *ccurv;
+  float4* cnc;  // (nx, ny, nz, curvature) of the cell
   uint32_t* cpaths;  // number of level-descending paths from layer 0
   int cCap;
   // neighbour edges (c of layer l -> n of layer l+1), first in (c, n) order,

@@ -1137,10 +1137,7 @@ __device__ __forceinline__ void writeCell(const DevState& s, int tnBase, int k, 
   s.c3[k] = s.tFS[tnBase + iu].y;
   s.ct1[k] = it;
   s.ct2[k] = iu;
-  s.cnx[k] = o.nx;
-  s.cny[k] = o.ny;
-  s.cnz[k] = o.nz;
-  s.ccurv[k] = o.curvature;
+  s.cnc[k] = make_float4(o.nx, o.ny, o.nz, o.curvature);
   s.clevel[k] = 1;
   s.cpaths[k] = 1;
   s.edgeCnt[k] = 0;
@@ -1225,14 +1222,20 @@ __device__ __forceinline__ void forEachSuccessor(const DevState& s, int w, int l
   const int cnt = s.cellCnt[u];
   if (cnt == 0) return;
   const int first = s.cellOff[u];
-  const float nx = s.cnx[c], ny = s.cny[c], nz = s.cnz[c], curv = s.ccurv[c];
+  const float4 a = s.cnc[c];  // (nx, ny, nz, curvature)
   const float maxNormal = neighbourMaxNormalDelta(l), maxCurv = neighbourMaxCurvatureDelta(l);
-  for (int in = first; in < first + cnt; ++in) {
-    const int n = cnBase + in;
-    const float dx = nx - s.cnx[n], dy = ny - s.cny[n], dz = nz - s.cnz[n];
-    const float deltaNormal = (dx * dx) + (dy * dy) + (dz * dz);
-    const float deltaCurvature = fabsf(curv - s.ccurv[n]);
-    if (deltaNormal < maxNormal && deltaCurvature < maxCurv) emit(in);
+  const int end = first + cnt;
+  for (int ib = first; ib < end; ib += 4) {  // four candidates at a time, loads first
+    float4 b[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) b[k] = s.cnc[cnBase + min(ib + k, end - 1)];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float dx = a.x - b[k].x, dy = a.y - b[k].y, dz = a.z - b[k].z;
+      const float deltaNormal = (dx * dx) + (dy * dy) + (dz * dz);
+      const float deltaCurvature = fabsf(a.w - b[k].w);
+      if (ib + k < end && deltaNormal < maxNormal && deltaCurvature < maxCurv) emit(ib + k);
+    }
   }
 }
 

@@ -120,7 +120,7 @@ struct itsca_ctx {
   DBuf<float2> tTP;
   DBuf<int> succCnt, succCand, eFrom, eTo, eTmp;
   DBuf<int> c1, c2, c3, ct1, ct2, clevel, edgeCnt, edgeOff, roadOff;
-  DBuf<float> cnx, cny, cnz, ccurv;
+  DBuf<float4> cnc;
   DBuf<uint32_t> cpaths;
   DBuf<int> pred, rCells, rLabel, rFake;
   DBuf<int> chunkStart, chunkSum, chunkOff, planScratch, dStatus;
@@ -205,10 +205,7 @@ struct itsca_ctx {
     HIP_TRY(ct1.reserve(cCap));
     HIP_TRY(ct2.reserve(cCap));
     HIP_TRY(clevel.reserve(cCap));
-    HIP_TRY(cnx.reserve(cCap));
-    HIP_TRY(cny.reserve(cCap));
-    HIP_TRY(cnz.reserve(cCap));
-    HIP_TRY(ccurv.reserve(cCap));
+    HIP_TRY(cnc.reserve(cCap));
     HIP_TRY(cpaths.reserve(cCap));
     HIP_TRY(edgeCnt.reserve(cCap));
     HIP_TRY(edgeOff.reserve(cCap));
@@ -331,10 +328,7 @@ struct itsca_ctx {
     s.ct1 = ct1.p;
     s.ct2 = ct2.p;
     s.clevel = clevel.p;
-    s.cnx = cnx.p;
-    s.cny = cny.p;
-    s.cnz = cnz.p;
-    s.ccurv = ccurv.p;
+    s.cnc = cnc.p;
     s.cpaths = cpaths.p;
     s.cCap = cCap;
     s.succCnt = succCnt.p;
@@ -726,9 +720,15 @@ itsca_status itsca_download(itsca_ctx* ctx, int item, int what, int layer, void*
         auto a = fetch(ctx->c1.p + g.base, g.count), b = fetch(ctx->c2.p + g.base, g.count),
              c = fetch(ctx->c3.p + g.base, g.count), t1 = fetch(ctx->ct1.p + g.base, g.count),
              t2 = fetch(ctx->ct2.p + g.base, g.count), lv = fetch(ctx->clevel.p + g.base, g.count);
-        auto nx = asInt(fetch(ctx->cnx.p + g.base, g.count)), ny = asInt(fetch(ctx->cny.p + g.base, g.count)),
-             nz = asInt(fetch(ctx->cnz.p + g.base, g.count)), cv = asInt(fetch(ctx->ccurv.p + g.base, g.count));
+        auto nc = fetch(ctx->cnc.p + g.base, g.count);
         HIP_TRY(hipStreamSynchronize(ctx->stream));
+        std::vector<int32_t> nx(g.count), ny(g.count), nz(g.count), cv(g.count);
+        for (int i = 0; i < g.count; ++i) {
+          std::memcpy(&nx[i], &nc[i].x, 4);
+          std::memcpy(&ny[i], &nc[i].y, 4);
+          std::memcpy(&nz[i], &nc[i].z, 4);
+          std::memcpy(&cv[i], &nc[i].w, 4);
+        }
         interleave({&a, &b, &c, &t1, &t2, &nx, &ny, &nz, &cv, &lv}, g.count);
         break;
       }
