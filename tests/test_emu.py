@@ -2,8 +2,9 @@
 compiled for the host against the HIP shim of tests/emu/ (one fiber per GPU
 thread, the wavefront exchanges and the MFMA instructions emulated), with
 AddressSanitizer.  Checks the kernels' logic and bounds without a GPU: golden
-fixtures bit for bit under two tunings, the learned filter's paths, and an
-event of 1831 clusters per layer against the oracle."""
+fixtures bit for bit under two tunings, the learned filter's paths, an event
+of 1831 clusters per layer and an event whose tracklets overflow the count's
+slots, against the oracle."""
 import glob
 import os
 import subprocess
@@ -77,6 +78,17 @@ t.run()
 assert itsca.dump_item(t, 0, ev.vertices[0]) == pyoracle.OracleVertex(ev, 0, stable=True).dump()
 t.close()
 print("mixed event 3006 vs oracle ok", flush=True)
+# a cluster with more tracklets than the count's slots: the fill's own search, against the oracle
+from golden_cases import dense_spot_event
+ev = dense_spot_event()
+t = itsca.GpuTracker(persistent_blocks=4, lib=lib)
+t.set_events([ev])
+t.run()
+n = t.counts(0)
+assert np.bincount(t.download(0, itsca.TRACKLETS, 0, n.tracklets[0], 4)[:, 0]).max() > itsca.TRACKLET_SLOTS
+assert itsca.dump_item(t, 0, ev.vertices[0]) == pyoracle.OracleVertex(ev, 0, stable=True).dump()
+t.close()
+print("slot-overflow event vs oracle ok", flush=True)
 print("ALL OK")
 """
 

@@ -12,7 +12,7 @@ import pytest
 
 import itsca
 import mlp_ref as M
-from golden_cases import fixture, small_cases
+from golden_cases import dense_spot_event, fixture, small_cases
 
 pytestmark = pytest.mark.gpu
 CASES = small_cases()
@@ -230,7 +230,7 @@ def test_filter_keeps_exactly_the_tracklets_scored_above_threshold(tracker, quan
     list restricted to rows with logit >= threshold, in the same order
     (clusters whose candidates overflowed their slots are kept whole)."""
     say(f"gpu filter keeps exactly the rows above the {quantile} quantile")
-    evs = itsca.generate_events(2, 31, 1) + itsca.generate_events(5, 32, 1)
+    evs = itsca.generate_events(2, 31, 1) + itsca.generate_events(5, 32, 1) + [dense_spot_event()]
     tracker.set_filter(False)
     tracker.set_events(evs)
     tracker.run()
@@ -246,7 +246,7 @@ def test_filter_keeps_exactly_the_tracklets_scored_above_threshold(tracker, quan
     tracker.set_filter(True, threshold)
     tracker.run()
     k = 0
-    dropped = 0
+    dropped = overflowed = 0
     for i in range(items):
         n = tracker.counts(i)
         for l in range(itsca.TRACKLET_LAYERS):
@@ -257,7 +257,9 @@ def test_filter_keeps_exactly_the_tracklets_scored_above_threshold(tracker, quan
             got = tracker.download(i, itsca.TRACKLETS, l, n.tracklets[l], 4)[:, :2]
             assert np.array_equal(got, t[keep]), (i, l)
             dropped += int((~keep).sum())
+            overflowed += int((raw > itsca.TRACKLET_SLOTS).sum())
     assert k == total and 0 < dropped < total
+    assert overflowed > 0  # the dense spot's clusters are kept whole
     tracker.set_filter(False)
 
 

@@ -11,7 +11,7 @@ import pytest
 
 import itsca
 import pyoracle
-from golden_cases import central, fixture, small_cases
+from golden_cases import central, dense_spot_event, fixture, small_cases
 
 pytestmark = pytest.mark.gpu
 CASES = small_cases()
@@ -92,6 +92,22 @@ def test_pileup_event_with_large_layers_matches_oracle(tracker):
     tracker.set_tuning(256, 0, 0)
     for v in (0, 3):
         assert itsca.dump_item(tracker, v, ev.vertices[v]) == pyoracle.OracleVertex(ev, v, stable=True).dump(), v
+
+
+def test_slot_overflow_event_matches_oracle(tracker):
+    """A dense spot gives one cluster of layer 0 and one of layer 2 more
+    tracklets than the 64 slots the tracklet count keeps per cluster: the fill
+    searches those clusters again, and every array still equals the oracle's."""
+    say("gpu slot-overflow event vs oracle")
+    ev = dense_spot_event()
+    tracker.set_tuning(256, 0, 0)
+    tracker.set_events([ev])
+    tracker.run()
+    n = tracker.counts(0)
+    for layer in (0, 2):
+        t = tracker.download(0, itsca.TRACKLETS, layer, n.tracklets[layer], 4)
+        assert np.bincount(t[:, 0]).max() > itsca.TRACKLET_SLOTS, layer
+    assert itsca.dump_item(tracker, 0, ev.vertices[0]) == pyoracle.OracleVertex(ev, 0, stable=True).dump()
 
 
 def test_cli_on_merged_pileup_in_binary_format(tracker):
