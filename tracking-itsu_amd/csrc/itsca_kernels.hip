@@ -568,8 +568,9 @@ __device__ __forceinline__ int subStart(uint4 e, int t)
 // kStaged: the records of the tested bins are in the LDS copy (search-index
 // order with more than one z sub-bin, else sorted order).  kScanBlock: window
 // candidates whose loads are issued together (more registers, fewer partial
-// blocks).
-template <bool kStaged, int kScanBlock, typename F>
+// blocks).  kSub: the bin-by-bin sub-bin search is compiled in (without it the
+// caller must have src.zSub == 1).
+template <bool kStaged, int kScanBlock, bool kSub, typename F>
 __device__ __forceinline__ void searchRows(const TrackletSearch& t, const int* table, int nNext, int l, float z,
                                            float r, float phi, const CandidateSource& src, F&& emit)
 {
@@ -634,7 +635,7 @@ __device__ __forceinline__ void searchRows(const TrackletSearch& t, const int* t
       if (ri < 0) ri += kPhiBins;
       off = src.off[ri];
     }
-    if (t.sub && src.zSub > 1) {
+    if (kSub && t.sub && src.zSub > 1) {
       auto loadS = [&](int j) { return kStaged ? src.lds[j + off] : loadGlobal(src.globalS, j); };
       for (int zb = t.zFirst; zb <= t.zLast; ++zb) {  // table positions clamped like maxBin
         const int b = min(firstBin + (zb - win.x), maxBin);
@@ -691,7 +692,7 @@ __device__ __forceinline__ void forEachTracklet(const DevState& s, int rw, int l
                             nullptr,
                             nullptr,
                             0};
-  searchRows<false, 2>(t, table, nNext, l, z, r, phi, src, emit);  // rare path: few registers
+  searchRows<false, 2, true>(t, table, nNext, l, z, r, phi, src, emit);  // rare path: few registers
 }
 
 // LDS copy of the next layer for the tracklet count.  A chunk's clusters are
@@ -704,7 +705,9 @@ constexpr int kStageRows = 8;    // phi rows of the copy
 // Count: each cluster's accepted candidates, in emission order, are kept in
 // kTrkSlots scratch slots so that the fill does not repeat the window search.
 // Five waves per SIMD (the LDS copy allows five workgroups of 256 per CU).
-template <int B>
+// kSub: launched with more than one z sub-bin; the instance for one z sub-bin
+// leaves the sub-bin search out and fits five waves without spills.
+template <int B, bool kSub>
 __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) kTrackletsCount(DevState s)
 {
   __shared__ int scratch[B / 64 + 1];
@@ -826,9 +829,9 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) kTr
                                 rowOff,
                                 copyRef};
       if (nrows > 0 && (t.sub || s.zSub == 1))  // with sub-bins the copy serves only the bin-by-bin search
-        searchRows<true, 8>(t, table, next.count, l, z, r, phi, src, emit);
+        searchRows<true, 8, kSub>(t, table, next.count, l, z, r, phi, src, emit);
       else
-        searchRows<false, 8>(t, table, next.count, l, z, r, phi, src, emit);
+        searchRows<false, 8, kSub>(t, table, next.count, l, z, r, phi, src, emit);
       s.trkCnt[p] = cnt;
     }
     const int total = blockSum(cnt, scratch);  // its barriers also end this chunk's use of the copy
@@ -1544,7 +1547,10 @@ void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t str
   done("kPrep", 0);
   hipLaunchKernelGGL(kPlanTracklets, dim3(1), dim3(1024), 0, stream, s, counts);
   done("kPlanTracklets", 1);
-  hipLaunchKernelGGL(kTrackletsCount<B>, dim3(G), dim3(B), 0, stream, s);
+  if (s.zSub > 1)
+    hipLaunchKernelGGL((kTrackletsCount<B, true>), dim3(G), dim3(B), 0, stream, s);
+  else
+    hipLaunchKernelGGL((kTrackletsCount<B, false>), dim3(G), dim3(B), 0, stream, s);
   done("kTrackletsCount", 1);
   if (cfg.filter) {
     launchTrackletFilter(s, B, G, cfg.threshold, stream);
