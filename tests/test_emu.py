@@ -54,6 +54,21 @@ for name in ("small_s1", "sparse_s6"):
             k += len(trk[l])
             got = t.download(0, itsca.TRACKLETS, l, t.counts(0).tracklets[l], 4)[:, :2]
             assert np.array_equal(got, trk[l][keep]), l
+    if name == "small_s1":
+        # the MFMA forward pass (swizzled LDS tile, emulated MFMA lane layouts)
+        # against float64, as in the GPU test: two full tiles and a partial one
+        import mlp_ref as M
+        rng = np.random.default_rng(5)
+        p = (rng.normal(size=M.PARAMS) * 0.3).astype(np.float32)
+        X = M.pack_rows((rng.normal(size=(130, 10)) * 1.5).astype(np.float32))
+        t.set_mlp_params(p)
+        t.train_set(X, np.zeros(130, np.uint8))
+        z = t.train_scores(130)
+        zr = M.forward(p, X)[3]
+        err = np.abs(z - zr)
+        assert np.max(err / np.maximum(1.0, np.abs(zr))) < 3e-2
+        assert np.quantile(err, 0.99) < 1e-4 * max(1.0, float(np.max(np.abs(zr))))
+        print("mfma forward vs float64 ok", flush=True)
     t.close()
     print(name, "ok", flush=True)
 # another launch shape and an odd number of z sub-bins of the search index

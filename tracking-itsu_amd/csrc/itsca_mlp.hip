@@ -115,17 +115,46 @@ __device__ __forceinline__ MlpWave loadMlp(const uint16_t* w)
   return m;
 }
 
+// LDS feature tile: 64 sample rows of 32 bytes, each four 8-byte units of 4
+// BF16 features.  Unit u of row r is stored at unit u ^ tileSwizzle(r).  The
+// MFMA operand read takes unit grp of rows col = 0..15 (16 lanes of 8 bytes);
+// unswizzled, rows c, c+4, c+8, c+12 (c, c+8 with 64 banks) fall on the same
+// banks.  Swizzled by row bits 2 and 3, those rows use four different units,
+// and the 16-byte row stores of rows r, r+4 and r, r+8 go to different halves,
+// so both the reads and the stores are free of bank conflicts with 32 or 64
+// four-byte banks.
+__device__ __forceinline__ int tileSwizzle(int row)
+{
+  const int b2 = (row >> 2) & 1, b3 = (row >> 3) & 1;
+  return ((b2 ^ b3) << 1) | b3;
+}
+
+// Row `row` of a tile from its two 16-byte halves (units 0-1 and 2-3).
+__device__ __forceinline__ void storeTileRow(uint16_t* tile, int row, uint4 lo, uint4 hi)
+{
+  const int s = tileSwizzle(row);
+  if (s & 1) {  // unit 2k+1 before unit 2k
+    lo = make_uint4(lo.z, lo.w, lo.x, lo.y);
+    hi = make_uint4(hi.z, hi.w, hi.x, hi.y);
+  }
+  uint4* r = reinterpret_cast<uint4*>(tile + row * kMlpInPad);
+  r[s >> 1] = lo;
+  r[(s >> 1) ^ 1] = hi;
+}
+
 // Logits of 64 samples whose BF16 features (16 per sample, 32 bytes) sit in
-// `tile` (LDS, [64][16]); lane i gets the logit of sample i.  The four groups
-// of 16 samples go through the network layer by layer, so that independent
-// MFMAs cover each other's latency: 8 + 8 + 4 MFMAs per 64 samples.
+// `tile` (LDS, [64][16], swizzled as above); lane i gets the logit of sample
+// i.  The four groups of 16 samples go through the network layer by layer, so
+// that independent MFMAs cover each other's latency: 8 + 8 + 4 MFMAs per 64
+// samples.
 __device__ __forceinline__ float mlpForward64(const MlpWave& m, const uint16_t* tile)
 {
   const int lane = threadIdx.x & 63, col = lane & 15, grp = lane >> 4;
   const uint2* rows = reinterpret_cast<const uint2*>(tile);
+  const int unit = grp ^ tileSwizzle(col);  // rows g*16 + col share bits 2 and 3 with col
   bf16x4 x[4];
 #pragma unroll
-  for (int g = 0; g < 4; ++g) x[g] = __builtin_bit_cast(bf16x4, rows[(g * 16 + col) * 4 + grp]);
+  for (int g = 0; g < 4; ++g) x[g] = __builtin_bit_cast(bf16x4, rows[(g * 16 + col) * 4 + unit]);
   uint2 h1[4][2], h2[4][2];
 #pragma unroll
   for (int g = 0; g < 4; ++g)
@@ -144,13 +173,19 @@ __device__ __forceinline__ float mlpForward64(const MlpWave& m, const uint16_t* 
   return mine;
 }
 
-// 16 BF16 features of one sample as two 16-byte stores (features 10..15 zero).
-__device__ __forceinline__ void packFeatures(const float f[kMlpIn], uint16_t* row)
+// 16 BF16 features of one sample as two 16-byte halves (features 10..15 zero).
+__device__ __forceinline__ void packFeatureRow(const float f[kMlpIn], uint4& lo, uint4& hi)
 {
   static_assert(kMlpIn == 10 && kMlpInPad == 16, "feature row layout");
+  lo = make_uint4(packBf16x2(f[0], f[1]), packBf16x2(f[2], f[3]), packBf16x2(f[4], f[5]), packBf16x2(f[6], f[7]));
+  hi = make_uint4(packBf16x2(f[8], f[9]), 0u, 0u, 0u);
+}
+
+// A training row in global memory (unswizzled, 32 bytes).
+__device__ __forceinline__ void packFeatures(const float f[kMlpIn], uint16_t* row)
+{
   uint4* r = reinterpret_cast<uint4*>(row);
-  r[0] = make_uint4(packBf16x2(f[0], f[1]), packBf16x2(f[2], f[3]), packBf16x2(f[4], f[5]), packBf16x2(f[6], f[7]));
-  r[1] = make_uint4(packBf16x2(f[8], f[9]), 0u, 0u, 0u);
+  packFeatureRow(f, r[0], r[1]);
 }
 
 // The cluster-i part of the features: sorted cluster i of layer l, whose
@@ -294,7 +329,9 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) kTr
         } else {
           for (int k = 0; k < kMlpIn; ++k) f[k] = 0.0f;
         }
-        packFeatures(f, &tiles[wave][lane * kMlpInPad]);
+        uint4 lo, hi;
+        packFeatureRow(f, lo, hi);
+        storeTileRow(tiles[wave], lane, lo, hi);
         waveSync();
         const float z = mlpForward64(m, tiles[wave]);
         const unsigned long long kb = __ballot(o < rn && z >= threshold);
@@ -417,9 +454,7 @@ __global__ void __launch_bounds__(256) kScoreRows(const uint16_t* w, const uint1
       lo = src[0];
       hi = src[1];
     }
-    uint4* dst = reinterpret_cast<uint4*>(&tiles[wave][lane * kMlpInPad]);
-    dst[0] = lo;
-    dst[1] = hi;
+    storeTileRow(tiles[wave], lane, lo, hi);
     waveSync();
     const float z = mlpForward64(m, tiles[wave]);
     if (o < n) out[o] = z;
