@@ -25,13 +25,13 @@ import itsca
 from golden_cases import fixture, small_cases
 lib = itsca.bind_hip(ctypes.CDLL(os.path.join(ROOT, "tests", "emu", "libitsca_emu.so")))
 cases = {n: p for n, p, _ in small_cases()}
-for name in ("small_s1", "sparse_s6"):
+for name in ("small_s1", "sparse_s6", "small_s5_nolayer3"):
     ev = itsca.load_events(cases[name])[0]
     t = itsca.GpuTracker(persistent_blocks=4, lib=lib)
     t.set_events([ev])
     t.run()
     assert itsca.dump_item(t, 0, ev.vertices[0]) == fixture(name, "stable", 0), name
-    if name == "small_s1":
+    if name in ("small_s1", "small_s5_nolayer3"):  # the second: a layer whose tracklet count never ran
         plain = Counter(itsca.road_tracks(t, 0))
         n = t.counts(0)
         trk = [t.download(0, itsca.TRACKLETS, l, n.tracklets[l], 4)[:, :2] for l in range(6)]
@@ -53,7 +53,7 @@ for name in ("small_s1", "sparse_s6"):
             keep = (z[k:k + len(trk[l])] >= thr) | (raw[trk[l][:, 0]] > itsca.TRACKLET_SLOTS)
             k += len(trk[l])
             got = t.download(0, itsca.TRACKLETS, l, t.counts(0).tracklets[l], 4)[:, :2]
-            assert np.array_equal(got, trk[l][keep]), l
+            assert np.array_equal(got, trk[l][keep]), (name, l)
     if name == "small_s1":
         # the MFMA forward pass (swizzled LDS tile, emulated MFMA lane layouts)
         # against float64, as in the GPU test: two full tiles and a partial one
