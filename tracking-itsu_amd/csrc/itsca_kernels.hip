@@ -1441,10 +1441,17 @@ __global__ void __launch_bounds__(B) kLevels(DevState s, int layer, int stage)
 // Roads: every root (layer-4 cell of level 5) emits its chains in the order of
 // the reference's depth-first traversal -- ascending predecessor at every
 // layer -- with Moore's majority vote of the MC labels (Tracker.cxx:573-652).
+// The roads of a chunk of roots are dealt one per thread (a root may have many
+// chains, most have one).  Road q of a root is its q-th chain in that order:
+// at every layer, the eligible predecessors (level one less) are taken in
+// ascending order, each standing for paths(p) chains, and q falls in one of
+// them.
 template <int B>
 __global__ void __launch_bounds__(B) kRoadsFill(DevState s)
 {
   __shared__ int scratch[B / 64 + 1];
+  __shared__ int exAll[B];
+  __shared__ uint8_t owner[kOwnerRound];
   constexpr int L4 = kCellLayers - 1;
   const int S = s.W;
   const int* chunkStart = s.chunkStart + kStRoads * (s.maxSegs + 1);
@@ -1452,81 +1459,78 @@ __global__ void __launch_bounds__(B) kRoadsFill(DevState s)
   for (int chunk = xcdBlockIndex(); chunk < nChunks; chunk += gridDim.x) {
     const int w = segmentOfChunk(chunkStart, S, chunk);
     const Seg c4 = s.celSeg[w * kCellLayers + L4];
-    const int in = (chunk - chunkStart[w]) * B + threadIdx.x;
+    const int first = (chunk - chunkStart[w]) * B;
+    const int in = first + threadIdx.x;
     int cnt = 0;
     if (in < c4.count && s.clevel[c4.base + in] == kCellsMinLevel) cnt = (int)min(s.cpaths[c4.base + in], 0x3fffffffu);
     int total;
     const int ex = blockExclusiveScan(cnt, scratch, total);
-    int out = s.chunkOff[(size_t)kStRoads * s.maxChunks + chunk] + ex;
-    if (cnt == 0) continue;
-    s.roadOff[c4.base + in] = out - s.roadSeg[w].base;
+    exAll[threadIdx.x] = ex;
+    const int out0 = s.chunkOff[(size_t)kStRoads * s.maxChunks + chunk];
+    if (cnt > 0) s.roadOff[c4.base + in] = out0 + ex - s.roadSeg[w].base;
     Seg cs[kCellLayers];
     for (int k = 0; k < kCellLayers; ++k) cs[k] = s.celSeg[w * kCellLayers + k];
     const int rw = s.recItem[w];
     const Seg cl[kLayers] = {s.clSeg[rw * kLayers + 0], s.clSeg[rw * kLayers + 1], s.clSeg[rw * kLayers + 2],
                              s.clSeg[rw * kLayers + 3], s.clSeg[rw * kLayers + 4], s.clSeg[rw * kLayers + 5],
                              s.clSeg[rw * kLayers + 6]};
-    // Iterative depth-first walk; pos[k] indexes the predecessor list of the
-    // cell chosen at layer k+1.
-    int cell[kCellLayers];
-    int pos[kCellLayers];
-    cell[L4] = in;
-    int layer = L4;
-    pos[L4 - 1] = -1;
-    const int end = out + cnt;
-    while (layer <= L4 && out < end) {
-      if (layer == 0) {
-        if (out < s.rCap) {
-          for (int k = 0; k < kCellLayers; ++k) s.rCells[(size_t)out * kCellLayers + k] = cell[k];
-          // Moore's voting over the 7 clusters of the chain.
-          const int base0 = cs[0].base + cell[0];
-          int value = s.cmc[cl[0].base + s.c1[base0]];
-          int count = 1;
-          bool fake = false;
-          const int second = s.cmc[cl[1].base + s.c2[base0]];
-          if (second == value) {
-            ++count;
-          } else {
-            value = second;
-            count = 1;
-            fake = true;
+    forEachSlotOutput<B>(cnt, ex, total, exAll, owner, [&](int c, int q0) {
+      const int out = out0 + exAll[c] + q0;
+      if (out >= s.rCap) return;
+      int cell[kCellLayers];
+      cell[L4] = first + c;
+      uint32_t q = (uint32_t)q0;
+      for (int layer = L4; layer > 0; --layer) {
+        const int n = cs[layer].base + cell[layer];
+        const int lvl = s.clevel[n];
+        const int off = s.edgSeg[w * (kCellLayers - 1) + layer - 1].base + s.edgeOff[n];
+        const int cnt2 = s.edgeCnt[n];
+        int pick = -1;
+        for (int a = 0; a < cnt2; ++a) {
+          const int p = s.pred[off + a];
+          const int pc = cs[layer - 1].base + p;
+          if (s.clevel[pc] != lvl - 1) continue;
+          const uint32_t paths = s.cpaths[pc];
+          if (q < paths) {
+            pick = p;
+            break;
           }
-          for (int k = 0; k < kCellLayers; ++k) {
-            const int current = s.cmc[cl[k + 2].base + s.c3[cs[k].base + cell[k]]];
-            if (current == value) {
-              ++count;
-            } else {
-              --count;
-              fake = true;
-            }
-            if (count == 0) {
-              value = current;
-              count = 1;
-            }
-          }
-          s.rLabel[out] = value;
-          s.rFake[out] = fake ? 1 : 0;
+          q -= paths;
         }
-        ++out;
-        layer = 1;
-        continue;
+        // paths(n) is the sum over the eligible predecessors, and q < paths(root),
+        // so some predecessor is always picked
+        cell[layer - 1] = pick < 0 ? 0 : pick;
       }
-      // advance to the next predecessor of cell[layer] whose level is one less
-      const int n = cs[layer].base + cell[layer];
-      const int lvl = s.clevel[n];
-      const int off = s.edgSeg[w * (kCellLayers - 1) + layer - 1].base + s.edgeOff[n];
-      const int cnt2 = s.edgeCnt[n];
-      int a = pos[layer - 1] + 1;
-      while (a < cnt2 && s.clevel[cs[layer - 1].base + s.pred[off + a]] != lvl - 1) ++a;
-      if (a >= cnt2) {
-        ++layer;  // exhausted: back up
-        continue;
+      for (int k = 0; k < kCellLayers; ++k) s.rCells[(size_t)out * kCellLayers + k] = cell[k];
+      // Moore's voting over the 7 clusters of the chain.
+      const int base0 = cs[0].base + cell[0];
+      int value = s.cmc[cl[0].base + s.c1[base0]];
+      int count = 1;
+      bool fake = false;
+      const int second = s.cmc[cl[1].base + s.c2[base0]];
+      if (second == value) {
+        ++count;
+      } else {
+        value = second;
+        count = 1;
+        fake = true;
       }
-      pos[layer - 1] = a;
-      cell[layer - 1] = s.pred[off + a];
-      --layer;
-      if (layer > 0) pos[layer - 1] = -1;
-    }
+      for (int k = 0; k < kCellLayers; ++k) {
+        const int current = s.cmc[cl[k + 2].base + s.c3[cs[k].base + cell[k]]];
+        if (current == value) {
+          ++count;
+        } else {
+          --count;
+          fake = true;
+        }
+        if (count == 0) {
+          value = current;
+          count = 1;
+        }
+      }
+      s.rLabel[out] = value;
+      s.rFake[out] = fake ? 1 : 0;
+    });
   }
 }
 
