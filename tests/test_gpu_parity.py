@@ -151,6 +151,39 @@ def test_runs_are_deterministic(tracker):
         assert np.array_equal(x, y)
 
 
+def _item_arrays(t, w):
+    n = t.counts(w)
+    out = [np.array(list(n.clusters) + list(n.tracklets) + list(n.cells) + list(n.neighbours) + [n.roads])]
+    out += [t.download(w, itsca.TRACKLETS, l, n.tracklets[l], 4) for l in range(itsca.TRACKLET_LAYERS)]
+    out += [t.download(w, itsca.CELLS, l, n.cells[l], 10) for l in range(itsca.CELL_LAYERS)]
+    for l in range(itsca.CELL_LAYERS - 1):
+        out.append(t.download(w, itsca.NEIGHBOUR_OFFSETS, l, n.cells[l + 1] + 1, 1))
+        out.append(t.download(w, itsca.NEIGHBOURS, l, n.neighbours[l], 1))
+    out.append(t.roads(w))
+    return out
+
+
+def test_large_batch_matches_single_event_runs(tracker):
+    """12 central events in one launch: the cell stage has about 21 000 chunks
+    of 256 tracklets, more than one tile (16 384) of the chunk scan.  Every
+    event's tracklets, cells, neighbours and roads equal those of the event run
+    on its own."""
+    say("gpu large batch vs single events")
+    evs = itsca.generate_events(2, 60, 12)
+    tracker.set_tuning(256, 0, 0)
+    tracker.set_events(evs)
+    tracker.run()
+    batch = [_item_arrays(tracker, w) for w in range(len(evs))]
+    # cell-stage chunks: tracklets of layers 0..4 in chunks of 256 per (event, layer)
+    assert sum(-(-int(x) // 256) for a in batch for x in a[0][7:12]) > 16384
+    for w, ev in enumerate(evs):
+        tracker.set_events([ev])
+        tracker.run()
+        single = _item_arrays(tracker, 0)
+        for k, (x, y) in enumerate(zip(batch[w], single)):
+            assert np.array_equal(x, y), (w, k)
+
+
 def test_capacity_growth_keeps_results(golden_dir):
     say("gpu capacity growth")
     ev = itsca.load_events(os.path.join(golden_dir, "small_s1.events.txt"))[0]

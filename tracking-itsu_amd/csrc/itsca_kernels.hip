@@ -396,13 +396,24 @@ __global__ void __launch_bounds__(kScanThreads) kScanChunks(DevState s, int stag
   // stores); each thread scans kScanPer consecutive sums of the tile (an LDS
   // row padded by one against bank conflicts), then one block scan of the
   // thread totals.
+  // The sums of the next tile are loaded into registers while the current
+  // tile is scanned.
+  constexpr int kTile = kScanThreads * kScanPer;
   int carry = 0;
   auto slot = [](int idx) { return (idx / kScanPer) * (kScanPer + 1) + idx % kScanPer; };
-  for (int c0 = 0; c0 < nChunks; c0 += kScanThreads * kScanPer) {
+  int next[kScanPer];
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    const int idx = k * kScanThreads + threadIdx.x;
+    next[k] = idx < nChunks ? sums[idx] : 0;
+  }
+  for (int c0 = 0; c0 < nChunks; c0 += kTile) {
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) tile[slot(k * kScanThreads + threadIdx.x)] = next[k];
 #pragma unroll
     for (int k = 0; k < kScanPer; ++k) {
-      const int idx = k * kScanThreads + threadIdx.x;
-      tile[slot(idx)] = c0 + idx < nChunks ? sums[c0 + idx] : 0;
+      const int idx = c0 + kTile + k * kScanThreads + threadIdx.x;
+      next[k] = idx < nChunks ? sums[idx] : 0;
     }
     __syncthreads();
     int* row = tile + threadIdx.x * (kScanPer + 1);
