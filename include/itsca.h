@@ -98,6 +98,19 @@ itsca_status itsca_run_async(itsca_ctx* ctx);
 itsca_status itsca_finish(itsca_ctx* ctx);
 
 int itsca_num_items(const itsca_ctx* ctx);
+
+/* Output capacities of the device buffers after the last run (elements, all
+ * work items and layers together); they grow on overflow and never shrink.
+ * Counterpart of the capacities the reference reserves per layer
+ * (PrimaryVertexContext.cxx:56-74), reported by its memory benchmark
+ * (Tracker.cxx:303-366). */
+typedef struct itsca_capacities {
+  long tracklets;
+  long cells;
+  long neighbours;
+  long roads;
+} itsca_capacities;
+itsca_status itsca_get_capacities(itsca_ctx* ctx, itsca_capacities* out);
 itsca_status itsca_get_counts(itsca_ctx* ctx, int item, itsca_counts* out);
 
 /* Copies one record kind of one work item into dst (cap_bytes long);

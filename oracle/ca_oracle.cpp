@@ -119,7 +119,7 @@ void initialize(const EventView& ev, bool stableSort, VertexResult& out)
 }
 
 // Tracker.cxx:44-117 and Tracklet.cxx:32-40
-void computeTracklets(VertexResult& ctx)
+void computeTracklets(VertexResult& ctx, bool exclusiveBound)
 {
   for (int l = 0; l < kTrackletLayers; ++l) {
     if (ctx.clusters[l].empty() || ctx.clusters[l + 1].empty()) return;  // early return, not continue
@@ -142,7 +142,8 @@ void computeTracklets(VertexResult& ctx)
         const int maxRow = table[maxBin];
         // Inclusive upper bound as in the reference (Tracker.cxx:88); the
         // "< nNext" guard stops the one read past the end of the layer.
-        for (int j = firstRow; j <= maxRow && j < nNext; ++j) {
+        const int lastRow = exclusiveBound ? maxRow - 1 : maxRow;
+        for (int j = firstRow; j <= lastRow && j < nNext; ++j) {
           const Cluster& n = next[j];
           const float deltaZ = std::abs(tanLambda * (n.r - c.r) + c.z - n.z);
           const float deltaPhi = std::abs(c.phi - n.phi);
@@ -385,13 +386,13 @@ void ints(std::string& s, const std::vector<int>& v)
 
 }  // namespace
 
-void runVertex(const EventView& ev, bool stableSort, VertexResult& out)
+void runVertex(const EventView& ev, bool stableSort, VertexResult& out, bool exclusiveBound)
 {
   out.vx = ev.vx;
   out.vy = ev.vy;
   out.vz = ev.vz;
   initialize(ev, stableSort, out);
-  computeTracklets(out);
+  computeTracklets(out, exclusiveBound);
   computeCells(out);
   findNeighbours(out);
   findTracks(out);

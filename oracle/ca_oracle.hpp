@@ -83,8 +83,13 @@ struct VertexResult {
 };
 
 // Runs the plain CA for one vertex.  stableSort selects std::stable_sort for
-// the per-layer bin ordering (the reference uses std::sort).
-void runVertex(const EventView& ev, bool stableSort, VertexResult& out);
+// the per-layer bin ordering (the reference uses std::sort).  exclusiveBound
+// is not the reference: it drops the extra candidate of the reference's
+// inclusive upper bound (Tracker.cxx:84-89), which leaves every tracklet the
+// window search can find exactly once, whatever the sort order; the tests use
+// it to show that the two sort orders differ only in the duplicates that
+// extra candidate adds.
+void runVertex(const EventView& ev, bool stableSort, VertexResult& out, bool exclusiveBound = false);
 
 // Canonical text dump; floats are written as their IEEE bit patterns.
 std::string dump(const VertexResult& r);

@@ -20,13 +20,14 @@ void* itsca_oracle_new(void) { return new VertexResult(); }
 void itsca_oracle_free(void* h) { delete static_cast<VertexResult*>(h); }
 
 // Runs one vertex.  Arrays hold every layer's clusters, layer after layer, in
-// file order; layer_offsets has 8 entries.
+// file order; layer_offsets has 8 entries.  mode bit 0: stable sort; bit 1:
+// exclusive candidate bound (not the reference; see runVertex).
 int itsca_oracle_run(void* h, const float* x, const float* y, const float* z, const int* id, const int* mc,
-                     const int* layer_offsets, const float* vertex, int stable_sort)
+                     const int* layer_offsets, const float* vertex, int mode)
 {
   if (!h || !layer_offsets || !vertex) return -1;
   itsca_oracle::EventView ev{x, y, z, id, mc, layer_offsets, vertex[0], vertex[1], vertex[2]};
-  itsca_oracle::runVertex(ev, stable_sort != 0, *static_cast<VertexResult*>(h));
+  itsca_oracle::runVertex(ev, (mode & 1) != 0, *static_cast<VertexResult*>(h), (mode & 2) != 0);
   return 0;
 }
 

@@ -53,7 +53,9 @@ def _ip(a):
 class OracleVertex:
     """Runs the CPU restatement for one vertex of one event."""
 
-    def __init__(self, event, vertex_index: int = 0, stable: bool = True):
+    def __init__(self, event, vertex_index: int = 0, stable: bool = True, exclusive_bound: bool = False):
+        """exclusive_bound (not the reference): no extra candidate past each
+        window row, so every tracklet the search finds appears once."""
         L = lib()
         self.h = L.itsca_oracle_new()
         self._keep = [np.ascontiguousarray(a, dtype=np.float32) for a in (event.x, event.y, event.z)]
@@ -62,7 +64,8 @@ class OracleVertex:
         self._offs = np.ascontiguousarray(event.layer_offsets, dtype=np.int32)
         v = np.ascontiguousarray(event.vertices[vertex_index], dtype=np.float32)
         st = L.itsca_oracle_run(self.h, _fp(self._keep[0]), _fp(self._keep[1]), _fp(self._keep[2]),
-                                _ip(self._ids), _ip(self._mc), _ip(self._offs), _fp(v), 1 if stable else 0)
+                                _ip(self._ids), _ip(self._mc), _ip(self._offs), _fp(v),
+                                (1 if stable else 0) | (2 if exclusive_bound else 0))
         if st != 0:
             raise RuntimeError("oracle run failed")
 

@@ -47,3 +47,22 @@ def keyed(text):
         res[("cells", l)] = Counter(tuple(c) + tuple(r[5:10]) for c, r in zip(cells[l], d[("cells", l)]))
     res["roads"] = Counter(tuple(cells[k][int(r[k])] for k in range(5)) + (r[6], r[7]) for r in d["roads"])
     return res
+
+
+def distinct_digest(text):
+    """SHA-256 of the distinct cluster-id-keyed tracklets, cells and roads
+    (multiplicities dropped), stage by stage in sorted order."""
+    import hashlib
+
+    k = keyed(text)
+    h = hashlib.sha256()
+    for stage in sorted(k, key=str):
+        h.update(repr(stage).encode())
+        for item in sorted(k[stage]):
+            h.update(repr(item).encode())
+    return h.hexdigest()
+
+
+def duplicates(text):
+    """Per stage, the number of records beyond the first of each keyed record."""
+    return {str(stage): sum(c.values()) - len(c) for stage, c in keyed(text).items()}

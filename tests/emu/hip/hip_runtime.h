@@ -63,6 +63,12 @@ void launch(dim3 grid, dim3 block, const std::function<void()>& body);
 inline void __syncthreads() { emu::syncthreads(); }
 inline void __threadfence_block() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 inline int __shfl_up(int v, unsigned d, int width = 64) { return emu::shflUp(v, d, width); }
+inline long long __shfl_up(long long v, unsigned d, int width = 64)
+{
+  const int lo = emu::shflUp((int)(unsigned)(unsigned long long)v, d, width);
+  const int hi = emu::shflUp((int)((unsigned long long)v >> 32), d, width);
+  return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
 inline float __shfl_xor(float v, int mask, int width = 64) { return emu::shflXor(v, mask, width); }
 inline int __shfl_xor(int v, int mask, int width = 64) { return emu::shflXorInt(v, mask, width); }
 inline int __shfl(int v, int src, int width = 64) { return emu::shflInt(v, src, width); }
@@ -107,6 +113,8 @@ inline int atomicMax(int* p, int v)
 inline int min(int a, int b) { return a < b ? a : b; }
 inline int max(int a, int b) { return a > b ? a : b; }
 inline unsigned min(unsigned a, unsigned b) { return a < b ? a : b; }
+inline long long min(long long a, long long b) { return a < b ? a : b; }
+inline long long max(long long a, long long b) { return a > b ? a : b; }
 
 struct float2 {
   float x, y;

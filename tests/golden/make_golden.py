@@ -28,6 +28,9 @@ ROOT = os.path.abspath(os.path.join(HERE, "..", ".."))
 sys.path.insert(0, os.path.join(ROOT, "tracking-itsu_amd"))
 import itsca  # noqa: E402
 
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+import dumps  # noqa: E402
+
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 HARNESS_STABLE = os.path.join(ROOT, "oracle", "_ref", "ref_harness_stable")
 
@@ -89,7 +92,35 @@ def small_cases():
     # two collisions in one event, two primary vertices
     a, b = itsca.generate_events(1, 7)[0], itsca.generate_events(1, 8)[0]
     cases["pileup2_s7"] = [merge(a, b)]
+    # clusters outside the layers in z: negative z bins (negative bins in phi
+    # row 0, aliased bins in row 1) and |z| = 1e10 (an index outside the int
+    # range, INT_MIN on x86)
+    cases["outside_s1"] = [outside_event()]
     return cases
+
+
+def outside_event():
+    base = itsca.generate_events(1, 1)[0]
+    vx, vy, vz = (float(a) for a in base.vertices[0])
+    radii = [2.34, 3.15, 3.93, 19.6, 24.55, 34.39, 39.34]
+    extra = {0: [(-20.0, 0.1), (1.0e10, 0.1), (-1.0e10, 0.12), (-30.0, 0.5), (-20.0, 0.1)],
+             1: [(-25.0, 0.05), (-19.0, 0.2), (3.0e9, 0.02), (-21.0, 0.6)]}
+    cols = {k: [] for k in ("x", "y", "z", "mc")}
+    offs = [0]
+    for layer in range(itsca.LAYERS):
+        sl = base.layer(layer)
+        for k, arr in (("x", base.x), ("y", base.y), ("z", base.z), ("mc", base.mc)):
+            cols[k] += list(arr[sl])
+        for z, phi in extra.get(layer, []):
+            cols["x"].append(vx + radii[layer] * np.cos(phi))
+            cols["y"].append(vy + radii[layer] * np.sin(phi))
+            cols["z"].append(z)
+            cols["mc"].append(-1)
+        offs.append(len(cols["x"]))
+    n = offs[-1]
+    return itsca.Event(np.array(cols["x"], np.float32), np.array(cols["y"], np.float32),
+                       np.array(cols["z"], np.float32), np.arange(n, dtype=np.int32), np.array(cols["mc"], np.int32),
+                       np.array(offs, np.int32), base.vertices.copy())
 
 
 def run_harness(exe, events_path, prefix):
@@ -130,11 +161,19 @@ def main():
                 counts.setdefault(parts[0], []).append(int(parts[2]))
             elif parts and parts[0] == "roads":
                 counts["roads"] = int(parts[1])
+        stable_dump = open(os.path.join(tmp, "stable_e0_v0.txt")).read()
         info = {
             "preset": 2, "seed": 2,
             "events_sha256": digest(ev_path),
             "ref_sha256": digest(os.path.join(tmp, "ref_e0_v0.txt")),
             "stable_sha256": digest(os.path.join(tmp, "stable_e0_v0.txt")),
+            # cluster-id-keyed sets, multiplicities dropped: the same for both
+            # sort orders (they differ only in duplicates of the inclusive
+            # candidate bound); duplicates counted per stage
+            "ref_distinct_sha256": dumps.distinct_digest(ref_dump),
+            "stable_distinct_sha256": dumps.distinct_digest(stable_dump),
+            "ref_duplicates": dumps.duplicates(ref_dump),
+            "stable_duplicates": dumps.duplicates(stable_dump),
             "counts": counts,
             "harness_log": log.strip(),
         }

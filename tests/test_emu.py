@@ -125,3 +125,91 @@ def test_kernels_under_cpu_emulation_with_asan():
     out = subprocess.run([sys.executable, "-c", "ROOT = %r\n" % ROOT + SCRIPT], env=env, capture_output=True,
                          text=True, timeout=1200)
     assert out.returncode == 0 and "ALL OK" in out.stdout, out.stdout[-2000:] + out.stderr[-6000:]
+
+
+EDGE_SCRIPT = r"""
+import ctypes, os, sys
+import numpy as np
+sys.path[:0] = [os.path.join(ROOT, "tracking-itsu_amd"), os.path.join(ROOT, "tests"), os.path.join(ROOT, "oracle")]
+import itsca, pyoracle
+from golden_cases import fixture, small_cases
+lib = itsca.bind_hip(ctypes.CDLL(os.path.join(ROOT, "tests", "emu", "libitsca_emu.so")))
+# Clusters outside the detector in z: below the layer (a negative z bin, so a
+# negative bin in phi row 0 and an aliased bin in row 1), and at |z| = 1e10
+# (an index outside the int range).  The reference's sort puts negative bins
+# first and its index table counts them inside bin 0; every array must still
+# equal the oracle's.
+base = itsca.generate_events(1, 1, 1)[0]
+vx, vy, vz = (float(a) for a in base.vertices[0])
+radii = [2.34, 3.15, 3.93, 19.6, 24.55, 34.39, 39.34]
+extra = {0: [(-20.0, 0.1), (1.0e10, 0.1), (-1.0e10, 0.12), (-30.0, 0.5), (-20.0, 0.1)],
+         1: [(-25.0, 0.05), (-19.0, 0.2), (3.0e9, 0.02), (-21.0, 0.6)]}
+cols = {k: [] for k in ("x", "y", "z", "mc")}
+offs = [0]
+for layer in range(itsca.LAYERS):
+    sl = base.layer(layer)
+    for k, a in (("x", base.x), ("y", base.y), ("z", base.z), ("mc", base.mc)):
+        cols[k] += list(a[sl])
+    for z, phi in extra.get(layer, []):
+        cols["x"].append(vx + radii[layer] * np.cos(phi))
+        cols["y"].append(vy + radii[layer] * np.sin(phi))
+        cols["z"].append(z)
+        cols["mc"].append(-1)
+    offs.append(len(cols["x"]))
+n = offs[-1]
+ev = itsca.Event(np.array(cols["x"], np.float32), np.array(cols["y"], np.float32), np.array(cols["z"], np.float32),
+                 np.arange(n, dtype=np.int32), np.array(cols["mc"], np.int32), np.array(offs, np.int32),
+                 base.vertices.copy())
+for zsub in (1, 4):
+    t = itsca.GpuTracker(persistent_blocks=4, lib=lib)
+    t.set_tuning(256, 4, zsub)
+    t.set_events([ev])
+    t.run()
+    cl = t.download(0, itsca.CLUSTERS, 0, n, 8)
+    assert int(cl[:, 3].min()) < 0  # negative bins are kept as the reference computes them
+    assert itsca.dump_item(t, 0, ev.vertices[0]) == pyoracle.OracleVertex(ev, 0, stable=True).dump(), zsub
+    t.close()
+print("clusters outside the layers vs oracle ok", flush=True)
+# non-finite coordinates are refused with an argument error
+bad = itsca.Event(ev.x.copy(), ev.y.copy(), ev.z.copy(), ev.cluster_id, ev.mc, ev.layer_offsets, ev.vertices)
+bad.z[3] = np.nan
+t = itsca.GpuTracker(persistent_blocks=4, lib=lib)
+try:
+    t.set_events([bad])
+    raise SystemExit("a NaN coordinate was accepted")
+except itsca.ItscaError as e:
+    assert "non-finite" in str(e), e
+t.close()
+print("non-finite input refused ok", flush=True)
+# roads per root cell bounded at 3 in this build: events within the bound give
+# the fixture, an event with a root of 4 roads fails with a capacity error
+cap = itsca.bind_hip(ctypes.CDLL(os.path.join(ROOT, "tests", "emu", "libitsca_emu_cap3.so")))
+cases = {nm: p for nm, p, _ in small_cases()}
+for name, ok in (("small_s1", True), ("small_s4", True), ("small_s2", False)):
+    e = itsca.load_events(cases[name])[0]
+    t = itsca.GpuTracker(persistent_blocks=4, lib=cap)
+    t.set_events([e])
+    try:
+        t.run()
+        assert ok, name
+        assert itsca.dump_item(t, 0, e.vertices[0]) == fixture(name, "stable", 0), name
+    except itsca.ItscaError as err:
+        assert not ok and "more than 3 roads" in str(err), (name, err)
+    t.close()
+print("root road bound ok", flush=True)
+print("ALL OK")
+"""
+
+
+def test_edge_inputs_under_cpu_emulation_with_asan():
+    rt = _asan_runtime()
+    if not os.path.exists(CLANG) or rt is None:
+        pytest.skip("ROCm clang++ with its AddressSanitizer runtime is needed")
+    build = subprocess.run(["make", "-C", EMU, "all"], capture_output=True, text=True, timeout=1200)
+    assert build.returncode == 0, build.stdout[-2000:] + build.stderr[-4000:]
+    env = dict(os.environ)
+    env["LD_PRELOAD"] = rt + (":" + env["LD_PRELOAD"] if env.get("LD_PRELOAD") else "")
+    env["ASAN_OPTIONS"] = "detect_leaks=0"
+    out = subprocess.run([sys.executable, "-c", "ROOT = %r\n" % ROOT + EDGE_SCRIPT], env=env, capture_output=True,
+                         text=True, timeout=1200)
+    assert out.returncode == 0 and "ALL OK" in out.stdout, out.stdout[-2000:] + out.stderr[-6000:]

@@ -70,6 +70,48 @@ def test_central_event_matches_reference_digest(tracker):
     assert hashlib.sha256(text.encode()).hexdigest() == info["stable_sha256"]
 
 
+@pytest.mark.parametrize("name,path,nv", CASES, ids=[c[0] for c in CASES])
+def test_matches_unmodified_reference_as_keyed_sets(tracker, name, path, nv):
+    """Against the reference as-is (std::sort, the committed ref fixtures):
+    the cluster-id-keyed tracklets, cells and roads are the same sets, and
+    every multiplicity difference is a duplicate of the inclusive candidate
+    bound (Tracker.cxx:84-89): the oracle without that extra candidate gives
+    the same set with every tracklet and cell once."""
+    from dumps import keyed
+
+    say(f"gpu case {name} vs unmodified reference (keyed)")
+    ev = itsca.load_events(path)[0]
+    tracker.set_events([ev])
+    tracker.run()
+    for v in range(nv):
+        gpu = keyed(itsca.dump_item(tracker, v, ev.vertices[v]))
+        ref = keyed(fixture(name, "ref", v))
+        once = keyed(pyoracle.OracleVertex(ev, v, stable=True, exclusive_bound=True).dump())
+        for stage in ref:
+            assert set(gpu[stage]) == set(ref[stage]) == set(once[stage]), (name, v, stage)
+            if stage != "roads":
+                assert max(once[stage].values(), default=1) == 1
+            for k in set(gpu[stage]) | set(ref[stage]):
+                if gpu[stage][k] != ref[stage][k]:  # a duplicated record in at least one of the two
+                    assert max(gpu[stage][k], ref[stage][k]) >= 2, (name, v, stage, k)
+
+
+def test_central_event_matches_unmodified_reference_keyed(tracker):
+    from dumps import distinct_digest, duplicates
+
+    say("gpu central event vs unmodified reference (keyed)")
+    info = central()
+    with tempfile.TemporaryDirectory() as tmp:
+        p = os.path.join(tmp, "c.txt")
+        itsca.write_events(info["preset"], info["seed"], 1, p)
+        ev = itsca.load_events(p)[0]
+    tracker.set_events([ev])
+    tracker.run()
+    text = itsca.dump_item(tracker, 0, ev.vertices[0])
+    assert distinct_digest(text) == info["ref_distinct_sha256"]
+    assert duplicates(text) == info["ref_duplicates"]
+
+
 def test_mixed_multiplicity_batch_matches_oracle(tracker):
     say("gpu mixed batch vs oracle")
     evs = itsca.generate_events(3, 3000, 6)

@@ -137,3 +137,43 @@ def test_levels_and_roads_are_longest_chains():
     roads = [tuple(int(v) for v in r[:5]) for r in d["roads"]]
     assert roads == sorted(chains, key=lambda c: c[::-1])
     assert len(roads) > 50
+
+
+# --- the two sort orders of the reference -----------------------------------
+
+
+@pytest.mark.parametrize("name,path,nv", CASES, ids=[c[0] for c in CASES])
+def test_sort_orders_differ_only_in_inclusive_bound_duplicates(name, path, nv):
+    """The reference as-is (std::sort) and the stable-order build give the same
+    cluster-id-keyed tracklets, cells and roads once multiplicities are
+    dropped.  Without the extra candidate of the inclusive bound
+    (Tracker.cxx:84-89) both orders give that same set with every tracklet and
+    cell once, so every multiplicity difference is a duplicate of that extra
+    candidate (PrimaryVertexContext.cxx:52-54 decides which cluster it is)."""
+    from dumps import keyed
+
+    ev = itsca.load_events(path)[0]
+    for v in range(nv):
+        ref, stable = keyed(fixture(name, "ref", v)), keyed(fixture(name, "stable", v))
+        for sort_stable in (False, True):
+            once = keyed(pyoracle.OracleVertex(ev, v, stable=sort_stable, exclusive_bound=True).dump())
+            for stage in ref:
+                assert set(ref[stage]) == set(stable[stage]) == set(once[stage]), (name, v, stage)
+                if stage != "roads":
+                    assert max(once[stage].values(), default=1) == 1, (name, v, stage)
+                # the duplicates only add records
+                assert all(ref[stage][k] >= 1 and stable[stage][k] >= 1 for k in once[stage])
+
+
+def test_central_event_sort_orders_have_the_same_keyed_sets():
+    from dumps import distinct_digest, duplicates
+
+    info = central()
+    with tempfile.TemporaryDirectory() as tmp:
+        p = os.path.join(tmp, "c.txt")
+        itsca.write_events(info["preset"], info["seed"], 1, p)
+        ev = itsca.load_events(p)[0]
+    assert info["ref_distinct_sha256"] == info["stable_distinct_sha256"]
+    once = pyoracle.OracleVertex(ev, 0, stable=False, exclusive_bound=True).dump()
+    assert distinct_digest(once) == info["ref_distinct_sha256"]
+    assert all(n == 0 for n in duplicates(once).values())

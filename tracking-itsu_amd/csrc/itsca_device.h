@@ -52,12 +52,23 @@ enum Stage { kStTracklets = 0, kStCells = 1, kStEdges = 2, kStLevels = 3, kStRoa
 // kStLevels + (l-1) for cell layers 1..4 (levels of layer 0 are all 1).
 
 enum StatusSlot {
-  kStatusOverflowTracklets = 0,
+  kStatusOverflowTracklets = 0,  // capacity exceeded: the number of outputs needed
   kStatusOverflowCells = 1,
   kStatusOverflowEdges = 2,
   kStatusOverflowRoads = 3,
+  kStatusRootRoads = 4,    // a root cell has more than kMaxRootRoads roads (not retried)
+  kStatusUndecodable = 5,  // a road chain could not be decoded from the path counts (not retried)
   kNumStatus = 8
 };
+
+// Roads of one root cell (chains of level-descending cells ending at it).  A
+// root with more sets kStatusRootRoads and the run fails with a capacity
+// error instead of dropping roads.  The bound keeps every per-chunk road sum
+// (256 roots) inside 32 bits.
+#ifndef ITSCA_MAX_ROOT_ROADS
+#define ITSCA_MAX_ROOT_ROADS (1 << 20)
+#endif
+constexpr uint32_t kMaxRootRoads = ITSCA_MAX_ROOT_ROADS;
 
 struct DevState {
   int W;  // work items

@@ -56,6 +56,35 @@ __device__ __forceinline__ int blockExclusiveScan(int v, int* scratch, int& tota
   return ex;
 }
 
+// The same in 64 bits (single-workgroup scans of chunk sums whose total may
+// pass 2^31); `scratch` holds blockDim/64 + 1 values.
+__device__ __forceinline__ long long blockExclusiveScan64(long long v, long long* scratch, long long& total)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nWaves = blockDim.x >> 6;
+  long long inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const long long t = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += t;
+  }
+  if (lane == 63) scratch[wave] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long s = 0;
+    for (int w = 0; w < nWaves; ++w) {
+      const long long t = scratch[w];
+      scratch[w] = s;
+      s += t;
+    }
+    scratch[nWaves] = s;
+  }
+  __syncthreads();
+  const long long ex = inc - v + scratch[wave];
+  total = scratch[nWaves];
+  __syncthreads();
+  return ex;
+}
+
 __device__ __forceinline__ int blockSum(int v, int* scratch)
 {
   int total;
@@ -111,6 +140,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
   __shared__ int tab[kTableSize];
   __shared__ int scratch[1024 / 64 + 1];
   __shared__ int geo[4];  // min r, max r (float bits; r >= 0 orders like ints), min and max z bin index
+  __shared__ int negCount;  // clusters with a negative bin (z below the layer in phi row 0)
   const int w = blockIdx.x / kLayers, l = blockIdx.x % kLayers;
   if (blockIdx.x == 0 && threadIdx.x < kNumStatus) s.status[threadIdx.x] = 0;
   const Seg seg = s.clSeg[w * kLayers + l];
@@ -128,6 +158,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
     geo[1] = 0;
     geo[2] = kZBins;
     geo[3] = -1;
+    negCount = 0;
   }
   __syncthreads();
   int rMinBits = __float_as_int(3.0e38f), rMaxBits = 0, zMin = kZBins, zMax = -1;
@@ -142,7 +173,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
     s.sR[base + i] = r;
     s.sBin[base + i] = b;
     s.trkCnt[base + i] = 0;
-    atomicAdd(&hist[b], 1);
+    atomicAdd(b >= 0 ? &hist[b] : &negCount, 1);
     rMinBits = min(rMinBits, __float_as_int(r));  // a NaN r has bits above every finite r
     rMaxBits = max(rMaxBits, __float_as_int(r));
     zMin = min(zMin, zb);
@@ -164,16 +195,21 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
   if (threadIdx.x == 0)
     s.segGeo[w * kLayers + l] = make_float4(__int_as_float(geo[0]), __int_as_float(geo[1]),
                                             geo[2] >= 0 && geo[3] < kZBins ? 1.0f : 0.0f, 0.0f);
-  // Exclusive scan of the 401 bin counts: table[b] = #clusters with bin < b.
+  // Exclusive scan of the 401 bin counts: table[b] = #clusters with bin < b
+  // for b >= 1, and table[0] = 0 (PrimaryVertexContext.cxx:90-119).  Clusters
+  // with a negative bin sort first, at positions [0, nNeg), so they fall in
+  // bin 0's range [table[0], table[1]) as in the reference; tab[b] is where the
+  // clusters of bin b itself start.
   int* table = s.table + (size_t)(w * kLayers + l) * kTableSize;
+  const int nNeg = negCount;
   {
     const int v = threadIdx.x < kTableSize ? hist[threadIdx.x] : 0;
     int total;
-    const int ex = blockExclusiveScan(v, scratch, total);
+    const int ex = blockExclusiveScan(v, scratch, total) + nNeg;
     if (threadIdx.x < kTableSize) {
       hist[threadIdx.x] = ex;  // becomes the insertion cursor
       tab[threadIdx.x] = ex;
-      table[threadIdx.x] = ex;
+      table[threadIdx.x] = threadIdx.x == 0 ? 0 : ex;
     }
   }
   __syncthreads();
@@ -187,10 +223,23 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
   constexpr int kPrepWaves = kPrepThreads / 64;
   static_assert(kPrepWaves * kTableSize <= kPrepLds, "per-wavefront bin counts live in the bucket array");
   int* waveHist = lBucket;  // [kPrepWaves][kTableSize]
+  // A cluster with a negative bin takes its rank among those (ordered by bin,
+  // then index, as a stable sort orders them) from a scan of the layer's bins;
+  // such clusters lie outside the detector and are normally absent.
+  auto negativeRank = [&](int i, int b) {
+    int rank = 0;
+    for (int j = 0; j < n; ++j) {
+      const int bj = s.sBin[base + j];
+      rank += (bj < b || (bj == b && j < i)) ? 1 : 0;
+    }
+    return rank;
+  };
   auto tilePositions = [&](int t0) -> int {
     const int i = t0 + threadIdx.x;
-    const bool valid = i < n;
-    const int b = valid ? s.sBin[base + i] : 511;  // bins are 0..400 (9 bits)
+    const int bRaw = i < n ? s.sBin[base + i] : 511;
+    const bool neg = bRaw < 0;
+    const bool valid = i < n && !neg;
+    const int b = valid ? bRaw : 511;  // bins of the match are 0..400 (9 bits)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     unsigned long long peers = ~0ull;
 #pragma unroll
@@ -213,8 +262,9 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
       hist[bb] = run;
     }
     __syncthreads();
-    const int pos = valid ? waveHist[wave * kTableSize + b] + __popcll(below) : -1;
+    int pos = valid ? waveHist[wave * kTableSize + b] + __popcll(below) : -1;
     __syncthreads();  // the counts are cleared by the next tile
+    if (nNeg > 0 && neg) pos = negativeRank(i, bRaw);
     return pos;
   };
   // When the layer fits in LDS the positions stay in registers and the
@@ -309,7 +359,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
   int* subCnt = lBucket;  // kBins * S counters, then cursors
   for (int k = threadIdx.x; k < kBins * S; k += blockDim.x) subCnt[k] = 0;
   __syncthreads();
-  auto smallBin = [&](int b) { return b < kBins && tab[b + 1] - tab[b] <= kSubMaxBin; };
+  auto smallBin = [&](int b) { return b >= 0 && b < kBins && tab[b + 1] - tab[b] <= kSubMaxBin; };
   for (int p = threadIdx.x; p < n; p += blockDim.x) {
     const int b = s.cbin[base + p];
     if (smallBin(b)) atomicAdd(&subCnt[b * S + zSubIndex(l, s.cz[base + p], b % kZBins, S)], 1);
@@ -336,7 +386,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
   __syncthreads();
   for (int p = threadIdx.x; p < n; p += blockDim.x) {
     const int b = s.cbin[base + p];
-    const int lo = b < kBins ? tab[b] : tab[kBins];
+    const int lo = b < 0 ? 0 : (b < kBins ? tab[b] : tab[kBins]);
     int q = p;
     if (smallBin(b)) q = lo + atomicAdd(&subCnt[b * S + zSubIndex(l, s.cz[base + p], b % kZBins, S)], 1);
     const float4 rec = s.crzp[base + p];
@@ -386,7 +436,7 @@ constexpr int kScanPer = 16;
 __global__ void __launch_bounds__(kScanThreads) kScanChunks(DevState s, int stage, int S, Seg* outSeg, int cap,
                                                             int statusSlot)
 {
-  __shared__ int scratch[kScanThreads / 64 + 1];
+  __shared__ long long scratch[kScanThreads / 64 + 1];
   __shared__ int tile[kScanThreads * (kScanPer + 1)];
   const int* chunkStart = s.chunkStart + stage * (s.maxSegs + 1);
   const int* sums = s.chunkSum + (size_t)stage * s.maxChunks;
@@ -395,11 +445,13 @@ __global__ void __launch_bounds__(kScanThreads) kScanChunks(DevState s, int stag
   // Tiles of 1024 * kScanPer sums go through LDS (coalesced loads and
   // stores); each thread scans kScanPer consecutive sums of the tile (an LDS
   // row padded by one against bank conflicts), then one block scan of the
-  // thread totals.
+  // thread totals.  The running sums are 64-bit (a stage's total may pass
+  // 2^31, e.g. roads); offsets are stored clamped to the capacity, which is
+  // where every fill stops writing.
   // The sums of the next tile are loaded into registers while the current
   // tile is scanned.
   constexpr int kTile = kScanThreads * kScanPer;
-  int carry = 0;
+  long long carry = 0;
   auto slot = [](int idx) { return (idx / kScanPer) * (kScanPer + 1) + idx % kScanPer; };
   int next[kScanPer];
 #pragma unroll
@@ -417,17 +469,17 @@ __global__ void __launch_bounds__(kScanThreads) kScanChunks(DevState s, int stag
     }
     __syncthreads();
     int* row = tile + threadIdx.x * (kScanPer + 1);
-    int mine = 0;
+    long long pre[kScanPer];
+    long long mine = 0;
 #pragma unroll
     for (int k = 0; k < kScanPer; ++k) {
-      const int t = row[k];
-      row[k] = mine;
-      mine += t;
+      pre[k] = mine;
+      mine += row[k];
     }
-    int total;
-    const int ex = carry + blockExclusiveScan(mine, scratch, total);  // its barriers order the row writes
+    long long total;
+    const long long ex = carry + blockExclusiveScan64(mine, scratch, total);  // its barriers order the row reads
 #pragma unroll
-    for (int k = 0; k < kScanPer; ++k) row[k] += ex;
+    for (int k = 0; k < kScanPer; ++k) row[k] = (int)min(ex + pre[k], (long long)cap);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kScanPer; ++k) {
@@ -437,10 +489,10 @@ __global__ void __launch_bounds__(kScanThreads) kScanChunks(DevState s, int stag
     carry += total;
     __syncthreads();  // the tile is refilled
   }
-  const int grand = carry;
   // On overflow every segment of the stage is emptied, so no later stage reads
   // past the capacity; the host grows the buffers and runs again.
-  const bool overflow = grand > cap;
+  const bool overflow = carry > (long long)cap;
+  const int grand = (int)min(carry, (long long)cap);
   for (int i = threadIdx.x; i < S; i += blockDim.x) {
     const int a = chunkStart[i], b = chunkStart[i + 1];
     const int base = a < nChunks ? offs[a] : grand;
@@ -448,7 +500,7 @@ __global__ void __launch_bounds__(kScanThreads) kScanChunks(DevState s, int stag
     outSeg[i].base = overflow ? 0 : base;
     outSeg[i].count = overflow ? 0 : end - base;
   }
-  if (threadIdx.x == 0 && overflow) s.status[statusSlot] = grand;
+  if (threadIdx.x == 0 && overflow) s.status[statusSlot] = (int)min(carry, 0x7fffffffLL);
 }
 
 // ---------------------------------------------------------------------------
@@ -751,7 +803,7 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) kTr
     // Union of the windows.  Rows are taken relative to the row before the
     // chunk's first cluster: the chunk's clusters sit in consecutive rows, and
     // a window starts at most one row before its cluster's row.
-    const int rowRef = (s.cbin[rcur.base + first] / kPhiBins + kPhiBins - 1) % kPhiBins;
+    const int rowRef = (max(s.cbin[rcur.base + first], 0) / kPhiBins + kPhiBins - 1) % kPhiBins;
     if (threadIdx.x == 0) {
       uni[0] = kPhiBins;
       uni[1] = -kPhiBins;
@@ -1440,7 +1492,12 @@ __global__ void __launch_bounds__(B) kLevels(DevState s, int layer, int stage)
       }
       s.clevel[n] = level;
       s.cpaths[n] = paths;
-      if (layer == kCellLayers - 1 && level == kCellsMinLevel) roots = (int)min(paths, 0x3fffffffu);
+      if (layer == kCellLayers - 1 && level == kCellsMinLevel) {
+        // A saturated count stays saturated through every later sum, so a
+        // root reached by one exceeds the bound too.
+        if (paths > kMaxRootRoads) s.status[kStatusRootRoads] = 1;
+        roots = (int)min(paths, kMaxRootRoads);
+      }
     }
     if (layer == kCellLayers - 1) {
       const int total = blockSum(roots, scratch);
@@ -1473,7 +1530,7 @@ __global__ void __launch_bounds__(B) kRoadsFill(DevState s)
     const int first = (chunk - chunkStart[w]) * B;
     const int in = first + threadIdx.x;
     int cnt = 0;
-    if (in < c4.count && s.clevel[c4.base + in] == kCellsMinLevel) cnt = (int)min(s.cpaths[c4.base + in], 0x3fffffffu);
+    if (in < c4.count && s.clevel[c4.base + in] == kCellsMinLevel) cnt = (int)min(s.cpaths[c4.base + in], kMaxRootRoads);
     int total;
     const int ex = blockExclusiveScan(cnt, scratch, total);
     exAll[threadIdx.x] = ex;
@@ -1486,8 +1543,9 @@ __global__ void __launch_bounds__(B) kRoadsFill(DevState s)
                              s.clSeg[rw * kLayers + 3], s.clSeg[rw * kLayers + 4], s.clSeg[rw * kLayers + 5],
                              s.clSeg[rw * kLayers + 6]};
     forEachSlotOutput<B>(cnt, ex, total, exAll, owner, [&](int c, int q0) {
-      const int out = out0 + exAll[c] + q0;
-      if (out >= s.rCap) return;
+      const long long outL = (long long)out0 + exAll[c] + q0;  // out0 <= rCap, the rest < 2^28
+      if (outL >= s.rCap) return;
+      const int out = (int)outL;
       int cell[kCellLayers];
       cell[L4] = first + c;
       uint32_t q = (uint32_t)q0;
@@ -1509,8 +1567,12 @@ __global__ void __launch_bounds__(B) kRoadsFill(DevState s)
           q -= paths;
         }
         // paths(n) is the sum over the eligible predecessors, and q < paths(root),
-        // so some predecessor is always picked
-        cell[layer - 1] = pick < 0 ? 0 : pick;
+        // so some predecessor is always picked; anything else fails the run.
+        if (pick < 0) {
+          s.status[kStatusUndecodable] = 1;
+          pick = 0;
+        }
+        cell[layer - 1] = pick;
       }
       for (int k = 0; k < kCellLayers; ++k) s.rCells[(size_t)out * kCellLayers + k] = cell[k];
       // Moore's voting over the 7 clusters of the chain.

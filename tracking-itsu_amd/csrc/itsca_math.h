@@ -235,10 +235,20 @@ ITSCA_HD float clusterPhi(float dx, float dy)
 {
   return normalizedPhi(fdlibmAtan2f(-dy, -dx) + kPi);
 }
-// IndexTableUtils.h:47-51 (float -> int truncation toward zero)
+// The reference's float -> int conversions (IndexTableUtils.h:49-55) as its
+// x86-64 build performs them (cvttss2si): truncation toward zero, and
+// INT32_MIN for NaN and for values outside the int range.  The gfx950
+// conversion saturates instead (and gives 0 for NaN), so out-of-range inputs
+// are mapped explicitly; in range both are the same truncation.
+ITSCA_HD int truncToIntX86(float v)
+{
+  return (v >= -2147483648.0f && v < 2147483648.0f) ? (int)v : (int)0x80000000;
+}
+// IndexTableUtils.h:47-51 (float -> int truncation toward zero).  A z below
+// the layer gives a negative index (a negative bin in phi row 0).
 ITSCA_HD int zBinIndex(int layer, float z)
 {
-  return (int)((z + layerHalfZ(layer)) * inverseZBinSize(layer));
+  return truncToIntX86((z + layerHalfZ(layer)) * inverseZBinSize(layer));
 }
 // z sub-bin of the tracklet search index (no counterpart in the reference):
 // z bin zb of layer l split into S parts.  For a fixed zb it is monotone in z
@@ -251,11 +261,14 @@ ITSCA_HD int zSubIndex(int layer, float z, int zb, int S)
   return sub < 0 ? 0 : (sub >= S ? S - 1 : sub);
 }
 // IndexTableUtils.h:53-56
-ITSCA_HD int phiBinIndex(float phi) { return (int)(phi * kInversePhiBinSize); }
-// IndexTableUtils.h:58-62 (row stride PhiBins, clamped to the 401st slot)
+ITSCA_HD int phiBinIndex(float phi) { return truncToIntX86(phi * kInversePhiBinSize); }
+// IndexTableUtils.h:58-62 (row stride PhiBins, clamped to the 401st slot; a
+// negative z index in phi row 0 gives a negative bin, which the reference's
+// sort puts before bin 0 and its index table counts inside bin 0's range).
+// The sum wraps like the x86 integer add it stands for.
 ITSCA_HD int binIndex(int zIndex, int phiIndex)
 {
-  const int b = phiIndex * kPhiBins + zIndex;
+  const int b = (int)((uint32_t)phiIndex * (uint32_t)kPhiBins + (uint32_t)zIndex);
   return b < kBins ? b : kBins;
 }
 

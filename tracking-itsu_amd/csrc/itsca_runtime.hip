@@ -62,6 +62,9 @@ struct HipError {
 
 }  // namespace
 
+// Largest output capacity of a stage (element indices are int32).
+constexpr double kMaxCapacity = 2.0e9;
+
 struct itsca_ctx {
   int device = 0;
   int cus = 0;
@@ -148,6 +151,20 @@ struct itsca_ctx {
     return code;
   }
 
+  // A road stage that cannot be completed by growing buffers; sets err.
+  bool roadFailure()
+  {
+    if (status[kStatusRootRoads] != 0) {
+      err = "a root cell has more than " + std::to_string(kMaxRootRoads) + " roads";
+      return true;
+    }
+    if (status[kStatusUndecodable] != 0) {
+      err = "a road could not be decoded from the path counts";
+      return true;
+    }
+    return false;
+  }
+
   void estimateCapacities()
   {
     double t = 0, c = 0;
@@ -161,7 +178,7 @@ struct itsca_ctx {
       c += std::max(4096.0, 0.25 * tw);
     }
     const double sc = capacityScale > 0 ? capacityScale : 1.0;
-    auto clampCap = [](double v) { return (int)std::min(v, 2.0e9); };
+    auto clampCap = [](double v) { return (int)std::min(v, kMaxCapacity); };
     tCap = clampCap(t * sc);
     cCap = clampCap(c * sc);
     eCap = cCap;
@@ -486,6 +503,16 @@ itsca_status itsca_add_event(itsca_ctx* ctx, const float* x, const float* y, con
   const int base = (int)ctx->hx.size();
   if ((long)base + total > 0x7fffffffL) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_add_event: batch too large");
   const int o = layer_offsets[0];
+  // Non-finite coordinates have no defined bin: the reference's conversions
+  // of NaN are undefined behaviour in C++, so they are refused here.
+  for (int i = o; i < o + total; ++i)
+    if (!std::isfinite(x[i]) || !std::isfinite(y[i]) || !std::isfinite(z[i]))
+      return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_add_event: cluster " + std::to_string(i - o) +
+                                               " has a non-finite coordinate");
+  for (int v = 0; v < 3 * num_vertices; ++v)
+    if (!std::isfinite(vertices[v]))
+      return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_add_event: vertex " + std::to_string(v / 3) +
+                                               " has a non-finite coordinate");
   ctx->hx.insert(ctx->hx.end(), x + o, x + o + total);
   ctx->hy.insert(ctx->hy.end(), y + o, y + o + total);
   ctx->hz.insert(ctx->hz.end(), z + o, z + o + total);
@@ -585,6 +612,7 @@ itsca_status itsca_finish(itsca_ctx* ctx)
   return guarded(ctx, "itsca_finish", [&]() -> itsca_status {
     HIP_TRY(hipSetDevice(ctx->device));
     ctx->readBack();
+    if (ctx->roadFailure()) return ctx->fail(ITSCA_ERR_CAPACITY, "itsca_finish: " + ctx->err);
     for (int k = 0; k < kNumStatus; ++k)
       if (ctx->status[k] != 0)
         return ctx->fail(ITSCA_ERR_CAPACITY, "itsca_finish: output capacity exceeded (stage " + std::to_string(k) +
@@ -602,10 +630,15 @@ itsca_status itsca_run(itsca_ctx* ctx)
     for (int attempt = 0; attempt < 6; ++attempt) {
       ctx->enqueue();
       ctx->readBack();
+      if (ctx->roadFailure()) return ctx->fail(ITSCA_ERR_CAPACITY, "itsca_run: " + ctx->err);
       bool grow = false;
+      bool tooLarge = false;
       auto need = [&](int slot, int& cap) {
         if (ctx->status[slot] != 0) {
-          cap = (int)std::min(2.0e9, 1.25 * (double)ctx->status[slot] + 4096);
+          // the scan reports at most INT32_MAX outputs: beyond the largest
+          // capacity the run cannot succeed
+          if ((double)ctx->status[slot] > kMaxCapacity) tooLarge = true;
+          cap = (int)std::min(kMaxCapacity, 1.25 * (double)ctx->status[slot] + 4096);
           grow = true;
         }
       };
@@ -613,6 +646,9 @@ itsca_status itsca_run(itsca_ctx* ctx)
       need(kStatusOverflowCells, ctx->cCap);
       need(kStatusOverflowEdges, ctx->eCap);
       need(kStatusOverflowRoads, ctx->rCap);
+      if (tooLarge)
+        return ctx->fail(ITSCA_ERR_CAPACITY, "itsca_run: a stage needs more than " +
+                                                 std::to_string((long long)kMaxCapacity) + " outputs");
       if (!grow) {
         ctx->ran = true;
         return ITSCA_OK;
@@ -624,6 +660,16 @@ itsca_status itsca_run(itsca_ctx* ctx)
 }
 
 int itsca_num_items(const itsca_ctx* ctx) { return ctx ? ctx->W : 0; }
+
+itsca_status itsca_get_capacities(itsca_ctx* ctx, itsca_capacities* out)
+{
+  if (!ctx || !out) return ITSCA_ERR_ARGUMENT;
+  out->tracklets = ctx->tCap;
+  out->cells = ctx->cCap;
+  out->neighbours = ctx->eCap;
+  out->roads = ctx->rCap;
+  return ITSCA_OK;
+}
 
 itsca_status itsca_get_counts(itsca_ctx* ctx, int item, itsca_counts* out)
 {

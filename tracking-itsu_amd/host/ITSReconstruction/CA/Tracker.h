@@ -42,6 +42,16 @@ class Tracker<true> {
   // TimeBenchmark output (Tracker.cxx:368-402): prep, tracklets, cells,
   // neighbours, levels+roads, total.
   std::vector<std::vector<Road>> clustersToTracksTimeBenchmark(const Event& event, std::ofstream& out);
+  // The same device stage times printed to std::cout in the reference's
+  // DEBUG format (Tracker.cxx:267-301, 654-684).  The MC labels are computed
+  // inside the roads kernel, so their own line reports 0 ms.
+  std::vector<std::vector<Road>> clustersToTracksVerbose(const Event& event);
+  // Per vertex, the reference's GPU-mode MemoryOccupancy lines
+  // (Tracker.cxx:303-366 with TRACKINGITSU_GPU_MODE): clusters per layer;
+  // cell capacity per layer (the cell buffer is one compact array shared by
+  // the layers, so each layer may use the whole of it); cells per layer;
+  // roads.
+  std::vector<std::vector<Road>> clustersToTracksMemoryBenchmark(const Event& event, std::ofstream& out);
 
   // Host snapshot of the device state of vertex `vertex` of the last run
   // (the batch's work item order: event by event, vertex by vertex).
@@ -58,6 +68,8 @@ class Tracker<true> {
 
  private:
   void runBatch(const std::vector<const Event*>& events);
+  // Runs vertex v of the event as its own work item with per-stage timing.
+  void runVertexProfiled(const Event& event, int v, float ms[5]);
   std::vector<Road> downloadRoads(int item) const;
   itsca_ctx* mCtx = nullptr;
 };

@@ -2,6 +2,7 @@
 // of libitsca_hip.so (include/itsca.h).
 #include <cstdint>
 #include <cstring>
+#include <iomanip>
 #include <iostream>
 #include <sstream>
 #include <stdexcept>
@@ -154,20 +155,31 @@ std::vector<std::vector<std::vector<Road>>> Tracker<true>::clustersToTracks(cons
   return out;
 }
 
+void Tracker<true>::runVertexProfiled(const Event& event, int v, float ms[5])
+{
+  Event single(event.getEventId());
+  single.addPrimaryVertex(event.getPrimaryVertex(v).x, event.getPrimaryVertex(v).y, event.getPrimaryVertex(v).z);
+  for (int l = 0; l < Constants::ITS::LayersNumber; ++l)
+    for (const Cluster& c : event.getLayer(l).getClusters())
+      single.pushClusterToLayer(l, c.clusterId, c.xCoordinate, c.yCoordinate, c.zCoordinate, c.alphaAngle,
+                                c.monteCarloId);
+  check(mCtx, itsca_set_profiling(mCtx, 1), "itsca_set_profiling");
+  try {
+    runBatch({&single});
+    check(mCtx, itsca_stage_times(mCtx, ms), "itsca_stage_times");
+  } catch (...) {
+    (void)itsca_set_profiling(mCtx, 0);
+    throw;
+  }
+  check(mCtx, itsca_set_profiling(mCtx, 0), "itsca_set_profiling");
+}
+
 std::vector<std::vector<Road>> Tracker<true>::clustersToTracksTimeBenchmark(const Event& event, std::ofstream& out)
 {
-  check(mCtx, itsca_set_profiling(mCtx, 1), "itsca_set_profiling");
   std::vector<std::vector<Road>> roads;
   for (int v = 0; v < event.getPrimaryVerticesNum(); ++v) {
-    Event single(event.getEventId());
-    single.addPrimaryVertex(event.getPrimaryVertex(v).x, event.getPrimaryVertex(v).y, event.getPrimaryVertex(v).z);
-    for (int l = 0; l < Constants::ITS::LayersNumber; ++l)
-      for (const Cluster& c : event.getLayer(l).getClusters())
-        single.pushClusterToLayer(l, c.clusterId, c.xCoordinate, c.yCoordinate, c.zCoordinate, c.alphaAngle,
-                                  c.monteCarloId);
-    runBatch({&single});
     float ms[5];
-    check(mCtx, itsca_stage_times(mCtx, ms), "itsca_stage_times");
+    runVertexProfiled(event, v, ms);
     float total = 0;
     for (float t : ms) {
       out << t << "\t";
@@ -176,7 +188,50 @@ std::vector<std::vector<Road>> Tracker<true>::clustersToTracksTimeBenchmark(cons
     out << total << std::endl;
     roads.push_back(downloadRoads(0));
   }
-  check(mCtx, itsca_set_profiling(mCtx, 0), "itsca_set_profiling");
+  return roads;
+}
+
+std::vector<std::vector<Road>> Tracker<true>::clustersToTracksVerbose(const Event& event)
+{
+  static const char* const kTasks[] = {"Tracklets Finding", "Cells Finding", "Neighbours Finding", "Tracks Finding",
+                                       "Computing Montecarlo Labels"};
+  std::vector<std::vector<Road>> roads;
+  for (int v = 0; v < event.getPrimaryVerticesNum(); ++v) {
+    float ms[5];
+    runVertexProfiled(event, v, ms);
+    std::cout << std::setw(2) << " - Context initialized in: " << ms[0] << "ms" << std::endl;
+    const float task[5] = {ms[1], ms[2], ms[3], ms[4], 0.0f};  // labels: inside the roads kernel
+    float total = ms[0];
+    for (int k = 0; k < 5; ++k) {
+      std::cout << std::setw(2) << " - " << kTasks[k] << " completed in: " << task[k] << "ms" << std::endl;
+      total += task[k];
+    }
+    std::cout << std::setw(2) << " - Vertex " << v + 1 << " completed in: " << total << "ms" << std::endl;
+    roads.push_back(downloadRoads(0));
+  }
+  return roads;
+}
+
+std::vector<std::vector<Road>> Tracker<true>::clustersToTracksMemoryBenchmark(const Event& event,
+                                                                              std::ofstream& out)
+{
+  std::vector<std::vector<Road>> roads;
+  for (int v = 0; v < event.getPrimaryVerticesNum(); ++v) {
+    float ms[5];
+    runVertexProfiled(event, v, ms);
+    itsca_counts n;
+    check(mCtx, itsca_get_counts(mCtx, 0, &n), "itsca_get_counts");
+    itsca_capacities cap;
+    check(mCtx, itsca_get_capacities(mCtx, &cap), "itsca_get_capacities");
+    for (int l = 0; l < Constants::ITS::LayersNumber; ++l) out << n.clusters[l] << "\t";
+    out << std::endl;
+    for (int l = 0; l < Constants::ITS::CellsPerRoad; ++l) out << cap.cells << "\t";
+    out << std::endl;
+    for (int l = 0; l < Constants::ITS::CellsPerRoad; ++l) out << n.cells[l] << "\t";
+    out << std::endl;
+    out << n.roads << std::endl;
+    roads.push_back(downloadRoads(0));
+  }
   return roads;
 }
 
