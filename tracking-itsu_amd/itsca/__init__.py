@@ -43,6 +43,11 @@ class Tuning(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_int), ("persistent_blocks", ctypes.c_int), ("z_subbins", ctypes.c_int)]
 
 
+class Capacities(ctypes.Structure):
+    _fields_ = [("tracklets", ctypes.c_long), ("cells", ctypes.c_long), ("neighbours", ctypes.c_long),
+                ("roads", ctypes.c_long)]
+
+
 class Counts(ctypes.Structure):
     _fields_ = [
         ("clusters", ctypes.c_int * 7),
@@ -56,6 +61,7 @@ class Counts(ctypes.Structure):
 HIP_SYMBOLS = [
     "itsca_create", "itsca_destroy", "itsca_last_error", "itsca_clear_events", "itsca_add_event", "itsca_upload",
     "itsca_run", "itsca_run_async", "itsca_finish", "itsca_num_items", "itsca_get_counts", "itsca_download",
+    "itsca_get_capacities",
     "itsca_set_profiling", "itsca_stage_times", "itsca_kernel_times", "itsca_kernel_name", "itsca_set_graph",
     "itsca_set_tuning", "itsca_get_tuning", "itsca_autotune",
     "itsca_mlp_init", "itsca_mlp_set_params", "itsca_mlp_get_params", "itsca_set_filter", "itsca_train_clear",
@@ -102,6 +108,7 @@ def bind_hip(lib: ctypes.CDLL) -> ctypes.CDLL:
         for f in ("itsca_upload", "itsca_run", "itsca_run_async", "itsca_finish", "itsca_num_items"):
             getattr(lib, f).argtypes = [P]
         lib.itsca_get_counts.argtypes = [P, I, ctypes.POINTER(Counts)]
+        lib.itsca_get_capacities.argtypes = [P, ctypes.POINTER(Capacities)]
         lib.itsca_download.argtypes = [P, I, I, I, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         lib.itsca_set_profiling.argtypes = [P, I]
         lib.itsca_set_graph.argtypes = [P, I]
@@ -419,6 +426,11 @@ class GpuTracker:
 
     def num_items(self) -> int:
         return self._lib.itsca_num_items(self.ctx)
+
+    def capacities(self) -> Capacities:
+        c = Capacities()
+        self._check(self._lib.itsca_get_capacities(self.ctx, ctypes.byref(c)), "itsca_get_capacities")
+        return c
 
     def counts(self, item: int) -> Counts:
         c = Counts()
