@@ -148,6 +148,38 @@ itsca_status itsca_autotune(itsca_ctx* ctx, int repeats, itsca_tuning* chosen, f
 itsca_status itsca_set_graph(itsca_ctx* ctx, int enable);
 
 /* ---------------------------------------------------------------------------
+ * One event without device memory allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture elsewhere in the program, tight serving loops, harnesses that
+ * forbid hipMalloc).  Runs the same pipeline as itsca_run (with the context's
+ * tuning and learned-filter setting) on one event with one primary vertex
+ * whose clusters the caller already holds in device memory, and writes the
+ * roads to the caller's device buffer.  Every intermediate array lives in a
+ * static device array of ITSCA_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends; the call allocates nothing and waits for its results.
+ *   - up to ITSCA_NOALLOC_MAX_CLUSTERS clusters (else ITSCA_ERR_ARGUMENT); an
+ *     intermediate stage that outgrows the scratch gives ITSCA_ERR_CAPACITY
+ *     (there is no growing);
+ *   - x, y, z, cluster_id, mc_id: device arrays; layer l occupies
+ *     [layer_offsets[l], layer_offsets[l+1]) of them; vertex: host values;
+ *   - roads: device buffer of roads_cap rows of 8 int in the ITSCA_ROADS
+ *     format; *num_roads gets the number of roads, and ITSCA_ERR_CAPACITY is
+ *     returned (with the first roads_cap rows written) when it exceeds roads_cap.
+ * Every call on one GPU shares that GPU's scratch: calls that overlap on one
+ * GPU (two contexts, two threads) race and must be serialised by the caller.
+ * --------------------------------------------------------------------------- */
+#define ITSCA_NOALLOC_SCRATCH_BYTES (128u << 20)
+#define ITSCA_NOALLOC_MAX_CLUSTERS 131072
+typedef struct itsca_device_event {
+  const float *x, *y, *z;
+  const int32_t *cluster_id, *mc_id;
+  int32_t layer_offsets[8];
+  float vertex[3];
+} itsca_device_event;
+itsca_status itsca_run_noalloc(itsca_ctx* ctx, const itsca_device_event* ev, int32_t* roads, int roads_cap,
+                               int* num_roads);
+
+/* ---------------------------------------------------------------------------
  * Learned tracklet filter (no counterpart in the reference; an optional stage
  * between the tracklet search and cell building).  A 10 -> 32 -> 32 -> 1 ReLU
  * network with BF16 weights and activations and FP32 accumulation scores every

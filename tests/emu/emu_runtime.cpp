@@ -277,13 +277,18 @@ void launch(dim3 grid, dim3 block, const std::function<void()>& body)
 }
 }  // namespace emu
 
+static long gAllocCalls = 0;  // hipMalloc and hipFree calls, for tests of entry points that must not allocate
+extern "C" long emu_alloc_calls() { return gAllocCalls; }
+
 hipError_t hipMalloc(void** p, size_t bytes)
 {
+  ++gAllocCalls;
   *p = std::malloc(bytes ? bytes : 1);
   return *p ? hipSuccess : hipErrorInvalidValue;
 }
 hipError_t hipFree(void* p)
 {
+  ++gAllocCalls;
   std::free(p);
   return hipSuccess;
 }

@@ -23,6 +23,7 @@
 #define __launch_bounds__(...)
 #define amdgpu_waves_per_eu(...)
 #define __shared__ static
+#define __align__(n) __attribute__((aligned(n)))
 
 struct dim3 {
   unsigned x = 1, y = 1, z = 1;
@@ -154,6 +155,13 @@ hipError_t hipMalloc(T** p, size_t bytes)
   return hipMalloc(reinterpret_cast<void**>(p), bytes);
 }
 hipError_t hipFree(void* p);
+// Device symbols are host objects here: the "device address" is the object's own.
+#define HIP_SYMBOL(sym) (&(sym))
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym)
+{
+  *p = const_cast<void*>(sym);
+  return hipSuccess;
+}
 hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKind, hipStream_t);
 hipError_t hipMemcpy(void* dst, const void* src, size_t bytes, hipMemcpyKind);
 hipError_t hipStreamSynchronize(hipStream_t);

@@ -197,6 +197,24 @@ for name, ok in (("small_s1", True), ("small_s4", True), ("small_s2", False)):
         assert not ok and "more than 3 roads" in str(err), (name, err)
     t.close()
 print("root road bound ok", flush=True)
+# itsca_run_noalloc: the same roads as itsca_run, no hipMalloc or hipFree
+# during the call, nothing written past the roads; a smaller event second
+so = ctypes.CDLL(os.path.join(ROOT, "tests", "emu", "libitsca_emu.so"))
+so.emu_alloc_calls.restype = ctypes.c_long
+t = itsca.GpuTracker(persistent_blocks=4, lib=lib)
+for name in ("sparse_s6", "small_s3"):
+    e = itsca.load_events(cases[name])[0]
+    t.set_events([e])
+    t.run()
+    want = t.roads(0)
+    arrs = [np.ascontiguousarray(a) for a in (e.x, e.y, e.z, e.cluster_id.astype(np.int32), e.mc.astype(np.int32))]
+    rows = np.full((len(want) + 50, 8), -7, np.int32)
+    before = so.emu_alloc_calls()
+    n = t.run_noalloc([a.ctypes.data for a in arrs], e.layer_offsets, e.vertices[0], rows.ctypes.data, len(rows))
+    assert so.emu_alloc_calls() == before, name
+    assert n == len(want) and np.array_equal(rows[:n], want) and (rows[n:] == -7).all(), name
+t.close()
+print("no-alloc entry point ok", flush=True)
 print("ALL OK")
 """
 

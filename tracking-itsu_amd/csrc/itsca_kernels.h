@@ -42,4 +42,7 @@ struct LaunchConfig {
 
 void launchPipeline(const DevState& s, const LaunchConfig& cfg, hipStream_t stream, KernelTimer* timer);
 
+// Roads of work item `item` as rows of 8 int (ITSCA_ROADS format), at most cap rows.
+void launchRoadsOut(const DevState& s, int item, int* rows, int cap, hipStream_t stream);
+
 }  // namespace itsca

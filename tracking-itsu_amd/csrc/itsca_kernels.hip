@@ -1607,7 +1607,33 @@ __global__ void __launch_bounds__(B) kRoadsFill(DevState s)
   }
 }
 
+// Road rows of one work item in the download format (cell of layers 0..4,
+// size, label, fake: 8 int per road) into a caller's device buffer of `cap`
+// rows; roads past cap are not written.
+__global__ void __launch_bounds__(256) kRoadsOut(DevState s, int item, int* rows, int cap)
+{
+  const Seg g = s.roadSeg[item];
+  const int n = min(g.count, cap);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int r = g.base + i;
+    int size = 0;
+    for (int k = 0; k < kCellLayers; ++k) {
+      const int c = s.rCells[(size_t)r * kCellLayers + k];
+      rows[(size_t)i * 8 + k] = c;
+      size += c >= 0 ? 1 : 0;
+    }
+    rows[(size_t)i * 8 + 5] = size;
+    rows[(size_t)i * 8 + 6] = s.rLabel[r];
+    rows[(size_t)i * 8 + 7] = s.rFake[r];
+  }
+}
+
 }  // namespace
+
+void launchRoadsOut(const DevState& s, int item, int* rows, int cap, hipStream_t stream)
+{
+  hipLaunchKernelGGL(kRoadsOut, dim3(256), dim3(256), 0, stream, s, item, rows, cap);
+}
 
 // ---------------------------------------------------------------------------
 // Launch sequence of one plain-CA run over the whole batch.

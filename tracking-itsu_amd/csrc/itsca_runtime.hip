@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -65,6 +66,39 @@ struct HipError {
 // Largest output capacity of a stage (element indices are int32).
 constexpr double kMaxCapacity = 2.0e9;
 
+// ---------------------------------------------------------------------------
+// Device scratch of itsca_run_noalloc: every intermediate array of one run
+// (sorted clusters, tables, tracklets, cells, edges, roads, chunk plans) is
+// carved out of this array, so the entry point allocates nothing.  Each GPU
+// the library runs on holds its own copy until the process ends.  In host
+// code `scratch` names a host-side stand-in, never device memory: its device
+// address is taken once per device with hipGetSymbolAddress (scratchAddress)
+// and only that address is used.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[ITSCA_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex gScratchMutex;
+void* gScratchAddr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, which must be current.
+void* scratchAddress(int device)
+{
+  std::lock_guard<std::mutex> lock(gScratchMutex);
+  if (device < 0 || device >= kMaxDevices) throw HipError{hipErrorInvalidDevice, "scratchAddress: device ordinal"};
+  if (!gScratchAddr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) throw HipError{e, "hipGetSymbolAddress(scratch)"};
+    gScratchAddr[device] = p;
+  }
+  return gScratchAddr[device];
+}
+
+}  // namespace
+
 struct itsca_ctx {
   int device = 0;
   int cus = 0;
@@ -108,6 +142,16 @@ struct itsca_ctx {
   // results of the last run
   std::vector<Seg> trkSeg, celSeg, edgSeg, roadSeg;
   int status[kNumStatus] = {};
+
+  // itsca_run_noalloc: the device address of `scratch` on this device, and
+  // fixed host staging for its small inputs and read-backs
+  unsigned char* scratchPtr = nullptr;
+  int naEvOff[8] = {};
+  int naZero[2] = {};
+  float naVertex[3] = {};
+  Seg naClSeg[kLayers] = {};
+  Seg naRoadSeg = {};
+  int naStatus[kNumStatus] = {};
 
   // device buffers
   DBuf<float> rx, ry, rz, itemVertex;
@@ -454,6 +498,7 @@ itsca_status itsca_create(itsca_ctx** out, const itsca_config* cfg)
     ctx->cus = prop.multiProcessorCount;
     if (ctx->persistentBlocks <= 0) ctx->persistentBlocks = 8 * ctx->cus;
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    ctx->scratchPtr = static_cast<unsigned char*>(scratchAddress(ctx->device));
     return ITSCA_OK;
   });
   // On failure the context is still returned so that the caller can read
@@ -669,6 +714,182 @@ itsca_status itsca_get_capacities(itsca_ctx* ctx, itsca_capacities* out)
   out->neighbours = ctx->eCap;
   out->roads = ctx->rCap;
   return ITSCA_OK;
+}
+
+namespace {
+
+// Lays out one work item's intermediate arrays in the scratch array of
+// `bytes` bytes at device address `base`, for n sorted clusters: the
+// per-cluster arrays first, then the rest split between tracklets, cells,
+// edges and roads in the proportions 10 : 1 : 1 : 1 (a central event has
+// about 9 tracklets, 1 edge and 0.2 roads per cell).  Returns false if the
+// clusters alone do not leave room for them.
+bool carveNoAlloc(unsigned char* base, size_t bytes, int n, DevState& s, int*& planScratch)
+{
+  size_t off = 0;
+  bool fits = true;
+  auto take = [&](size_t count, size_t elem) -> void* {
+    off = (off + 255) & ~(size_t)255;
+    void* p = base + off;
+    off += count * elem;
+    if (off > bytes) fits = false;
+    return p;
+  };
+  const size_t N = (size_t)std::max(n, 1);
+  s.itemEvent = static_cast<int*>(take(1, 4));
+  s.recItem = static_cast<int*>(take(1, 4));
+  s.itemVertex = static_cast<float*>(take(3, 4));
+  s.evLayerOff = static_cast<int*>(take(8, 4));
+  s.clSeg = static_cast<Seg*>(take(kLayers, sizeof(Seg)));
+  s.trkSeg = static_cast<Seg*>(take(kTrackletLayers, sizeof(Seg)));
+  s.celSeg = static_cast<Seg*>(take(kCellLayers, sizeof(Seg)));
+  s.edgSeg = static_cast<Seg*>(take(kCellLayers - 1, sizeof(Seg)));
+  s.roadSeg = static_cast<Seg*>(take(1, sizeof(Seg)));
+  s.status = static_cast<int*>(take(kNumStatus, 4));
+  planScratch = static_cast<int*>(take(kLayers, 4));
+  s.table = static_cast<int*>(take((size_t)kLayers * kTableSize, 4));
+  s.subTab = static_cast<uint4*>(take((size_t)kLayers * kBins, sizeof(uint4)));
+  s.segGeo = static_cast<float4*>(take(kLayers, sizeof(float4)));
+  s.cxyrz = static_cast<float4*>(take(N, sizeof(float4)));
+  s.crzp = static_cast<float4*>(take(N, sizeof(float4)));
+  s.crzpS = static_cast<float4*>(take(N, sizeof(float4)));
+  s.cz = static_cast<float*>(take(N, 4));
+  s.cphi = static_cast<float*>(take(N, 4));
+  s.cr = static_cast<float*>(take(N, 4));
+  s.sPhi = static_cast<float*>(take(N, 4));
+  s.sR = static_cast<float*>(take(N, 4));
+  s.cid = static_cast<int*>(take(N, 4));
+  s.cmc = static_cast<int*>(take(N, 4));
+  s.cbin = static_cast<int*>(take(N, 4));
+  s.sBin = static_cast<int*>(take(N, 4));
+  s.sInv = static_cast<int*>(take(N, 4));
+  s.trkCnt = static_cast<int*>(take(N, 4));
+  s.trkOff = static_cast<int*>(take(N, 4));
+  s.trkKept = static_cast<int*>(take(N, 4));
+  s.trkCand = static_cast<int*>(take(N * kTrkSlots, 4));
+  s.chunkStart = static_cast<int*>(take((size_t)kNumStages * (kLayers + 1), 4));
+  if (!fits) return false;
+  // bytes per unit k of capacity: 10 tracklets, 1 cell, 1 edge, 1 road, and
+  // two chunk-plan ints per stage for every kMinChunk tracklets
+  const size_t perTracklet = 8 + 8 + 4 + 4 + 4 * kCellSlots;
+  const size_t perCell = 6 * 4 + 16 + 4 + 3 * 4 + 4 + 4 * kEdgeSlots;
+  const size_t perEdge = 4 * 4, perRoad = 4 * kCellLayers + 4 + 4;
+  const size_t perUnit = 10 * perTracklet + perCell + perEdge + perRoad + 2 * 4 * kNumStages * 10 / kMinChunk + 64;
+  const size_t left = bytes - off;
+  const size_t k = std::min<size_t>(left / perUnit, (size_t)(kMaxCapacity / 10));
+  if (k < 64) return false;
+  s.tCap = (int)(10 * k);
+  s.cCap = s.eCap = s.rCap = (int)k;
+  s.maxSegs = kLayers;
+  s.maxChunks = (int)((std::max<size_t>(N, 10 * k) + kMinChunk - 1) / kMinChunk) + 8 + 1;
+  s.chunkSum = static_cast<int*>(take((size_t)kNumStages * s.maxChunks, 4));
+  s.chunkOff = static_cast<int*>(take((size_t)kNumStages * s.maxChunks, 4));
+  s.tFS = static_cast<int2*>(take(s.tCap, sizeof(int2)));
+  s.tTP = static_cast<float2*>(take(s.tCap, sizeof(float2)));
+  s.cellCnt = static_cast<int*>(take(s.tCap, 4));
+  s.cellOff = static_cast<int*>(take(s.tCap, 4));
+  s.cellCand = static_cast<int*>(take((size_t)s.tCap * kCellSlots, 4));
+  s.c1 = static_cast<int*>(take(s.cCap, 4));
+  s.c2 = static_cast<int*>(take(s.cCap, 4));
+  s.c3 = static_cast<int*>(take(s.cCap, 4));
+  s.ct1 = static_cast<int*>(take(s.cCap, 4));
+  s.ct2 = static_cast<int*>(take(s.cCap, 4));
+  s.clevel = static_cast<int*>(take(s.cCap, 4));
+  s.cnc = static_cast<float4*>(take(s.cCap, sizeof(float4)));
+  s.cpaths = static_cast<uint32_t*>(take(s.cCap, 4));
+  s.edgeCnt = static_cast<int*>(take(s.cCap, 4));
+  s.edgeOff = static_cast<int*>(take(s.cCap, 4));
+  s.roadOff = static_cast<int*>(take(s.cCap, 4));
+  s.succCnt = static_cast<int*>(take(s.cCap, 4));
+  s.succCand = static_cast<int*>(take((size_t)s.cCap * kEdgeSlots, 4));
+  s.pred = static_cast<int*>(take(s.eCap, 4));
+  s.eFrom = static_cast<int*>(take(s.eCap, 4));
+  s.eTo = static_cast<int*>(take(s.eCap, 4));
+  s.eTmp = static_cast<int*>(take(s.eCap, 4));
+  s.rCells = static_cast<int*>(take((size_t)s.rCap * kCellLayers, 4));
+  s.rLabel = static_cast<int*>(take(s.rCap, 4));
+  s.rFake = static_cast<int*>(take(s.rCap, 4));
+  return fits;
+}
+
+}  // namespace
+
+itsca_status itsca_run_noalloc(itsca_ctx* ctx, const itsca_device_event* ev, int32_t* roads, int roads_cap,
+                               int* num_roads)
+{
+  if (!ctx) return ITSCA_ERR_ARGUMENT;
+  if (!ev || !num_roads || roads_cap < 0 || (roads_cap > 0 && !roads))
+    return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_run_noalloc: missing event, road buffer or count");
+  *num_roads = 0;
+  const int32_t* offs = ev->layer_offsets;
+  int maxLayer = 0;
+  for (int l = 0; l < kLayers; ++l) {
+    if (offs[l + 1] < offs[l] || offs[l] < 0)
+      return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_run_noalloc: layer offsets must be non-negative and non-decreasing");
+    maxLayer = std::max(maxLayer, offs[l + 1] - offs[l]);
+  }
+  const long n = (long)offs[kLayers] - offs[0];
+  if (n > ITSCA_NOALLOC_MAX_CLUSTERS)
+    return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_run_noalloc: " + std::to_string(n) + " clusters, more than the " +
+                                             std::to_string(ITSCA_NOALLOC_MAX_CLUSTERS) + " this entry point takes");
+  if (n > 0 && (!ev->x || !ev->y || !ev->z || !ev->cluster_id || !ev->mc_id))
+    return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_run_noalloc: missing cluster arrays");
+  for (int k = 0; k < 3; ++k)
+    if (!std::isfinite(ev->vertex[k]))
+      return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_run_noalloc: the vertex has a non-finite coordinate");
+  return guarded(ctx, "itsca_run_noalloc", [&]() -> itsca_status {
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (!ctx->scratchPtr) return ctx->fail(ITSCA_ERR_STATE, "itsca_run_noalloc: no device scratch");
+    DevState s{};
+    int* planScratch = nullptr;
+    if (!carveNoAlloc(ctx->scratchPtr, ITSCA_NOALLOC_SCRATCH_BYTES, (int)n, s, planScratch))
+      return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_run_noalloc: the event does not fit the device scratch");
+    s.W = 1;
+    s.chunk = ctx->blockSize;
+    s.zSub = ctx->zSub;
+    s.rx = ev->x;
+    s.ry = ev->y;
+    s.rz = ev->z;
+    s.rid = ev->cluster_id;
+    s.rmc = ev->mc_id;
+    s.mlpW = ctx->mlpWbf.p;
+    // small inputs: one work item, its layer segments, the vertex
+    for (int l = 0; l < 8; ++l) ctx->naEvOff[l] = offs[l];
+    for (int l = 0; l < kLayers; ++l) ctx->naClSeg[l] = Seg{offs[l] - offs[0], offs[l + 1] - offs[l]};
+    for (int k = 0; k < 3; ++k) ctx->naVertex[k] = ev->vertex[k];
+    ctx->naZero[0] = ctx->naZero[1] = 0;
+    HIP_TRY(hipMemcpyAsync(const_cast<int*>(s.evLayerOff), ctx->naEvOff, sizeof ctx->naEvOff, hipMemcpyHostToDevice,
+                           ctx->stream));
+    HIP_TRY(hipMemcpyAsync(const_cast<Seg*>(s.clSeg), ctx->naClSeg, sizeof ctx->naClSeg, hipMemcpyHostToDevice,
+                           ctx->stream));
+    HIP_TRY(hipMemcpyAsync(const_cast<float*>(s.itemVertex), ctx->naVertex, sizeof ctx->naVertex,
+                           hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(const_cast<int*>(s.itemEvent), ctx->naZero, 4, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(const_cast<int*>(s.recItem), ctx->naZero, 4, hipMemcpyHostToDevice, ctx->stream));
+    LaunchConfig cfg{ctx->persistentBlocks, planScratch, maxLayer, ctx->filterOn && ctx->mlpWbf.p != nullptr,
+                     ctx->threshold};
+    launchPipeline(s, cfg, ctx->stream, nullptr);
+    if (roads_cap > 0) launchRoadsOut(s, 0, roads, roads_cap, ctx->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(&ctx->naRoadSeg, s.roadSeg, sizeof(Seg), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->naStatus, s.status, sizeof ctx->naStatus, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const int* st = ctx->naStatus;
+    if (st[kStatusRootRoads] != 0)
+      return ctx->fail(ITSCA_ERR_CAPACITY,
+                       "itsca_run_noalloc: a root cell has more than " + std::to_string(kMaxRootRoads) + " roads");
+    if (st[kStatusUndecodable] != 0)
+      return ctx->fail(ITSCA_ERR_CAPACITY, "itsca_run_noalloc: a road could not be decoded from the path counts");
+    for (int k = 0; k < kNumStatus; ++k)
+      if (st[k] != 0)
+        return ctx->fail(ITSCA_ERR_CAPACITY, "itsca_run_noalloc: stage " + std::to_string(k) + " needs " +
+                                                 std::to_string(st[k]) + " outputs, more than the device scratch holds");
+    *num_roads = ctx->naRoadSeg.count;
+    if (ctx->naRoadSeg.count > roads_cap)
+      return ctx->fail(ITSCA_ERR_CAPACITY, "itsca_run_noalloc: " + std::to_string(ctx->naRoadSeg.count) +
+                                               " roads, more than the " + std::to_string(roads_cap) + " rows given");
+    return ITSCA_OK;
+  });
 }
 
 itsca_status itsca_get_counts(itsca_ctx* ctx, int item, itsca_counts* out)

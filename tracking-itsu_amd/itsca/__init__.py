@@ -48,6 +48,14 @@ class Capacities(ctypes.Structure):
                 ("roads", ctypes.c_long)]
 
 
+class DeviceEvent(ctypes.Structure):
+    """itsca_device_event: device addresses of one event's cluster arrays,
+    host layer offsets and vertex."""
+    _fields_ = [("x", ctypes.c_void_p), ("y", ctypes.c_void_p), ("z", ctypes.c_void_p),
+                ("cluster_id", ctypes.c_void_p), ("mc_id", ctypes.c_void_p),
+                ("layer_offsets", ctypes.c_int32 * 8), ("vertex", ctypes.c_float * 3)]
+
+
 class Counts(ctypes.Structure):
     _fields_ = [
         ("clusters", ctypes.c_int * 7),
@@ -61,7 +69,7 @@ class Counts(ctypes.Structure):
 HIP_SYMBOLS = [
     "itsca_create", "itsca_destroy", "itsca_last_error", "itsca_clear_events", "itsca_add_event", "itsca_upload",
     "itsca_run", "itsca_run_async", "itsca_finish", "itsca_num_items", "itsca_get_counts", "itsca_download",
-    "itsca_get_capacities",
+    "itsca_get_capacities", "itsca_run_noalloc",
     "itsca_set_profiling", "itsca_stage_times", "itsca_kernel_times", "itsca_kernel_name", "itsca_set_graph",
     "itsca_set_tuning", "itsca_get_tuning", "itsca_autotune",
     "itsca_mlp_init", "itsca_mlp_set_params", "itsca_mlp_get_params", "itsca_set_filter", "itsca_train_clear",
@@ -109,6 +117,7 @@ def bind_hip(lib: ctypes.CDLL) -> ctypes.CDLL:
             getattr(lib, f).argtypes = [P]
         lib.itsca_get_counts.argtypes = [P, I, ctypes.POINTER(Counts)]
         lib.itsca_get_capacities.argtypes = [P, ctypes.POINTER(Capacities)]
+        lib.itsca_run_noalloc.argtypes = [P, ctypes.POINTER(DeviceEvent), P, I, ctypes.POINTER(I)]
         lib.itsca_download.argtypes = [P, I, I, I, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         lib.itsca_set_profiling.argtypes = [P, I]
         lib.itsca_set_graph.argtypes = [P, I]
@@ -431,6 +440,21 @@ class GpuTracker:
         c = Capacities()
         self._check(self._lib.itsca_get_capacities(self.ctx, ctypes.byref(c)), "itsca_get_capacities")
         return c
+
+    def run_noalloc(self, arrays, layer_offsets, vertex, roads, roads_cap: int) -> int:
+        """itsca_run_noalloc: `arrays` are the device addresses of x, y, z,
+        cluster ids and MC ids, `roads` the device address of roads_cap rows
+        of 8 int; returns the number of roads."""
+        ev = DeviceEvent(*[ctypes.c_void_p(int(a)) for a in arrays],
+                         (ctypes.c_int32 * 8)(*[int(o) for o in layer_offsets]),
+                         (ctypes.c_float * 3)(*[float(v) for v in vertex]))
+        n = ctypes.c_int(0)
+        st = self._lib.itsca_run_noalloc(self.ctx, ctypes.byref(ev), ctypes.c_void_p(int(roads)), int(roads_cap),
+                                         ctypes.byref(n))
+        if st != 0:
+            raise ItscaError("itsca_run_noalloc: %s (status %d, roads %d)"
+                             % (self._lib.itsca_last_error(self.ctx).decode(), st, n.value))
+        return n.value
 
     def counts(self, item: int) -> Counts:
         c = Counts()
