@@ -69,6 +69,27 @@ def kernel_bytes(name, c, kept=None):
     return 0
 
 
+STAGE_KERNELS = {"prep+index tables": ("kPrep",), "tracklets": ("kTrackletsCount", "kTrackletsScore", "kTrackletsFill"),
+                 "cells": ("kCellsCount", "kCellsFill"), "neighbours": ("kEdgesCount", "kEdgesFill", "kEdgesTranspose"),
+                 "levels+roads": ("kLevels", "kRoadsFill")}
+
+
+def stage_bandwidth(counts, kernel_ms, kept=None):
+    """Per stage: kernel time (HIP events), algorithmic bytes (the per-kernel
+    figures of kernel_bytes summed over the stage's kernels and the batch) and
+    their ratio as GB/s and as a fraction of the 8 TB/s HBM peak."""
+    out = {}
+    kept = kept or [None] * len(counts)
+    for stage, kernels in STAGE_KERNELS.items():
+        ms = sum(kernel_ms.get(k, 0.0) for k in kernels)
+        b = sum(kernel_bytes(k, c, kk) for k in kernels if k in kernel_ms for c, kk in zip(counts, kept))
+        if ms > 0:
+            gbs = b / (ms * 1e-3) / 1e9
+            out[stage] = {"ms": round(ms, 4), "bytes_alg": b, "GBps": round(gbs, 1),
+                          "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ca_central_batch256_hbm_traffic.md")
 
 
@@ -92,23 +113,44 @@ def measured_traffic(kernel, batch, seed):
     return None
 
 
-def cpu_baseline(events_path, n_events, seed):
+def cpu_baseline(itsca, tmp, n_events, seed, per_core):
     """The reference CPU tracker itself (built from its sources into
-    oracle/_ref/), single thread, on a bounded sample of the same workload."""
+    oracle/_ref/) on a bounded sample of the same workload: one independent
+    tracker process per host core (at most 16, the host share of one GPU),
+    each on its own slice of per_core events, timed together (events over the
+    slowest process's time); and the same binary on one core."""
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
     if os.path.exists(exe):
+        events_path = os.path.join(tmp, "cpu_sample.txt")
+        itsca.write_events(2, seed, n_events, events_path)
         out = subprocess.run([exe, "time", events_path, "1"], check=True, capture_output=True, text=True).stdout
-        r = json.loads(out.strip().splitlines()[-1])
-        return {"value": round(r["events"] / r["seconds"], 3), "unit": "events/s", "cores": 1, "kind": "reference",
-                "sample": "%d central events (seeds %d..%d), single pass after one warm-up event, %.1f s"
-                          % (n_events, seed, seed + n_events - 1, r["seconds"])}
+        single = json.loads(out.strip().splitlines()[-1])
+        cores = max(1, min(16, os.cpu_count() or 1))
+        paths = []
+        for k in range(cores):
+            p = os.path.join(tmp, "cpu_core%d.txt" % k)
+            itsca.write_events(2, seed + k * per_core, per_core, p)
+            paths.append(p)
+        t0 = time.perf_counter()
+        procs = [subprocess.Popen([exe, "time", p, "1"], stdout=subprocess.PIPE, text=True) for p in paths]
+        res = [json.loads(pr.communicate()[0].strip().splitlines()[-1]) for pr in procs]
+        wall = time.perf_counter() - t0
+        slowest = max(r["seconds"] for r in res)
+        total = sum(r["events"] for r in res)
+        return {"value": round(total / slowest, 3), "unit": "events/s", "cores": cores, "kind": "reference",
+                "sample": "%d central events (seeds %d..%d), %d per process, one process per core, each a single "
+                          "pass after one warm-up event; slowest process %.1f s (wall %.1f s)"
+                          % (total, seed, seed + cores * per_core - 1, per_core, slowest, wall),
+                "single_core": {"value": round(single["events"] / single["seconds"], 3), "cores": 1,
+                                "sample": "%d central events (seeds %d..%d), %.1f s"
+                                          % (n_events, seed, seed + n_events - 1, single["seconds"])}}
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle  # noqa: E402
-    import itsca  # noqa: E402
-    evs = itsca.load_events(events_path)
-    s = pyoracle.time_events(evs, 1, 1)
-    return {"value": round(len(evs) / s, 3), "unit": "events/s", "cores": 1, "kind": "port",
-            "sample": "%d central events, single pass" % len(evs)}
+    evs = itsca.generate_events(2, seed, n_events)
+    cores = max(1, min(16, os.cpu_count() or 1))
+    s = pyoracle.time_events(evs, 1, cores)
+    return {"value": round(len(evs) / s, 3), "unit": "events/s", "cores": cores, "kind": "port",
+            "sample": "%d central events, single pass, one tracker per thread" % len(evs)}
 
 
 def time_form(tracker, steps, warmup):
@@ -147,18 +189,82 @@ def kernel_profile(tracker, runs=5):
     return kernel_ms, launches, stage_ms
 
 
-def road_quality(tracker):
-    """Fake roads (Moore vote of the MC labels, Tracker.cxx:573-652), and
-    distinct MC labels among the other roads, over all work items."""
-    fake = correct = 0
-    for i in range(tracker.num_items()):
-        r = tracker.roads(i)
-        if len(r) == 0:
+def road_report(itsca, tracker, events, events_path, labels_path):
+    """Correct, duplicate and fake roads against the MC labels map, classified
+    like the reference's road report (IOUtils.cxx:80-170; pions with 7
+    clusters), summed over the events of the batch."""
+    roads, item = [], 0
+    for ev in events:
+        roads.append([tracker.roads(item + v) for v in range(len(ev.vertices))])
+        item += len(ev.vertices)
+    counts = itsca.roads_report(events_path, labels_path, roads)
+    return {"correct": sum(c[0] for c in counts), "duplicate": sum(c[1] for c in counts),
+            "fake": sum(c[2] for c in counts)}
+
+
+def end_to_end(itsca, events, params, threshold, batches=6):
+    """The whole path a caller of the reference entry sees, per batch of
+    events held in host memory: events in (itsca_add_event + itsca_upload),
+    the pipeline, roads out to host memory (itsca_download_roads), no graph.
+    Two contexts on two host threads, each with its own HIP stream, take
+    alternate batches, so one batch's copies overlap the other's kernels."""
+    import threading
+    out = {}
+    for form in ("plain", "learned"):
+        trackers = [itsca.GpuTracker(device=0) for _ in range(2)]
+        roads = [0, 0]
+        for t in trackers:
+            t.set_mlp_params(params)
+            t.set_filter(form == "learned", threshold)
+            t.set_events(events)  # sizes the buffers
+            t.run()
+            t.all_roads()
+
+        def worker(k):
+            t = trackers[k]
+            for _ in range(k, batches, 2):
+                t.set_events(events)
+                t.run()
+                rows, _ = t.all_roads()
+                roads[k] += len(rows)
+
+        t0 = time.perf_counter()
+        threads = [threading.Thread(target=worker, args=(k,)) for k in range(2)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        elapsed = time.perf_counter() - t0
+        for t in trackers:
+            t.close()
+        out[form] = {"events_per_s": round(batches * len(events) / elapsed, 2),
+                     "ms_per_batch": round(1e3 * elapsed / batches, 3), "roads_to_host": sum(roads)}
+    out["workload"] = ("%d batches of the %d central events, host memory to host memory: upload, run, roads "
+                       "download; two contexts on two threads alternating batches" % (batches, len(events)))
+    return out
+
+
+def cpp_api_latency(itsca, tmp, seed, n=12):
+    """Tracker<true>::clustersToTracks(event), the C++ API a reference caller
+    uses, one event per call (upload, run, roads back by value), through
+    bin/itsca-track: mean ms per central event after its warm-up event."""
+    import re
+    path = os.path.join(tmp, "api_events.txt")
+    itsca.write_events(2, seed, n, path)
+    exe = os.path.join(itsca.BIN_DIR, "itsca-track")
+    out = {}
+    for form, extra in (("plain", []), ("learned", ["--filter", os.path.join(ROOT, "tracking-itsu_amd", "itsca",
+                                                                           "filter_weights.json")])):
+        r = subprocess.run([exe, path] + extra, capture_output=True, text=True, timeout=300)
+        if r.returncode != 0:
+            out[form] = {"error": r.stdout[-300:] + r.stderr[-300:]}
             continue
-        fake += int((r[:, 7] != 0).sum())
-        good = r[r[:, 7] == 0][:, 6]
-        correct += len(set(int(x) for x in good.tolist() if x >= 0))
-    return fake, correct
+        ms = [float(x) for x in re.findall(r"processed in: ([0-9.eE+-]+)ms", r.stdout)]
+        out[form] = {"ms_per_event": round(sum(ms) / len(ms), 4), "events": len(ms),
+                     "events_per_s": round(1e3 * len(ms) / sum(ms), 2)}
+    out["workload"] = "%d central events (seeds %d..%d), one Tracker<true>::clustersToTracks call each" % (
+        n, seed, seed + n - 1)
+    return out
 
 
 def single_events(itsca, steps=50):
@@ -178,20 +284,30 @@ def single_events(itsca, steps=50):
     return out
 
 
-def mixed_batch(itsca, steps):
+def mixed_batch(itsca, steps, tuning_fixed=None):
     """256 events of log-uniform multiplicity in [50, 8000] tracks in one
-    launch, plain form, launch shape chosen by the autotuner, graph replay."""
+    launch, plain form, launch shape chosen by the autotuner, graph replay;
+    per-stage kernel times and algorithmic GB/s."""
     evs = itsca.generate_events(3, 1000, 256)
     t = itsca.GpuTracker(device=0)
     t.set_events(evs)
     t.run()
-    tuning, _ = t.autotune(3)
+    if tuning_fixed:
+        t.set_tuning(*[int(x) for x in tuning_fixed.split(",")])
+        tuning = t.tuning()
+    else:
+        tuning, _ = t.autotune(3)
     s = time_form(t, steps, 2)
+    counts = [t.counts(i) for i in range(t.num_items())]
+    kernel_ms, _, _ = kernel_profile(t)
     out = {"workload": "256 events, multiplicity log-uniform in [50, 8000] tracks (seeds 1000..1255), one launch",
            "form": "plain, hipGraph replay", "events_per_s": round(256 * steps / s, 2),
            "ms_per_step": round(1e3 * s / steps, 4),
            "tuning": {"block_size": tuning.block_size, "persistent_blocks": tuning.persistent_blocks,
-                      "z_subbins": tuning.z_subbins}}
+                      "z_subbins": tuning.z_subbins},
+           "kernel_ms": {k: round(v, 4) for k, v in sorted(kernel_ms.items(), key=lambda kv: -kv[1])},
+           "stage_bandwidth": stage_bandwidth(counts, kernel_ms),
+           "bytes_alg_per_step": sum(bytes_alg(c) for c in counts)}
     t.close()
     return out
 
@@ -199,25 +315,28 @@ def mixed_batch(itsca, steps):
 def pileup_batch(itsca, ft, params, threshold, steps, events=64):
     """Pile-up events (4 collisions, z ~ U(-10, 10) cm, 2000-8000 primaries
     each, 3 % noise), every vertex its own work item, plain and learned."""
-    evs = itsca.generate_events(5, 40000, events)
-    t = itsca.GpuTracker(device=0)
-    t.set_events(evs)
-    t.set_mlp_params(params)
-    t.set_filter(False)
-    s_plain = time_form(t, steps, 1)
-    q_plain = road_quality(t)
-    t.set_filter(True, threshold)
-    s_learned = time_form(t, steps, 1)
-    q_learned = road_quality(t)
-    quality = ft.evaluate(t, evs, threshold)
-    t.close()
+    with tempfile.TemporaryDirectory() as tmp:
+        ev_path, lab_path = os.path.join(tmp, "pileup.txt"), os.path.join(tmp, "pileup_labels.txt")
+        itsca.write_events(5, 40000, events, ev_path, lab_path)
+        evs = itsca.load_events(ev_path)
+        t = itsca.GpuTracker(device=0)
+        t.set_events(evs)
+        t.set_mlp_params(params)
+        t.set_filter(False)
+        s_plain = time_form(t, steps, 1)
+        q_plain = road_report(itsca, t, evs, ev_path, lab_path)
+        t.set_filter(True, threshold)
+        s_learned = time_form(t, steps, 1)
+        q_learned = road_report(itsca, t, evs, ev_path, lab_path)
+        quality = ft.evaluate(t, evs, threshold)
+        t.close()
     return {"workload": "%d pile-up events (4 vertices each, seeds 40000..%d), one launch" % (events, 40000 + events - 1),
             "plain_events_per_s": round(events * steps / s_plain, 2),
             "learned_events_per_s": round(events * steps / s_learned, 2),
             "speedup": round(s_plain / s_learned, 4), "track_recall": round(quality["recall"], 6),
             "plain_tracks": quality["plain_tracks"], "tracks_outside_plain": quality["extra"],
-            "fake_roads": {"plain": q_plain[0], "learned": q_learned[0]},
-            "correct_labels": {"plain": q_plain[1], "learned": q_learned[1]},
+            "road_report": {"plain": q_plain, "learned": q_learned,
+                            "method": "IOUtils::writeRoadsReport classification against the labels map"},
             "tracklets_kept": round(quality["tracklets_kept"], 4)}
 
 
@@ -228,7 +347,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=256, help="central events per step")
     ap.add_argument("--seed", type=int, default=HELD_OUT_SEED, help="first seed of the central events")
-    ap.add_argument("--cpu-events", type=int, default=200, help="events in the CPU-baseline sample")
+    ap.add_argument("--cpu-events", type=int, default=200, help="events in the single-core CPU-baseline sample")
+    ap.add_argument("--cpu-events-per-core", type=int, default=40,
+                    help="events per process in the all-cores CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed-multiplicity and pile-up batches")
     ap.add_argument("--weights", default=None, help="learned filter weights (default: the shipped file)")
@@ -236,6 +357,9 @@ def main():
                     help="fixed block size, persistent workgroups and z sub-bins of the plain form instead of the "
                          "autotuner's choice (for profiler runs)")
     ap.add_argument("--tuning-learned", default=None, metavar="B,P,Z", help="the same for the learned form")
+    ap.add_argument("--only", choices=["central", "mixed"], default=None,
+                    help="run one workload alone (for profiler runs): the central batch (both forms) or the "
+                         "mixed-multiplicity batch")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy kernel of this many bytes (a reference dispatch of "
                          "known traffic for hardware-counter runs)")
@@ -258,6 +382,10 @@ def main():
         print("# copy kernel: %d bytes read, %d bytes written" % (args.copy_bytes, args.copy_bytes), file=sys.stderr,
               flush=True)
         del src, dst
+
+    if args.only == "mixed":
+        print(json.dumps({"mixed_batch": mixed_batch(itsca, args.steps, args.tuning_plain)}), flush=True)
+        return
 
     t_gen = time.time()
     events = itsca.generate_events(2, args.seed, args.batch)
@@ -332,7 +460,8 @@ def main():
         "plain": {"value": round(plain_eps, 2), "unit": "events/s", "ms_per_step": round(1e3 * plain_s / args.steps, 4),
                   "tuning": plain_tuning,
                   "kernel_ms": {k: round(v, 4) for k, v in sorted(plain_kernels.items(), key=lambda kv: -kv[1])},
-                  "stage_ms": {STAGES[k]: round(plain_stages[k], 4) for k in range(5)}},
+                  "stage_ms": {STAGES[k]: round(plain_stages[k], 4) for k in range(5)},
+                  "stage_bandwidth": stage_bandwidth(counts, plain_kernels)},
         "learned_vs_plain": {"speedup": round(events_per_s / plain_eps, 4),
                              "track_recall": round(quality["recall"], 6), "plain_tracks": quality["plain_tracks"],
                              "tracks_found": quality["found"], "tracks_outside_plain": quality["extra"],
@@ -346,24 +475,30 @@ def main():
                      "pipeline_GBps": round(balg / (ms_per_step * 1e-3) / 1e9, 2)},
         "kernel_ms": {k: round(v, 4) for k, v in sorted(kernel_ms.items(), key=lambda kv: -kv[1])},
         "stage_ms": {STAGES[k]: round(stage_ms[k], 4) for k in range(5)},
+        "stage_bandwidth": stage_bandwidth(counts, kernel_ms, kept),
         "counts_event0": {"tracklets": list(counts[0].tracklets), "cells": list(counts[0].cells),
                           "roads": counts[0].roads},
     }
     tracker.close()
+    if args.only == "central":
+        print(json.dumps(line), flush=True)
+        return
     if not args.no_extra:
+        tmpdir = tempfile.TemporaryDirectory()
         for key, fn in (("single_event", lambda: single_events(itsca)),
                         ("mixed_batch", lambda: mixed_batch(itsca, 5)),
-                        ("pileup_batch", lambda: pileup_batch(itsca, ft, params, threshold, 3))):
+                        ("pileup_batch", lambda: pileup_batch(itsca, ft, params, threshold, 3)),
+                        ("end_to_end", lambda: end_to_end(itsca, events, params, threshold)),
+                        ("cpp_api_per_event", lambda: cpp_api_latency(itsca, tmpdir.name, args.seed))):
             try:
                 line[key] = fn()
             except Exception as e:  # reported in the line; the main measurement stands
                 line[key] = {"error": "%s: %s" % (type(e).__name__, e)}
             print("# %s done" % key, file=sys.stderr, flush=True)
+        tmpdir.cleanup()
     if not args.no_cpu:
         with tempfile.TemporaryDirectory() as tmp:
-            p = os.path.join(tmp, "cpu_sample.txt")
-            itsca.write_events(2, args.seed, args.cpu_events, p)
-            line["cpu_baseline"] = cpu_baseline(p, args.cpu_events, args.seed)
+            line["cpu_baseline"] = cpu_baseline(itsca, tmp, args.cpu_events, args.seed, args.cpu_events_per_core)
     print(json.dumps(line), flush=True)
 
 

@@ -99,6 +99,16 @@ itsca_status itsca_finish(itsca_ctx* ctx);
 
 int itsca_num_items(const itsca_ctx* ctx);
 
+/* The roads of every work item of the last run in one copy: rows of 8 int
+ * (the ITSCA_ROADS format), item after item; first_road[i] is the first row
+ * of item i and first_road[num_items] the total (first_road holds
+ * num_items + 1 entries).  cap_rows is the capacity of rows; *total receives
+ * the number of roads (ITSCA_ERR_ARGUMENT, nothing copied, when it exceeds
+ * cap_rows).  The batch counterpart of the roads that
+ * Tracker::clustersToTracks returns by value (Tracker.cxx:244-265). */
+itsca_status itsca_download_roads(itsca_ctx* ctx, int32_t* rows, size_t cap_rows, int32_t* first_road,
+                                  size_t* total);
+
 /* Output capacities of the device buffers after the last run (elements, all
  * work items and layers together); they grow on overflow and never shrink.
  * Counterpart of the capacities the reference reserves per layer

@@ -39,6 +39,16 @@ int itsca_events_load_labels(itsca_events* ev, const char* labels_path);
  * Returns a new batch (free it with itsca_events_free), NULL on error. */
 itsca_events* itsca_events_merge_pileup(const itsca_events* ev, int pile_up);
 
+/* Road report of one event, as IOUtils::writeRoadsReport classifies it
+ * (IOUtils.cxx:125-170), for a batch that holds labels (itsca_events_write
+ * with labels, or itsca_events_load_labels): roads[v] holds num_roads[v] rows
+ * of 8 int of vertex v (the ITSCA_ROADS rows of itsca.h: 5 cells, size,
+ * label, fake).  Only labels of charged pions with 7 clusters count
+ * (IOUtils.cxx:109).  counts receives the correct, duplicate and fake roads;
+ * returns 0, or -1 if the batch has no labels for the event. */
+int itsca_events_roads_report(const itsca_events* ev, int event, const int32_t* const* roads,
+                              const int32_t* num_roads, int num_vertices, long counts[3]);
+
 int itsca_events_count(const itsca_events* ev);
 int itsca_events_num_vertices(const itsca_events* ev, int event);
 /* offsets[8]: layer l occupies [offsets[l], offsets[l+1]) of the copy below. */

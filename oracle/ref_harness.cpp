@@ -14,6 +14,8 @@
 //          estimate vs actual)
 //        ref_harness time <events.txt> <repeats>
 //          times the same per-vertex sequence per event (after one warm-up run)
+//        ref_harness report <events.txt> <labels.txt> <dir>
+//          writes the reference's road report files into <dir>
 //
 // The reference's tracklet search reads one element past the end of the next
 // layer's cluster vector whenever the window's last bin is at or after the
@@ -38,7 +40,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <new>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -193,12 +197,40 @@ int timeMode(const char* events, int repeats)
   return 0;
 }
 
+// The reference's road report (main.cpp:49-61, 130-134): every event's roads
+// (Tracker.cxx:244-265, through trackVertex) classified against the labels
+// file by IOUtils::loadLabels and IOUtils::writeRoadsReport
+// (IOUtils.cxx:80-170), into <dir>/CorrectRoads.txt, DuplicateRoads.txt and
+// FakeRoads.txt.
+int reportMode(const char* events, const char* labels, const std::string& dir)
+{
+  std::vector<Event> evs = IOUtils::loadEventData(events);
+  std::vector<std::unordered_map<int, Label>> labelsMap = IOUtils::loadLabels((int)evs.size(), labels);
+  std::ofstream correct(dir + "/CorrectRoads.txt"), duplicate(dir + "/DuplicateRoads.txt"),
+      fake(dir + "/FakeRoads.txt");
+  Tracker<false> tracker{};
+  for (size_t e = 0; e < evs.size(); ++e) {
+    std::vector<std::vector<Road>> roads;
+    for (int v = 0; v < evs[e].getPrimaryVerticesNum(); ++v) {
+      trackVertex(tracker, evs[e], v);
+      roads.emplace_back(tracker.mPrimaryVertexContext.getRoads());
+    }
+    if (e < labelsMap.size()) IOUtils::writeRoadsReport(correct, duplicate, fake, roads, labelsMap[e]);
+    std::printf("event %zu:", e);
+    for (const auto& r : roads) std::printf(" %zu", r.size());
+    std::printf(" roads\n");
+  }
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv)
 {
+  if (argc >= 5 && !std::strcmp(argv[1], "report")) return reportMode(argv[2], argv[3], argv[4]);
   if (argc >= 4 && !std::strcmp(argv[1], "dump")) return dumpMode(argv[2], argv[3]);
   if (argc >= 4 && !std::strcmp(argv[1], "time")) return timeMode(argv[2], std::atoi(argv[3]));
-  std::fprintf(stderr, "usage: ref_harness dump <events> <prefix> | time <events> <repeats>\n");
+  std::fprintf(stderr, "usage: ref_harness dump <events> <prefix> | time <events> <repeats> | "
+                       "report <events> <labels> <dir>\n");
   return 2;
 }

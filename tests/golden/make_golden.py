@@ -8,6 +8,8 @@ harness).  Output, all under tests/golden/:
   <case>.events.txt            the input event (reference text format)
   <case>.ref.v<k>.txt          reference dump of vertex k, as-is build (std::sort)
   <case>.stable.v<k>.txt       reference dump of vertex k, stable-order build
+  report_*                     road reports of events with MC labels (see
+                               write_report_fixtures)
   central_s2.json              for the 8000-track central event (not committed
                                as text): SHA-256 of the generated event file and
                                of both dumps, and the output counts
@@ -123,6 +125,53 @@ def outside_event():
                        np.array(offs, np.int32), base.vertices.copy())
 
 
+REPORT_FILES = ("CorrectRoads.txt", "DuplicateRoads.txt", "FakeRoads.txt")
+
+
+def write_report_fixtures():
+    """The reference's road report (IOUtils::loadLabels + writeRoadsReport run
+    by the harness) for events with MC labels:
+      report_s21       4 small events (preset 1, seeds 21..24)
+      report_pileup3   6 small events (seeds 31..36) merged into 2 pile-up
+                       events of 3 vertices (labels of the k-th collision
+                       offset by (k - 1) * 100000)
+    committed as <case>.ev.txt, <case>.labels.txt and
+    <case>.<ref|stable>.<report file>; and for one central event (preset 2,
+    seed 20000) the SHA-256 of each report file in report_central.json."""
+    with tempfile.TemporaryDirectory() as tmp:
+        itsca.write_events(1, 21, 4, os.path.join(HERE, "report_s21.ev.txt"),
+                           os.path.join(HERE, "report_s21.labels.txt"))
+        src, lab = os.path.join(tmp, "src.txt"), os.path.join(tmp, "src_lab.txt")
+        itsca.write_events(1, 31, 6, src, lab)
+        itsca.merge_pileup(src, 3, os.path.join(HERE, "report_pileup3.ev.txt"), lab,
+                           os.path.join(HERE, "report_pileup3.labels.txt"))
+        for case in ("report_s21", "report_pileup3"):
+            for kind, exe in (("ref", HARNESS), ("stable", HARNESS_STABLE)):
+                out = os.path.join(tmp, case + kind)
+                os.makedirs(out)
+                subprocess.run([exe, "report", os.path.join(HERE, case + ".ev.txt"),
+                                os.path.join(HERE, case + ".labels.txt"), out], check=True, capture_output=True)
+                for f in REPORT_FILES:
+                    with open(os.path.join(out, f)) as a, open(os.path.join(HERE, "%s.%s.%s" % (case, kind, f)), "w") as b:
+                        b.write(a.read())
+            print(case, "report written")
+        ev, lab = os.path.join(tmp, "central.txt"), os.path.join(tmp, "central_lab.txt")
+        itsca.write_events(2, 20000, 1, ev, lab)
+        info = {"preset": 2, "seed": 20000, "events_sha256": hashlib.sha256(open(ev, "rb").read()).hexdigest(),
+                "labels_sha256": hashlib.sha256(open(lab, "rb").read()).hexdigest()}
+        for kind, exe in (("ref", HARNESS), ("stable", HARNESS_STABLE)):
+            out = os.path.join(tmp, "central" + kind)
+            os.makedirs(out)
+            subprocess.run([exe, "report", ev, lab, out], check=True, capture_output=True)
+            for f in REPORT_FILES:
+                text = open(os.path.join(out, f)).read()
+                info["%s_%s_sha256" % (kind, f)] = hashlib.sha256(text.encode()).hexdigest()
+                info["%s_%s_lines" % (kind, f)] = text.count("\n")
+        with open(os.path.join(HERE, "report_central.json"), "w") as f:
+            json.dump(info, f, indent=1)
+        print("report_central", info)
+
+
 def run_harness(exe, events_path, prefix):
     out = subprocess.run([exe, "dump", events_path, prefix], check=True, capture_output=True, text=True)
     return out.stdout
@@ -132,6 +181,7 @@ def main():
     for exe in (HARNESS, HARNESS_STABLE):
         if not os.path.exists(exe):
             sys.exit(f"{exe} missing: run make -C oracle ref")
+    write_report_fixtures()
     for name, events in small_cases().items():
         ev_path = os.path.join(HERE, name + ".events.txt")
         write_event_file(events, ev_path)

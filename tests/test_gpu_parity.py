@@ -302,3 +302,65 @@ def test_autotune_keeps_results():
     for i in range(3):
         assert np.array_equal(t.roads(i), ref[i])
     t.close()
+
+
+REPORT_FILES = ("CorrectRoads.txt", "DuplicateRoads.txt", "FakeRoads.txt")
+
+
+def _track_report(tmp, events_path, labels_path):
+    """bin/itsca-track <events> <labels> in tmp: the report files it writes."""
+    import shutil
+    import subprocess
+    ev, lab = os.path.join(tmp, "ev.txt"), os.path.join(tmp, "labels.txt")
+    shutil.copy(events_path, ev)
+    shutil.copy(labels_path, lab)
+    out = subprocess.run([os.path.join(itsca.BIN_DIR, "itsca-track"), ev, lab], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    return {f: open(os.path.join(tmp, f)).read() for f in REPORT_FILES}
+
+
+def _blocks(text):
+    """Rows per event of a report file with -1 separators, as sets."""
+    out = []
+    for line in text.splitlines():
+        if line.strip() == "-1":
+            out.append(set())
+        elif out:
+            out[-1].add(line)
+    return out
+
+
+@pytest.mark.parametrize("case", ["report_s21", "report_pileup3"])
+def test_cli_road_report_matches_reference(golden_dir, case):
+    """itsca-track writes the reference's road report files: byte for byte
+    those of the stable-order reference build, and per event the same sets of
+    correct and fake labels as the reference as-is.  The as-is build's sort
+    order moves roads and changes which roads the inclusive candidate bound
+    duplicates (test_matches_unmodified_reference_as_keyed_sets), so its rows
+    come in another order and its DuplicateRoads.txt lists other repeats."""
+    say(f"gpu cli road report {case}")
+    with tempfile.TemporaryDirectory() as tmp:
+        got = _track_report(tmp, os.path.join(golden_dir, case + ".ev.txt"),
+                            os.path.join(golden_dir, case + ".labels.txt"))
+    for f in REPORT_FILES:
+        want = open(os.path.join(golden_dir, "%s.stable.%s" % (case, f))).read()
+        assert got[f] == want, f
+    for f in ("CorrectRoads.txt", "FakeRoads.txt"):
+        ref = open(os.path.join(golden_dir, "%s.ref.%s" % (case, f))).read()
+        assert _blocks(got[f]) == _blocks(ref), f
+
+
+def test_cli_road_report_central_event(golden_dir):
+    import json
+    say("gpu cli road report, central event")
+    info = json.load(open(os.path.join(golden_dir, "report_central.json")))
+    with tempfile.TemporaryDirectory() as tmp:
+        ev, lab = os.path.join(tmp, "c.txt"), os.path.join(tmp, "c_lab.txt")
+        itsca.write_events(info["preset"], info["seed"], 1, ev, lab)
+        assert hashlib.sha256(open(ev, "rb").read()).hexdigest() == info["events_sha256"]
+        assert hashlib.sha256(open(lab, "rb").read()).hexdigest() == info["labels_sha256"]
+        os.makedirs(os.path.join(tmp, "run"))
+        got = _track_report(os.path.join(tmp, "run"), ev, lab)
+    for f in REPORT_FILES:
+        assert hashlib.sha256(got[f].encode()).hexdigest() == info["stable_%s_sha256" % f], f

@@ -142,6 +142,7 @@ struct itsca_ctx {
   // results of the last run
   std::vector<Seg> trkSeg, celSeg, edgSeg, roadSeg;
   int status[kNumStatus] = {};
+  std::vector<int> hostRoads;  // staging of itsca_download_roads
 
   // itsca_run_noalloc: the device address of `scratch` on this device, and
   // fixed host staging for its small inputs and read-backs
@@ -705,6 +706,53 @@ itsca_status itsca_run(itsca_ctx* ctx)
 }
 
 int itsca_num_items(const itsca_ctx* ctx) { return ctx ? ctx->W : 0; }
+
+itsca_status itsca_download_roads(itsca_ctx* ctx, int32_t* rows, size_t cap_rows, int32_t* first_road,
+                                  size_t* total)
+{
+  return guarded(ctx, "itsca_download_roads", [&]() -> itsca_status {
+    if (!ctx->ran) return ctx->fail(ITSCA_ERR_STATE, "itsca_download_roads: no completed run");
+    if (!first_road || !total) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_download_roads: missing output");
+    // roads are compact in item order (one scan over the batch), so one copy
+    // per array covers every item
+    size_t n = 0;
+    for (int w = 0; w < ctx->W; ++w) {
+      first_road[w] = (int32_t)n;
+      n += (size_t)ctx->roadSeg[w].count;
+    }
+    first_road[ctx->W] = (int32_t)n;
+    *total = n;
+    if (n > cap_rows) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_download_roads: destination too small");
+    if (n == 0) return ITSCA_OK;
+    if (!rows) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_download_roads: missing rows");
+    const size_t base = (size_t)ctx->roadSeg[0].base;
+    for (int w = 1; w < ctx->W; ++w)
+      if ((size_t)ctx->roadSeg[w].base != base + (size_t)first_road[w])
+        return ctx->fail(ITSCA_ERR_STATE, "itsca_download_roads: roads are not in item order");
+    HIP_TRY(hipSetDevice(ctx->device));
+    ctx->hostRoads.resize(n * (kCellLayers + 2));
+    int* cells = ctx->hostRoads.data();
+    int* lab = cells + n * kCellLayers;
+    int* fake = lab + n;
+    HIP_TRY(hipMemcpyAsync(cells, ctx->rCells.p + base * kCellLayers, n * kCellLayers * 4, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HIP_TRY(hipMemcpyAsync(lab, ctx->rLabel.p + base, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(fake, ctx->rFake.p + base, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < n; ++i) {
+      int32_t* r = rows + i * 8;
+      int size = 0;
+      for (int k = 0; k < kCellLayers; ++k) {
+        r[k] = cells[i * kCellLayers + k];
+        size += r[k] >= 0 ? 1 : 0;
+      }
+      r[5] = size;
+      r[6] = lab[i];
+      r[7] = fake[i];
+    }
+    return ITSCA_OK;
+  });
+}
 
 itsca_status itsca_get_capacities(itsca_ctx* ctx, itsca_capacities* out)
 {

@@ -71,6 +71,8 @@ class Tracker<true> {
   // Runs vertex v of the event as its own work item with per-stage timing.
   void runVertexProfiled(const Event& event, int v, float ms[5]);
   std::vector<Road> downloadRoads(int item) const;
+  // The roads of every work item of the last run, in one copy per array.
+  std::vector<std::vector<Road>> allRoads() const;
   itsca_ctx* mCtx = nullptr;
 };
 

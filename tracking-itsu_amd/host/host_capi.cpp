@@ -6,9 +6,11 @@
 #include <new>
 #include <sstream>
 #include <string>
+#include <unordered_map>
 
 #include "events.hpp"
 #include "itsca_host.h"
+#include "roads_report.hpp"
 
 struct itsca_events {
   itsca::EventBatch batch;
@@ -75,6 +77,31 @@ itsca_events* itsca_events_merge_pileup(const itsca_events* ev, int pile_up)
 }
 
 void itsca_events_free(itsca_events* ev) { delete ev; }
+
+int itsca_events_roads_report(const itsca_events* ev, int event, const int32_t* const* roads,
+                              const int32_t* num_roads, int num_vertices, long counts[3])
+{
+  if (!ev || !counts || num_vertices < 0 || (num_vertices > 0 && (!roads || !num_roads))) return -1;
+  if (event < 0 || event >= (int)ev->batch.labels.size()) return -1;
+  // IOUtils.cxx:109: only charged pions that left 7 clusters are labels
+  std::unordered_map<int, int> labels;
+  for (const itsca::McLabel& m : ev->batch.labels[event])
+    if (std::abs(m.pdg) == 211 && m.nClusters == 7) labels.emplace(m.mc, m.mc);
+  counts[0] = counts[1] = counts[2] = 0;
+  itsca::RoadClassifier classify;
+  for (int v = 0; v < num_vertices; ++v)
+    for (int i = 0; i < num_roads[v]; ++i) {
+      const int32_t* row = roads[v] + (size_t)i * 8;  // 5 cells, size, label, fake
+      const int cls = classify.next(row[6], row[7] != 0, [&](int label, int& mc) {
+        const auto it = labels.find(label);
+        if (it == labels.end()) return false;
+        mc = it->second;
+        return true;
+      });
+      if (cls >= 0) ++counts[cls];
+    }
+  return 0;
+}
 
 int itsca_events_count(const itsca_events* ev) { return ev ? (int)ev->batch.events.size() : -1; }
 

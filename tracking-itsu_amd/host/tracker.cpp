@@ -10,6 +10,7 @@
 #include <unordered_set>
 
 #include "events.hpp"
+#include "roads_report.hpp"
 #include "ITSReconstruction/CA/IOUtils.h"
 #include "ITSReconstruction/CA/Tracker.h"
 #include "itsca.h"
@@ -135,12 +136,35 @@ std::vector<Road> Tracker<true>::downloadRoads(int item) const
   return roads;
 }
 
+std::vector<std::vector<Road>> Tracker<true>::allRoads() const
+{
+  const int items = itsca_num_items(mCtx);
+  std::vector<int32_t> first(items + 1);
+  size_t total = 0;
+  const itsca_status st = itsca_download_roads(mCtx, nullptr, 0, first.data(), &total);
+  if (st != ITSCA_OK && total == 0) check(mCtx, st, "itsca_download_roads");
+  std::vector<int32_t> rows(total * 8 + 1);
+  check(mCtx, itsca_download_roads(mCtx, rows.data(), total, first.data(), &total), "itsca_download_roads");
+  std::vector<std::vector<Road>> out(items);
+  for (int w = 0; w < items; ++w) {
+    out[w].reserve(first[w + 1] - first[w]);
+    for (int i = first[w]; i < first[w + 1]; ++i) {
+      const int32_t* r = &rows[(size_t)i * 8];
+      Road road;
+      for (int k = 0; k < Constants::ITS::CellsPerRoad; ++k)
+        if (r[k] >= 0) road.addCell(k, r[k]);
+      road.setLabel(r[6]);
+      road.setFakeRoad(r[7] != 0);
+      out[w].push_back(road);
+    }
+  }
+  return out;
+}
+
 std::vector<std::vector<Road>> Tracker<true>::clustersToTracks(const Event& event)
 {
   runBatch({&event});
-  std::vector<std::vector<Road>> roads;
-  for (int v = 0; v < event.getPrimaryVerticesNum(); ++v) roads.push_back(downloadRoads(v));
-  return roads;
+  return allRoads();
 }
 
 std::vector<std::vector<std::vector<Road>>> Tracker<true>::clustersToTracks(const std::vector<Event>& events)
@@ -148,10 +172,11 @@ std::vector<std::vector<std::vector<Road>>> Tracker<true>::clustersToTracks(cons
   std::vector<const Event*> ptrs;
   for (const Event& e : events) ptrs.push_back(&e);
   runBatch(ptrs);
+  std::vector<std::vector<Road>> items = allRoads();
   std::vector<std::vector<std::vector<Road>>> out(events.size());
-  int item = 0;
+  size_t item = 0;
   for (size_t e = 0; e < events.size(); ++e)
-    for (int v = 0; v < events[e].getPrimaryVerticesNum(); ++v) out[e].push_back(downloadRoads(item++));
+    for (int v = 0; v < events[e].getPrimaryVerticesNum(); ++v) out[e].push_back(std::move(items[item++]));
   return out;
 }
 
@@ -357,21 +382,19 @@ std::vector<std::unordered_map<int, Label>> loadLabels(int eventsNum, const std:
 void writeRoadsReport(std::ofstream& correct, std::ofstream& duplicate, std::ofstream& fake,
                       const std::vector<std::vector<Road>>& roads, const std::unordered_map<int, Label>& labels)
 {
-  std::unordered_set<int> found;
+  itsca::RoadClassifier classify;
+  std::ofstream* out[3] = {&correct, &duplicate, &fake};
   correct << -1 << std::endl;
   fake << -1 << std::endl;
   for (const std::vector<Road>& vertexRoads : roads) {
     for (const Road& road : vertexRoads) {
       const auto it = labels.find(road.getLabel());
-      if (it == labels.end()) continue;
-      if (road.isFakeRoad()) {
-        fake << it->second << std::endl;
-      } else if (found.count(it->second.monteCarloId)) {
-        duplicate << it->second << std::endl;
-      } else {
-        correct << it->second << std::endl;
-        found.insert(it->second.monteCarloId);
-      }
+      const int cls = classify.next(road.getLabel(), road.isFakeRoad(), [&](int, int& mc) {
+        if (it == labels.end()) return false;
+        mc = it->second.monteCarloId;
+        return true;
+      });
+      if (cls >= 0) *out[cls] << it->second << std::endl;
     }
   }
 }

@@ -69,7 +69,7 @@ class Counts(ctypes.Structure):
 HIP_SYMBOLS = [
     "itsca_create", "itsca_destroy", "itsca_last_error", "itsca_clear_events", "itsca_add_event", "itsca_upload",
     "itsca_run", "itsca_run_async", "itsca_finish", "itsca_num_items", "itsca_get_counts", "itsca_download",
-    "itsca_get_capacities", "itsca_run_noalloc",
+    "itsca_get_capacities", "itsca_run_noalloc", "itsca_download_roads",
     "itsca_set_profiling", "itsca_stage_times", "itsca_kernel_times", "itsca_kernel_name", "itsca_set_graph",
     "itsca_set_tuning", "itsca_get_tuning", "itsca_autotune",
     "itsca_mlp_init", "itsca_mlp_set_params", "itsca_mlp_get_params", "itsca_set_filter", "itsca_train_clear",
@@ -83,6 +83,7 @@ HOST_SYMBOLS = [
     "itsca_events_generate", "itsca_events_load", "itsca_events_write", "itsca_events_free", "itsca_events_count",
     "itsca_events_num_vertices", "itsca_events_layer_offsets", "itsca_events_copy", "itsca_filter_load",
     "itsca_events_write_binary", "itsca_events_load_labels", "itsca_events_merge_pileup",
+    "itsca_events_roads_report",
 ]
 
 
@@ -117,6 +118,7 @@ def bind_hip(lib: ctypes.CDLL) -> ctypes.CDLL:
             getattr(lib, f).argtypes = [P]
         lib.itsca_get_counts.argtypes = [P, I, ctypes.POINTER(Counts)]
         lib.itsca_get_capacities.argtypes = [P, ctypes.POINTER(Capacities)]
+        lib.itsca_download_roads.argtypes = [P, P, ctypes.c_size_t, P, ctypes.POINTER(ctypes.c_size_t)]
         lib.itsca_run_noalloc.argtypes = [P, ctypes.POINTER(DeviceEvent), P, I, ctypes.POINTER(I)]
         lib.itsca_download.argtypes = [P, I, I, I, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         lib.itsca_set_profiling.argtypes = [P, I]
@@ -168,6 +170,8 @@ def host_lib() -> ctypes.CDLL:
         lib.itsca_events_load_labels.argtypes = [P, ctypes.c_char_p]
         lib.itsca_events_merge_pileup.argtypes = [P, I]
         lib.itsca_events_merge_pileup.restype = P
+        lib.itsca_events_roads_report.argtypes = [P, I, ctypes.POINTER(ctypes.c_void_p), I32, I,
+                                                  ctypes.POINTER(ctypes.c_long)]
         _host = lib
     return _host
 
@@ -441,6 +445,19 @@ class GpuTracker:
         self._check(self._lib.itsca_get_capacities(self.ctx, ctypes.byref(c)), "itsca_get_capacities")
         return c
 
+    def all_roads(self):
+        """(rows, first): the roads of every work item in one copy, rows of 8
+        int; item i's roads are rows[first[i]:first[i + 1]]."""
+        first = np.zeros(self.num_items() + 1, dtype=np.int32)
+        total = ctypes.c_size_t(0)
+        st = self._lib.itsca_download_roads(self.ctx, None, 0, first.ctypes.data, ctypes.byref(total))
+        rows = np.zeros((max(total.value, 1), 8), dtype=np.int32)
+        if st != 0 and total.value == 0:
+            self._check(st, "itsca_download_roads")
+        self._check(self._lib.itsca_download_roads(self.ctx, rows.ctypes.data, total.value, first.ctypes.data,
+                                                   ctypes.byref(total)), "itsca_download_roads")
+        return rows[:total.value], first
+
     def run_noalloc(self, arrays, layer_offsets, vertex, roads, roads_cap: int) -> int:
         """itsca_run_noalloc: `arrays` are the device addresses of x, y, z,
         cluster ids and MC ids, `roads` the device address of roads_cap rows
@@ -473,6 +490,32 @@ class GpuTracker:
     def roads(self, item: int) -> np.ndarray:
         """(R, 8) int32: cell per layer 0..4, size, label, fake."""
         return self.download(item, ROADS, 0, self.counts(item).roads, 8)
+
+
+def roads_report(events_path: str, labels_path: str, roads) -> List[tuple]:
+    """Correct, duplicate and fake roads of every event, classified like
+    IOUtils::writeRoadsReport (IOUtils.cxx:125-170) against the labels file;
+    roads[e][v] holds the (R, 8) road rows of vertex v of event e."""
+    lib = host_lib()
+    h = lib.itsca_events_load(events_path.encode())
+    if not h:
+        raise ItscaError("cannot read events from %s" % events_path)
+    try:
+        if lib.itsca_events_load_labels(h, labels_path.encode()) != 0:
+            raise ItscaError("cannot read labels from %s" % labels_path)
+        out = []
+        for e, per_vertex in enumerate(roads):
+            rows = [np.ascontiguousarray(r, dtype=np.int32).reshape(-1, 8) for r in per_vertex]
+            ptrs = (ctypes.c_void_p * max(len(rows), 1))(*[r.ctypes.data for r in rows])
+            n = np.array([len(r) for r in rows] or [0], dtype=np.int32)
+            counts = (ctypes.c_long * 3)()
+            if lib.itsca_events_roads_report(h, e, ptrs, n.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(rows),
+                                             counts) != 0:
+                raise ItscaError("no labels for event %d" % e)
+            out.append(tuple(counts))
+        return out
+    finally:
+        lib.itsca_events_free(h)
 
 
 def road_tracks(tracker: GpuTracker, item: int) -> List[tuple]:
