@@ -133,6 +133,7 @@ struct DevState {
   // plans
   int* chunkStart;  // kNumStages * (maxSegs + 1)
   int *chunkSum, *chunkOff;  // kNumStages * maxChunks
+  int2* chunkInfo;           // kNumStages * maxChunks: (segment, first element) of each chunk (kChunkSegs)
   int maxChunks;
   int* status;  // kNumStatus
 };

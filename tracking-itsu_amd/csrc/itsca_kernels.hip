@@ -412,6 +412,44 @@ __device__ void planChunks(const int* counts, int S, int chunk, int* chunkStart,
   if (threadIdx.x == 0) chunkStart[S] = carry;
 }
 
+// The segment of every chunk of a stage and the chunk's first element within
+// it, so that a stage kernel finds both with one load instead of a binary
+// search of dependent loads per chunk.  Grid-wide; launched after the stage's
+// plan.
+__global__ void __launch_bounds__(256) kChunkSegs(DevState s, int stage, int S)
+{
+  const int* chunkStart = s.chunkStart + stage * (s.maxSegs + 1);
+  int2* info = s.chunkInfo + (size_t)stage * s.maxChunks;
+  const int nChunks = chunkStart[S];
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nChunks; c += gridDim.x * blockDim.x) {
+    const int sg = segmentOfChunk(chunkStart, S, c);
+    info[c] = make_int2(sg, (c - chunkStart[sg]) * s.chunk);
+  }
+}
+
+// The chunks of a stage that a workgroup walks (dealt XCD by XCD, as
+// xcdBlockIndex), each with its (segment, first element) from the stage's
+// chunk map; the next chunk's entry is loaded one iteration ahead, so its
+// latency overlaps the current chunk's work.
+struct ChunkWalk {
+  const int2* info;
+  int n, step, chunk;
+  int2 next;
+  __device__ __forceinline__ ChunkWalk(const DevState& s, int stage, int nChunks, int start)
+      : info(s.chunkInfo + (size_t)stage * s.maxChunks), n(nChunks), step(gridDim.x), chunk(start)
+  {
+    next = chunk < n ? info[chunk] : make_int2(0, 0);
+  }
+  __device__ __forceinline__ bool valid() const { return chunk < n; }
+  __device__ __forceinline__ int2 take()
+  {
+    const int2 cur = next;
+    if (chunk + step < n) next = info[chunk + step];
+    return cur;
+  }
+  __device__ __forceinline__ void advance() { chunk += step; }
+};
+
 // Tracklet layer l is computed only while every layer 0..l+1 has clusters
 // (early return, Tracker.cxx:49-52).
 __global__ void __launch_bounds__(1024) kPlanTracklets(DevState s, int* counts)
@@ -781,13 +819,15 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) kTr
   const int S = s.W * kTrackletLayers;
   const int* chunkStart = s.chunkStart + kStTracklets * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
-  for (int chunk = xcdBlockIndex(); chunk < nChunks; chunk += gridDim.x) {
-    const int sg = segmentOfChunk(chunkStart, S, chunk);
+  for (ChunkWalk cw(s, kStTracklets, nChunks, xcdBlockIndex()); cw.valid(); cw.advance()) {
+    const int chunk = cw.chunk;
+    const int2 ci = cw.take();
+    const int sg = ci.x;
     const int w = sg / kTrackletLayers, l = sg % kTrackletLayers;
     const Seg cur = s.clSeg[w * kLayers + l], next = s.clSeg[w * kLayers + l + 1];
     const int rw = s.recItem[w];
     const Seg rcur = s.clSeg[rw * kLayers + l], rnext = s.clSeg[rw * kLayers + l + 1];
-    const int first = (chunk - chunkStart[sg]) * B;
+    const int first = ci.y;
     const int i = first + threadIdx.x;
     const int p = cur.base + i;
     const int* table = s.table + (size_t)(rw * kLayers + l + 1) * kTableSize;
@@ -953,11 +993,13 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) kTr
   const int S = s.W * kTrackletLayers;
   const int* chunkStart = s.chunkStart + kStTracklets * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
-  for (int chunk = xcdBlockIndex(); chunk < nChunks; chunk += gridDim.x) {
-    const int sg = segmentOfChunk(chunkStart, S, chunk);
+  for (ChunkWalk cw(s, kStTracklets, nChunks, xcdBlockIndex()); cw.valid(); cw.advance()) {
+    const int chunk = cw.chunk;
+    const int2 ci = cw.take();
+    const int sg = ci.x;
     const int w = sg / kTrackletLayers, l = sg % kTrackletLayers;
     const Seg cur = s.clSeg[w * kLayers + l], next = s.clSeg[w * kLayers + l + 1];
-    const int first = (chunk - chunkStart[sg]) * B;
+    const int first = ci.y;
     const int i = first + threadIdx.x;
     const int p = cur.base + i;
     const int cnt = i < cur.count ? s.trkCnt[p] : 0;
@@ -1126,11 +1168,13 @@ __global__ void __launch_bounds__(B) kCellsCount(DevState s)
   const int S = s.W * kCellLayers;
   const int* chunkStart = s.chunkStart + kStCells * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
-  for (int chunk = xcdBlockIndex(); chunk < nChunks; chunk += gridDim.x) {
-    const int sg = segmentOfChunk(chunkStart, S, chunk);
+  for (ChunkWalk cw(s, kStCells, nChunks, xcdBlockIndex()); cw.valid(); cw.advance()) {
+    const int chunk = cw.chunk;
+    const int2 ci = cw.take();
+    const int sg = ci.x;
     const int w = sg / kCellLayers, l = sg % kCellLayers;
     const Seg tl = s.trkSeg[w * kTrackletLayers + l];
-    const int it = (chunk - chunkStart[sg]) * B + threadIdx.x;
+    const int it = ci.y + threadIdx.x;
     int cnt = 0;
     if (it < tl.count) {
       const int t = tl.base + it;
@@ -1225,11 +1269,13 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(8))) kCe
   const int S = s.W * kCellLayers;
   const int* chunkStart = s.chunkStart + kStCells * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
-  for (int chunk = xcdBlockIndex(); chunk < nChunks; chunk += gridDim.x) {
-    const int sg = segmentOfChunk(chunkStart, S, chunk);
+  for (ChunkWalk cw(s, kStCells, nChunks, xcdBlockIndex()); cw.valid(); cw.advance()) {
+    const int chunk = cw.chunk;
+    const int2 ci = cw.take();
+    const int sg = ci.x;
     const int w = sg / kCellLayers, l = sg % kCellLayers;
     const Seg tl = s.trkSeg[w * kTrackletLayers + l];
-    const int firstT = (chunk - chunkStart[sg]) * B;
+    const int firstT = ci.y;
     const int it = firstT + threadIdx.x;
     const int t = tl.base + it;
     const int cnt = it < tl.count ? s.cellCnt[t] : 0;
@@ -1316,11 +1362,13 @@ __global__ void __launch_bounds__(B) kEdgesCount(DevState s)
   const int S = s.W * (kCellLayers - 1);
   const int* chunkStart = s.chunkStart + kStEdges * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
-  for (int chunk = xcdBlockIndex(); chunk < nChunks; chunk += gridDim.x) {
-    const int sg = segmentOfChunk(chunkStart, S, chunk);
+  for (ChunkWalk cw(s, kStEdges, nChunks, xcdBlockIndex()); cw.valid(); cw.advance()) {
+    const int chunk = cw.chunk;
+    const int2 ci = cw.take();
+    const int sg = ci.x;
     const int w = sg / (kCellLayers - 1), l = sg % (kCellLayers - 1);
     const Seg cl = s.celSeg[w * kCellLayers + l];
-    const int ic = (chunk - chunkStart[sg]) * B + threadIdx.x;
+    const int ic = ci.y + threadIdx.x;
     int cnt = 0;
     if (ic < cl.count) {
       const int c = cl.base + ic;
@@ -1345,11 +1393,13 @@ __global__ void __launch_bounds__(B) kEdgesFill(DevState s)
   const int S = s.W * (kCellLayers - 1);
   const int* chunkStart = s.chunkStart + kStEdges * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
-  for (int chunk = xcdBlockIndex(); chunk < nChunks; chunk += gridDim.x) {
-    const int sg = segmentOfChunk(chunkStart, S, chunk);
+  for (ChunkWalk cw(s, kStEdges, nChunks, xcdBlockIndex()); cw.valid(); cw.advance()) {
+    const int chunk = cw.chunk;
+    const int2 ci = cw.take();
+    const int sg = ci.x;
     const int w = sg / (kCellLayers - 1), l = sg % (kCellLayers - 1);
     const Seg cl = s.celSeg[w * kCellLayers + l];
-    const int firstC = (chunk - chunkStart[sg]) * B;
+    const int firstC = ci.y;
     const int ic = firstC + threadIdx.x;
     const int c = cl.base + ic;
     const int cnt = ic < cl.count ? s.succCnt[c] : 0;
@@ -1465,11 +1515,13 @@ __global__ void __launch_bounds__(B) kLevels(DevState s, int layer, int stage)
   const int S = s.W;
   const int* chunkStart = s.chunkStart + stage * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
-  for (int chunk = xcdBlockIndex(); chunk < nChunks; chunk += gridDim.x) {
-    const int w = segmentOfChunk(chunkStart, S, chunk);
+  for (ChunkWalk cw(s, stage, nChunks, xcdBlockIndex()); cw.valid(); cw.advance()) {
+    const int chunk = cw.chunk;
+    const int2 ci = cw.take();
+    const int w = ci.x;
     const Seg cn = s.celSeg[w * kCellLayers + layer];
     const Seg cp = s.celSeg[w * kCellLayers + layer - 1];
-    const int in = (chunk - chunkStart[w]) * B + threadIdx.x;
+    const int in = ci.y + threadIdx.x;
     int roots = 0;
     if (in < cn.count) {
       const int n = cn.base + in;
@@ -1524,10 +1576,12 @@ __global__ void __launch_bounds__(B) kRoadsFill(DevState s)
   const int S = s.W;
   const int* chunkStart = s.chunkStart + kStRoads * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
-  for (int chunk = xcdBlockIndex(); chunk < nChunks; chunk += gridDim.x) {
-    const int w = segmentOfChunk(chunkStart, S, chunk);
+  for (ChunkWalk cw(s, kStRoads, nChunks, xcdBlockIndex()); cw.valid(); cw.advance()) {
+    const int chunk = cw.chunk;
+    const int2 ci = cw.take();
+    const int w = ci.x;
     const Seg c4 = s.celSeg[w * kCellLayers + L4];
-    const int first = (chunk - chunkStart[w]) * B;
+    const int first = ci.y;
     const int in = first + threadIdx.x;
     int cnt = 0;
     if (in < c4.count && s.clevel[c4.base + in] == kCellsMinLevel) cnt = (int)min(s.cpaths[c4.base + in], kMaxRootRoads);
@@ -1638,6 +1692,8 @@ void launchRoadsOut(const DevState& s, int item, int* rows, int cap, hipStream_t
 // ---------------------------------------------------------------------------
 // Launch sequence of one plain-CA run over the whole batch.
 // ---------------------------------------------------------------------------
+constexpr int kSegBlocks = 512;  // workgroups of kChunkSegs
+
 template <int B>
 void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t stream, KernelTimer* timer)
 {
@@ -1653,6 +1709,7 @@ void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t str
     hipLaunchKernelGGL(kPrep<32>, dim3(s.W * kLayers), dim3(kPrepThreads), 0, stream, s);
   done("kPrep", 0);
   hipLaunchKernelGGL(kPlanTracklets, dim3(1), dim3(1024), 0, stream, s, counts);
+  hipLaunchKernelGGL(kChunkSegs, dim3(kSegBlocks), dim3(256), 0, stream, s, (int)kStTracklets, s.W * kTrackletLayers);
   done("kPlanTracklets", 1);
   if (s.zSub > 1)
     hipLaunchKernelGGL((kTrackletsCount<B, true>), dim3(G), dim3(B), 0, stream, s);
@@ -1669,6 +1726,7 @@ void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t str
   hipLaunchKernelGGL(kTrackletsFill<B>, dim3(G), dim3(B), 0, stream, s);
   done("kTrackletsFill", 1);
   hipLaunchKernelGGL(kPlanCells, dim3(1), dim3(1024), 0, stream, s, counts);
+  hipLaunchKernelGGL(kChunkSegs, dim3(kSegBlocks), dim3(256), 0, stream, s, (int)kStCells, s.W * kCellLayers);
   done("kPlanCells", 2);
   hipLaunchKernelGGL(kCellsCount<B>, dim3(G), dim3(B), 0, stream, s);
   done("kCellsCount", 2);
@@ -1678,6 +1736,7 @@ void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t str
   hipLaunchKernelGGL(kCellsFill<B>, dim3(G), dim3(B), 0, stream, s);
   done("kCellsFill", 2);
   hipLaunchKernelGGL(kPlanEdges, dim3(1), dim3(1024), 0, stream, s, counts);
+  hipLaunchKernelGGL(kChunkSegs, dim3(kSegBlocks), dim3(256), 0, stream, s, (int)kStEdges, s.W * (kCellLayers - 1));
   done("kPlanEdges", 3);
   hipLaunchKernelGGL(kEdgesCount<B>, dim3(G), dim3(B), 0, stream, s);
   done("kEdgesCount", 3);
@@ -1693,6 +1752,7 @@ void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t str
     // writes each chunk's number of roads.
     const int stage = layer == kCellLayers - 1 ? (int)kStRoads : kStLevels + layer - 1;
     hipLaunchKernelGGL(kPlanLayer, dim3(1), dim3(1024), 0, stream, s, counts, layer, stage);
+    hipLaunchKernelGGL(kChunkSegs, dim3(kSegBlocks), dim3(256), 0, stream, s, stage, s.W);
     done("kPlanLayer", 4);
     hipLaunchKernelGGL(kLevels<B>, dim3(G), dim3(B), 0, stream, s, layer, stage);
     done("kLevels", 4);

@@ -206,20 +206,6 @@ __device__ __forceinline__ void featuresOf(const DevState& s, const TrackletSeed
                     l + 1 < kTrackletLayers ? s.trkCnt[cj] : 0, f);
 }
 
-// Segment (item, layer) of a chunk: the largest sg with chunkStart[sg] <= chunk.
-__device__ __forceinline__ int segmentOf(const int* chunkStart, int S, int chunk)
-{
-  int lo = 0, hi = S;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (chunkStart[mid] <= chunk)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
 // ---------------------------------------------------------------------------
 // Pipeline filter.  Runs after the tracklet count.  Every wavefront handles
 // the 64 clusters of its lanes on its own (wavefront barriers only): a
@@ -261,12 +247,16 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) kTr
   const int S = s.W * kTrackletLayers;
   const int* chunkStart = s.chunkStart + kStTracklets * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
+  const int2* chunkInfo = s.chunkInfo + (size_t)kStTracklets * s.maxChunks;
+  int2 nextInfo = (int)blockIdx.x < nChunks ? chunkInfo[blockIdx.x] : make_int2(0, 0);
   for (int chunk = blockIdx.x; chunk < nChunks; chunk += gridDim.x) {
-    const int sg = segmentOf(chunkStart, S, chunk);
+    const int2 ci = nextInfo;  // this chunk's (segment, first cluster); the next one's is loaded ahead
+    if (chunk + (int)gridDim.x < nChunks) nextInfo = chunkInfo[chunk + gridDim.x];
+    const int sg = ci.x;
     const int w = sg / kTrackletLayers, l = sg % kTrackletLayers;
     const Seg cur = s.clSeg[w * kLayers + l], next = s.clSeg[w * kLayers + l + 1];
     const int recCur = recSeg(s, w, l).base, recNext = recSeg(s, w, l + 1).base;
-    const int first = (chunk - chunkStart[sg]) * B;
+    const int first = ci.y;
     const int i = first + threadIdx.x;
     const int p = cur.base + i;
     const int raw = i < cur.count ? s.trkCnt[p] : 0;

@@ -172,6 +172,7 @@ struct itsca_ctx {
   DBuf<uint32_t> cpaths;
   DBuf<int> pred, rCells, rLabel, rFake;
   DBuf<int> chunkStart, chunkSum, chunkOff, planScratch, dStatus;
+  DBuf<int2> chunkInfo;
 
   void dropGraph()
   {
@@ -288,6 +289,7 @@ struct itsca_ctx {
     HIP_TRY(chunkStart.reserve((size_t)kNumStages * (maxSegs + 1)));
     HIP_TRY(chunkSum.reserve((size_t)kNumStages * maxChunks));
     HIP_TRY(chunkOff.reserve((size_t)kNumStages * maxChunks));
+    HIP_TRY(chunkInfo.reserve((size_t)kNumStages * maxChunks));
     HIP_TRY(planScratch.reserve(maxSegs));
     HIP_TRY(dStatus.reserve(kNumStatus));
   }
@@ -412,6 +414,7 @@ struct itsca_ctx {
     s.chunkStart = chunkStart.p;
     s.chunkSum = chunkSum.p;
     s.chunkOff = chunkOff.p;
+    s.chunkInfo = chunkInfo.p;
     s.maxChunks = maxChunks;
     s.status = dStatus.p;
     return s;
@@ -822,7 +825,7 @@ bool carveNoAlloc(unsigned char* base, size_t bytes, int n, DevState& s, int*& p
   const size_t perTracklet = 8 + 8 + 4 + 4 + 4 * kCellSlots;
   const size_t perCell = 6 * 4 + 16 + 4 + 3 * 4 + 4 + 4 * kEdgeSlots;
   const size_t perEdge = 4 * 4, perRoad = 4 * kCellLayers + 4 + 4;
-  const size_t perUnit = 10 * perTracklet + perCell + perEdge + perRoad + 2 * 4 * kNumStages * 10 / kMinChunk + 64;
+  const size_t perUnit = 10 * perTracklet + perCell + perEdge + perRoad + 4 * 4 * kNumStages * 10 / kMinChunk + 64;
   const size_t left = bytes - off;
   const size_t k = std::min<size_t>(left / perUnit, (size_t)(kMaxCapacity / 10));
   if (k < 64) return false;
@@ -832,6 +835,7 @@ bool carveNoAlloc(unsigned char* base, size_t bytes, int n, DevState& s, int*& p
   s.maxChunks = (int)((std::max<size_t>(N, 10 * k) + kMinChunk - 1) / kMinChunk) + 8 + 1;
   s.chunkSum = static_cast<int*>(take((size_t)kNumStages * s.maxChunks, 4));
   s.chunkOff = static_cast<int*>(take((size_t)kNumStages * s.maxChunks, 4));
+  s.chunkInfo = static_cast<int2*>(take((size_t)kNumStages * s.maxChunks, sizeof(int2)));
   s.tFS = static_cast<int2*>(take(s.tCap, sizeof(int2)));
   s.tTP = static_cast<float2*>(take(s.tCap, sizeof(float2)));
   s.cellCnt = static_cast<int*>(take(s.tCap, 4));
