@@ -126,6 +126,9 @@ struct float4 {
 struct int2 {
   int x, y;
 };
+struct int4 {
+  int x, y, z, w;
+};
 struct uint2 {
   unsigned x, y;
 };
@@ -134,6 +137,7 @@ struct uint4 {
 };
 inline float2 make_float2(float a, float b) { return float2{a, b}; }
 inline int2 make_int2(int a, int b) { return int2{a, b}; }
+inline int4 make_int4(int a, int b, int c, int d) { return int4{a, b, c, d}; }
 inline uint2 make_uint2(unsigned a, unsigned b) { return uint2{a, b}; }
 inline uint4 make_uint4(unsigned a, unsigned b, unsigned c, unsigned d) { return uint4{a, b, c, d}; }
 inline float4 make_float4(float a, float b, float c, float d) { return float4{a, b, c, d}; }

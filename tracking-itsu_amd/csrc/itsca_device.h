@@ -133,7 +133,7 @@ struct DevState {
   // plans
   int* chunkStart;  // kNumStages * (maxSegs + 1)
   int *chunkSum, *chunkOff;  // kNumStages * maxChunks
-  int2* chunkInfo;           // kNumStages * maxChunks: (segment, first element) of each chunk (kChunkSegs)
+  int4* chunkInfo;           // kNumStages * maxChunks: (segment, first element, its global index, elements) per chunk
   int maxChunks;
   int* status;  // kNumStatus
 };

@@ -10,6 +10,7 @@
 // so each float expression rounds as the reference's x86-64 SSE build does.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "itsca_device.h"
@@ -412,41 +413,61 @@ __device__ void planChunks(const int* counts, int S, int chunk, int* chunkStart,
   if (threadIdx.x == 0) chunkStart[S] = carry;
 }
 
-// The segment of every chunk of a stage and the chunk's first element within
-// it, so that a stage kernel finds both with one load instead of a binary
-// search of dependent loads per chunk.  Grid-wide; launched after the stage's
-// plan.
-__global__ void __launch_bounds__(256) kChunkSegs(DevState s, int stage, int S)
+// Per chunk of a stage: its segment, its first element within the segment,
+// the global index of that element and the number of elements it holds, so
+// that a stage kernel finds them with one load instead of a binary search of
+// dependent loads per chunk.  Grid-wide; launched after the stage's plan,
+// when the segments of the stage's elements are final.  elements: where the
+// element segment of segment sg is -- 0: clusters of (item, layer) with
+// sg = w*6+l; 1: tracklets, sg = w*5+l; 2: cells, sg = w*4+l; 3: cells of
+// cell layer `layer`, sg = w.
+__global__ void __launch_bounds__(256) kChunkSegs(DevState s, int stage, int S, int elements, int layer)
 {
   const int* chunkStart = s.chunkStart + stage * (s.maxSegs + 1);
-  int2* info = s.chunkInfo + (size_t)stage * s.maxChunks;
+  int4* info = s.chunkInfo + (size_t)stage * s.maxChunks;
   const int nChunks = chunkStart[S];
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nChunks; c += gridDim.x * blockDim.x) {
     const int sg = segmentOfChunk(chunkStart, S, c);
-    info[c] = make_int2(sg, (c - chunkStart[sg]) * s.chunk);
+    const int first = (c - chunkStart[sg]) * s.chunk;
+    Seg e;
+    if (elements == 0)
+      e = s.clSeg[(sg / kTrackletLayers) * kLayers + sg % kTrackletLayers];
+    else if (elements == 1)
+      e = s.trkSeg[(sg / kCellLayers) * kTrackletLayers + sg % kCellLayers];
+    else if (elements == 2)
+      e = s.celSeg[(sg / (kCellLayers - 1)) * kCellLayers + sg % (kCellLayers - 1)];
+    else
+      e = s.celSeg[sg * kCellLayers + layer];
+    info[c] = make_int4(sg, first, e.base + first, min(s.chunk, e.count - first));
   }
 }
 
 // The chunks of a stage that a workgroup walks (dealt XCD by XCD, as
-// xcdBlockIndex), each with its (segment, first element) from the stage's
-// chunk map; the next chunk's entry is loaded one iteration ahead, so its
-// latency overlaps the current chunk's work.
+// xcdBlockIndex), each with its entry (segment, first element, global index
+// of the first element, elements) from the stage's chunk map.  Entries are
+// loaded two chunks ahead; after take(), next() is the following chunk's
+// entry, so a kernel can issue that chunk's first loads while it works on the
+// current one.
 struct ChunkWalk {
-  const int2* info;
+  const int4* info;
   int n, step, chunk;
-  int2 next;
+  int4 cur, nxt;
   __device__ __forceinline__ ChunkWalk(const DevState& s, int stage, int nChunks, int start)
       : info(s.chunkInfo + (size_t)stage * s.maxChunks), n(nChunks), step(gridDim.x), chunk(start)
   {
-    next = chunk < n ? info[chunk] : make_int2(0, 0);
+    cur = chunk < n ? info[chunk] : make_int4(0, 0, 0, 0);
+    nxt = chunk + step < n ? info[chunk + step] : make_int4(0, 0, 0, 0);
   }
   __device__ __forceinline__ bool valid() const { return chunk < n; }
-  __device__ __forceinline__ int2 take()
+  __device__ __forceinline__ int4 take()
   {
-    const int2 cur = next;
-    if (chunk + step < n) next = info[chunk + step];
-    return cur;
+    const int4 c = cur;
+    cur = nxt;
+    nxt = chunk + 2 * step < n ? info[chunk + 2 * step] : make_int4(0, 0, 0, 0);
+    return c;
   }
+  __device__ __forceinline__ bool hasNext() const { return chunk + step < n; }
+  __device__ __forceinline__ const int4& next() const { return cur; }
   __device__ __forceinline__ void advance() { chunk += step; }
 };
 
@@ -821,7 +842,7 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) kTr
   const int nChunks = chunkStart[S];
   for (ChunkWalk cw(s, kStTracklets, nChunks, xcdBlockIndex()); cw.valid(); cw.advance()) {
     const int chunk = cw.chunk;
-    const int2 ci = cw.take();
+    const int4 ci = cw.take();
     const int sg = ci.x;
     const int w = sg / kTrackletLayers, l = sg % kTrackletLayers;
     const Seg cur = s.clSeg[w * kLayers + l], next = s.clSeg[w * kLayers + l + 1];
@@ -993,16 +1014,21 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) kTr
   const int S = s.W * kTrackletLayers;
   const int* chunkStart = s.chunkStart + kStTracklets * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
-  for (ChunkWalk cw(s, kStTracklets, nChunks, xcdBlockIndex()); cw.valid(); cw.advance()) {
+  ChunkWalk cw(s, kStTracklets, nChunks, xcdBlockIndex());
+  // tracklet count of this thread's cluster in a chunk; the next chunk's is loaded ahead
+  auto elemOf = [&](const int4& e) { return (int)threadIdx.x < e.w ? s.trkCnt[e.z + threadIdx.x] : 0; };
+  int cntNext = cw.valid() ? elemOf(cw.next()) : int{};
+  for (; cw.valid(); cw.advance()) {
     const int chunk = cw.chunk;
-    const int2 ci = cw.take();
+    const int4 ci = cw.take();
+    const int cnt = cntNext;
+    if (cw.hasNext()) cntNext = elemOf(cw.next());
     const int sg = ci.x;
     const int w = sg / kTrackletLayers, l = sg % kTrackletLayers;
     const Seg cur = s.clSeg[w * kLayers + l], next = s.clSeg[w * kLayers + l + 1];
     const int first = ci.y;
     const int i = first + threadIdx.x;
     const int p = cur.base + i;
-    const int cnt = i < cur.count ? s.trkCnt[p] : 0;
     const int capped = cnt <= kTrkSlots ? cnt : 0;
     int total, totalCap;
     const int ex = blockExclusiveScan(cnt, scratch, total);
@@ -1059,7 +1085,15 @@ struct CellSeed {
   float x1, y1, z1, r1, r1sq, x2, y2, r2sq, d1x, d1y, d1z, vx, vy, vz, tTanL, tPhi;
 };
 
+__device__ __forceinline__ CellSeed cellSeedOf(const DevState& s, int w, int l, int2 fs, float2 tp);
+
 __device__ __forceinline__ CellSeed cellSeed(const DevState& s, int w, int l, int t)
+{
+  return cellSeedOf(s, w, l, s.tFS[t], s.tTP[t]);
+}
+
+// The same from the tracklet's record (tFS[t], tTP[t]) already loaded.
+__device__ __forceinline__ CellSeed cellSeedOf(const DevState& s, int w, int l, int2 fs, float2 tp)
 {
   CellSeed c;
   c.l = l;
@@ -1067,7 +1101,6 @@ __device__ __forceinline__ CellSeed cellSeed(const DevState& s, int w, int l, in
   const Seg cl0 = recSeg(s, w, l), cl1 = recSeg(s, w, l + 1);  // records
   c.cl2Base = recSeg(s, w, l + 2).base;
   const int item1 = s.clSeg[w * kLayers + l + 1].base;  // the item's own counts and offsets
-  const int2 fs = s.tFS[t];
   const int k = fs.y;
   c.nCand = s.trkCnt[item1 + k];  // tracklets of layer l+1 starting at this cluster
   c.uFirst = c.nCand > 0 ? s.trkOff[item1 + k] : 0;
@@ -1088,7 +1121,6 @@ __device__ __forceinline__ CellSeed cellSeed(const DevState& s, int w, int l, in
   c.d1x = c.x2 - c.x1;
   c.d1y = c.y2 - c.y1;
   c.d1z = c.r2sq - c.r1sq;
-  const float2 tp = s.tTP[t];
   c.tTanL = tp.x;
   c.tPhi = tp.y;
   return c;
@@ -1168,9 +1200,26 @@ __global__ void __launch_bounds__(B) kCellsCount(DevState s)
   const int S = s.W * kCellLayers;
   const int* chunkStart = s.chunkStart + kStCells * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
-  for (ChunkWalk cw(s, kStCells, nChunks, xcdBlockIndex()); cw.valid(); cw.advance()) {
+  ChunkWalk cw(s, kStCells, nChunks, xcdBlockIndex());
+  // the record of this thread's tracklet in a chunk; the next chunk's is loaded ahead
+  struct Rec {
+    int2 fs;
+    float2 tp;
+  };
+  auto recOf = [&](const int4& e) {
+    Rec r{make_int2(0, 0), make_float2(0.0f, 0.0f)};
+    if ((int)threadIdx.x < e.w) {
+      r.fs = s.tFS[e.z + threadIdx.x];
+      r.tp = s.tTP[e.z + threadIdx.x];
+    }
+    return r;
+  };
+  Rec recNext = cw.valid() ? recOf(cw.next()) : Rec{make_int2(0, 0), make_float2(0.0f, 0.0f)};
+  for (; cw.valid(); cw.advance()) {
     const int chunk = cw.chunk;
-    const int2 ci = cw.take();
+    const int4 ci = cw.take();
+    const Rec rec = recNext;
+    if (cw.hasNext()) recNext = recOf(cw.next());
     const int sg = ci.x;
     const int w = sg / kCellLayers, l = sg % kCellLayers;
     const Seg tl = s.trkSeg[w * kTrackletLayers + l];
@@ -1178,7 +1227,7 @@ __global__ void __launch_bounds__(B) kCellsCount(DevState s)
     int cnt = 0;
     if (it < tl.count) {
       const int t = tl.base + it;
-      const CellSeed c = cellSeed(s, w, l, t);
+      const CellSeed c = cellSeedOf(s, w, l, rec.fs, rec.tp);
       int* slots = s.cellCand + (size_t)t * kCellSlots;
       int m = 0;
       const int uEnd = c.uFirst + c.nCand;
@@ -1269,16 +1318,22 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(8))) kCe
   const int S = s.W * kCellLayers;
   const int* chunkStart = s.chunkStart + kStCells * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
-  for (ChunkWalk cw(s, kStCells, nChunks, xcdBlockIndex()); cw.valid(); cw.advance()) {
+  ChunkWalk cw(s, kStCells, nChunks, xcdBlockIndex());
+  // cell count of this thread's tracklet in a chunk (0 past its end); the next
+  // chunk's is loaded while the current one is filled
+  auto countOf = [&](const int4& e) { return (int)threadIdx.x < e.w ? s.cellCnt[e.z + threadIdx.x] : 0; };
+  int cntNext = cw.valid() ? countOf(cw.next()) : 0;
+  for (; cw.valid(); cw.advance()) {
     const int chunk = cw.chunk;
-    const int2 ci = cw.take();
+    const int4 ci = cw.take();
+    const int cnt = cntNext;
+    if (cw.hasNext()) cntNext = countOf(cw.next());
     const int sg = ci.x;
     const int w = sg / kCellLayers, l = sg % kCellLayers;
     const Seg tl = s.trkSeg[w * kTrackletLayers + l];
     const int firstT = ci.y;
     const int it = firstT + threadIdx.x;
     const int t = tl.base + it;
-    const int cnt = it < tl.count ? s.cellCnt[t] : 0;
     const int capped = cnt <= kCellSlots ? cnt : 0;
     int total, totalCap;
     const int ex = blockExclusiveScan(cnt, scratch, total);
@@ -1364,7 +1419,7 @@ __global__ void __launch_bounds__(B) kEdgesCount(DevState s)
   const int nChunks = chunkStart[S];
   for (ChunkWalk cw(s, kStEdges, nChunks, xcdBlockIndex()); cw.valid(); cw.advance()) {
     const int chunk = cw.chunk;
-    const int2 ci = cw.take();
+    const int4 ci = cw.take();
     const int sg = ci.x;
     const int w = sg / (kCellLayers - 1), l = sg % (kCellLayers - 1);
     const Seg cl = s.celSeg[w * kCellLayers + l];
@@ -1393,16 +1448,21 @@ __global__ void __launch_bounds__(B) kEdgesFill(DevState s)
   const int S = s.W * (kCellLayers - 1);
   const int* chunkStart = s.chunkStart + kStEdges * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
-  for (ChunkWalk cw(s, kStEdges, nChunks, xcdBlockIndex()); cw.valid(); cw.advance()) {
+  ChunkWalk cw(s, kStEdges, nChunks, xcdBlockIndex());
+  // successor count of this thread's cell in a chunk; the next chunk's is loaded ahead
+  auto elemOf = [&](const int4& e) { return (int)threadIdx.x < e.w ? s.succCnt[e.z + threadIdx.x] : 0; };
+  int cntNext = cw.valid() ? elemOf(cw.next()) : int{};
+  for (; cw.valid(); cw.advance()) {
     const int chunk = cw.chunk;
-    const int2 ci = cw.take();
+    const int4 ci = cw.take();
+    const int cnt = cntNext;
+    if (cw.hasNext()) cntNext = elemOf(cw.next());
     const int sg = ci.x;
     const int w = sg / (kCellLayers - 1), l = sg % (kCellLayers - 1);
     const Seg cl = s.celSeg[w * kCellLayers + l];
     const int firstC = ci.y;
     const int ic = firstC + threadIdx.x;
     const int c = cl.base + ic;
-    const int cnt = ic < cl.count ? s.succCnt[c] : 0;
     const int capped = cnt <= kEdgeSlots ? cnt : 0;
     int total, totalCap;
     const int ex = blockExclusiveScan(cnt, scratch, total);
@@ -1515,9 +1575,18 @@ __global__ void __launch_bounds__(B) kLevels(DevState s, int layer, int stage)
   const int S = s.W;
   const int* chunkStart = s.chunkStart + stage * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
-  for (ChunkWalk cw(s, stage, nChunks, xcdBlockIndex()); cw.valid(); cw.advance()) {
+  ChunkWalk cw(s, stage, nChunks, xcdBlockIndex());
+  // predecessor count and offset of this thread's cell in a chunk; the next chunk's are loaded ahead
+  auto predOf = [&](const int4& e) {
+    return (int)threadIdx.x < e.w ? make_int2(s.edgeCnt[e.z + threadIdx.x], s.edgeOff[e.z + threadIdx.x])
+                                  : make_int2(0, 0);
+  };
+  int2 predNext = cw.valid() ? predOf(cw.next()) : make_int2(0, 0);
+  for (; cw.valid(); cw.advance()) {
     const int chunk = cw.chunk;
-    const int2 ci = cw.take();
+    const int4 ci = cw.take();
+    const int2 pr = predNext;
+    if (cw.hasNext()) predNext = predOf(cw.next());
     const int w = ci.x;
     const Seg cn = s.celSeg[w * kCellLayers + layer];
     const Seg cp = s.celSeg[w * kCellLayers + layer - 1];
@@ -1525,11 +1594,11 @@ __global__ void __launch_bounds__(B) kLevels(DevState s, int layer, int stage)
     int roots = 0;
     if (in < cn.count) {
       const int n = cn.base + in;
-      const int cnt = s.edgeCnt[n];
+      const int cnt = pr.x;
       int level = 1;
       uint32_t paths = 1;
       if (cnt > 0) {
-        const int off = s.edgSeg[w * (kCellLayers - 1) + layer - 1].base + s.edgeOff[n];
+        const int off = s.edgSeg[w * (kCellLayers - 1) + layer - 1].base + pr.y;
         int best = 0;
         for (int a = 0; a < cnt; ++a) best = max(best, s.clevel[cp.base + s.pred[off + a]]);
         level = best + 1;
@@ -1578,7 +1647,7 @@ __global__ void __launch_bounds__(B) kRoadsFill(DevState s)
   const int nChunks = chunkStart[S];
   for (ChunkWalk cw(s, kStRoads, nChunks, xcdBlockIndex()); cw.valid(); cw.advance()) {
     const int chunk = cw.chunk;
-    const int2 ci = cw.take();
+    const int4 ci = cw.take();
     const int w = ci.x;
     const Seg c4 = s.celSeg[w * kCellLayers + L4];
     const int first = ci.y;
@@ -1692,11 +1761,15 @@ void launchRoadsOut(const DevState& s, int item, int* rows, int cap, hipStream_t
 // ---------------------------------------------------------------------------
 // Launch sequence of one plain-CA run over the whole batch.
 // ---------------------------------------------------------------------------
-constexpr int kSegBlocks = 512;  // workgroups of kChunkSegs
-
 template <int B>
 void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t stream, KernelTimer* timer)
 {
+  // workgroups of kChunkSegs for a stage of at most `elements` elements: one
+  // thread per chunk, at most 512 workgroups
+  auto segBlocks = [&](long elements) {
+    const long chunks = elements / B + (long)s.W * kLayers + 1;
+    return (unsigned)std::min<long>(512, (chunks + 255) / 256);
+  };
   const int G = cfg.persistentBlocks;
   int* counts = cfg.planScratch;
   auto done = [&](const char* name, int stage) {
@@ -1709,7 +1782,8 @@ void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t str
     hipLaunchKernelGGL(kPrep<32>, dim3(s.W * kLayers), dim3(kPrepThreads), 0, stream, s);
   done("kPrep", 0);
   hipLaunchKernelGGL(kPlanTracklets, dim3(1), dim3(1024), 0, stream, s, counts);
-  hipLaunchKernelGGL(kChunkSegs, dim3(kSegBlocks), dim3(256), 0, stream, s, (int)kStTracklets, s.W * kTrackletLayers);
+  hipLaunchKernelGGL(kChunkSegs, dim3(segBlocks((long)s.W * kTrackletLayers * cfg.maxLayerClusters)), dim3(256), 0, stream, s,
+                     (int)kStTracklets, s.W * kTrackletLayers, 0, 0);
   done("kPlanTracklets", 1);
   if (s.zSub > 1)
     hipLaunchKernelGGL((kTrackletsCount<B, true>), dim3(G), dim3(B), 0, stream, s);
@@ -1726,7 +1800,7 @@ void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t str
   hipLaunchKernelGGL(kTrackletsFill<B>, dim3(G), dim3(B), 0, stream, s);
   done("kTrackletsFill", 1);
   hipLaunchKernelGGL(kPlanCells, dim3(1), dim3(1024), 0, stream, s, counts);
-  hipLaunchKernelGGL(kChunkSegs, dim3(kSegBlocks), dim3(256), 0, stream, s, (int)kStCells, s.W * kCellLayers);
+  hipLaunchKernelGGL(kChunkSegs, dim3(segBlocks(s.tCap)), dim3(256), 0, stream, s, (int)kStCells, s.W * kCellLayers, 1, 0);
   done("kPlanCells", 2);
   hipLaunchKernelGGL(kCellsCount<B>, dim3(G), dim3(B), 0, stream, s);
   done("kCellsCount", 2);
@@ -1736,7 +1810,8 @@ void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t str
   hipLaunchKernelGGL(kCellsFill<B>, dim3(G), dim3(B), 0, stream, s);
   done("kCellsFill", 2);
   hipLaunchKernelGGL(kPlanEdges, dim3(1), dim3(1024), 0, stream, s, counts);
-  hipLaunchKernelGGL(kChunkSegs, dim3(kSegBlocks), dim3(256), 0, stream, s, (int)kStEdges, s.W * (kCellLayers - 1));
+  hipLaunchKernelGGL(kChunkSegs, dim3(segBlocks(s.cCap)), dim3(256), 0, stream, s, (int)kStEdges, s.W * (kCellLayers - 1), 2,
+                     0);
   done("kPlanEdges", 3);
   hipLaunchKernelGGL(kEdgesCount<B>, dim3(G), dim3(B), 0, stream, s);
   done("kEdgesCount", 3);
@@ -1752,7 +1827,7 @@ void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t str
     // writes each chunk's number of roads.
     const int stage = layer == kCellLayers - 1 ? (int)kStRoads : kStLevels + layer - 1;
     hipLaunchKernelGGL(kPlanLayer, dim3(1), dim3(1024), 0, stream, s, counts, layer, stage);
-    hipLaunchKernelGGL(kChunkSegs, dim3(kSegBlocks), dim3(256), 0, stream, s, stage, s.W);
+    hipLaunchKernelGGL(kChunkSegs, dim3(segBlocks(s.cCap)), dim3(256), 0, stream, s, stage, s.W, 3, layer);
     done("kPlanLayer", 4);
     hipLaunchKernelGGL(kLevels<B>, dim3(G), dim3(B), 0, stream, s, layer, stage);
     done("kLevels", 4);

@@ -247,11 +247,22 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) kTr
   const int S = s.W * kTrackletLayers;
   const int* chunkStart = s.chunkStart + kStTracklets * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
-  const int2* chunkInfo = s.chunkInfo + (size_t)kStTracklets * s.maxChunks;
-  int2 nextInfo = (int)blockIdx.x < nChunks ? chunkInfo[blockIdx.x] : make_int2(0, 0);
-  for (int chunk = blockIdx.x; chunk < nChunks; chunk += gridDim.x) {
-    const int2 ci = nextInfo;  // this chunk's (segment, first cluster); the next one's is loaded ahead
-    if (chunk + (int)gridDim.x < nChunks) nextInfo = chunkInfo[chunk + gridDim.x];
+  const int4* chunkInfo = s.chunkInfo + (size_t)kStTracklets * s.maxChunks;
+  // this chunk's entry (segment, first cluster, its global index, clusters)
+  // and its clusters' raw tracklet counts; the next chunk's are loaded ahead
+  auto rawOf = [&](const int4& e) { return (int)threadIdx.x < e.w ? s.trkCnt[e.z + threadIdx.x] : 0; };
+  const int G = gridDim.x;
+  int4 curInfo = (int)blockIdx.x < nChunks ? chunkInfo[blockIdx.x] : make_int4(0, 0, 0, 0);
+  int rawCur = (int)blockIdx.x < nChunks ? rawOf(curInfo) : 0;
+  int4 nextInfo = (int)blockIdx.x + G < nChunks ? chunkInfo[blockIdx.x + G] : make_int4(0, 0, 0, 0);
+  for (int chunk = blockIdx.x; chunk < nChunks; chunk += G) {
+    const int4 ci = curInfo;
+    const int raw = rawCur;
+    if (chunk + G < nChunks) {
+      curInfo = nextInfo;
+      rawCur = rawOf(nextInfo);
+      if (chunk + 2 * G < nChunks) nextInfo = chunkInfo[chunk + 2 * G];
+    }
     const int sg = ci.x;
     const int w = sg / kTrackletLayers, l = sg % kTrackletLayers;
     const Seg cur = s.clSeg[w * kLayers + l], next = s.clSeg[w * kLayers + l + 1];
@@ -259,7 +270,6 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) kTr
     const int first = ci.y;
     const int i = first + threadIdx.x;
     const int p = cur.base + i;
-    const int raw = i < cur.count ? s.trkCnt[p] : 0;
     const int scored = raw <= kTrkSlots ? raw : 0;
     int inc = scored;  // wavefront inclusive scan
     for (int d = 1; d < 64; d <<= 1) {

@@ -172,7 +172,7 @@ struct itsca_ctx {
   DBuf<uint32_t> cpaths;
   DBuf<int> pred, rCells, rLabel, rFake;
   DBuf<int> chunkStart, chunkSum, chunkOff, planScratch, dStatus;
-  DBuf<int2> chunkInfo;
+  DBuf<int4> chunkInfo;
 
   void dropGraph()
   {
@@ -825,7 +825,7 @@ bool carveNoAlloc(unsigned char* base, size_t bytes, int n, DevState& s, int*& p
   const size_t perTracklet = 8 + 8 + 4 + 4 + 4 * kCellSlots;
   const size_t perCell = 6 * 4 + 16 + 4 + 3 * 4 + 4 + 4 * kEdgeSlots;
   const size_t perEdge = 4 * 4, perRoad = 4 * kCellLayers + 4 + 4;
-  const size_t perUnit = 10 * perTracklet + perCell + perEdge + perRoad + 4 * 4 * kNumStages * 10 / kMinChunk + 64;
+  const size_t perUnit = 10 * perTracklet + perCell + perEdge + perRoad + 6 * 4 * kNumStages * 10 / kMinChunk + 64;
   const size_t left = bytes - off;
   const size_t k = std::min<size_t>(left / perUnit, (size_t)(kMaxCapacity / 10));
   if (k < 64) return false;
@@ -835,7 +835,7 @@ bool carveNoAlloc(unsigned char* base, size_t bytes, int n, DevState& s, int*& p
   s.maxChunks = (int)((std::max<size_t>(N, 10 * k) + kMinChunk - 1) / kMinChunk) + 8 + 1;
   s.chunkSum = static_cast<int*>(take((size_t)kNumStages * s.maxChunks, 4));
   s.chunkOff = static_cast<int*>(take((size_t)kNumStages * s.maxChunks, 4));
-  s.chunkInfo = static_cast<int2*>(take((size_t)kNumStages * s.maxChunks, sizeof(int2)));
+  s.chunkInfo = static_cast<int4*>(take((size_t)kNumStages * s.maxChunks, sizeof(int4)));
   s.tFS = static_cast<int2*>(take(s.tCap, sizeof(int2)));
   s.tTP = static_cast<float2*>(take(s.tCap, sizeof(float2)));
   s.cellCnt = static_cast<int*>(take(s.tCap, 4));
