@@ -177,3 +177,32 @@ def test_central_event_sort_orders_have_the_same_keyed_sets():
     once = pyoracle.OracleVertex(ev, 0, stable=False, exclusive_bound=True).dump()
     assert distinct_digest(once) == info["ref_distinct_sha256"]
     assert all(n == 0 for n in duplicates(once).values())
+
+
+def test_dca_rounding_bound_holds():
+    """The first-order bound on the float32 rounding of the cell DCA chain
+    (tests/fp_bounds.py), which the contraction study uses to judge DCA
+    disagreements, against float32 evaluations of the chain: one rounding
+    per operation, and every multiply-add fused; both stay within the bound
+    of the float64 value for the cells of a central event."""
+    from fp_bounds import dca_float32, dca_with_bound
+
+    ev = itsca.generate_events(2, 20000, 1)[0]
+    o = pyoracle.OracleVertex(ev, 0, stable=True)
+    cl = [_clusters(o, l) for l in range(7)]
+    worst = [0.0, 0.0]
+    n = 0
+    for l in range(5):
+        cells = o.records(2, l)
+        for row in cells[:: max(1, len(cells) // 300)]:
+            xs = [float(cl[l + k]["x"][row[k]]) for k in range(3)]
+            ys = [float(cl[l + k]["y"][row[k]]) for k in range(3)]
+            v, e, _ = dca_with_bound(xs, ys, 0.0, 0.0)
+            if not (np.isfinite(v) and np.isfinite(e)):
+                continue
+            n += 1
+            for k, fused in enumerate((False, True)):
+                d = abs(dca_float32(xs, ys, 0.0, 0.0, fused) - v)
+                worst[k] = max(worst[k], d / e)
+    assert n > 1000
+    assert worst[0] <= 1.0 and worst[1] <= 1.0, worst
