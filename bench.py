@@ -466,7 +466,8 @@ def main():
                              "track_recall": round(quality["recall"], 6), "plain_tracks": quality["plain_tracks"],
                              "tracks_found": quality["found"], "tracks_outside_plain": quality["extra"],
                              "tracklets_kept": round(quality["tracklets_kept"], 4),
-                             "cells_kept": round(quality["cells_kept"], 4)},
+                             "cells_kept": round(quality["cells_kept"], 4),
+                             "road_cells_kept": round(quality["road_cells_kept"], 6)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": measured_traffic(top, args.batch, args.seed),

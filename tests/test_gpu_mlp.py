@@ -287,10 +287,15 @@ def test_trained_filter_gain_over_plain_and_untrained(tracker):
     params, thr, _ = ft.load_filter()
     tracker.set_mlp_params(params)
     r = ft.evaluate(tracker, evs, thr)
-    say(f"  trained: recall {r['recall']:.5f}, outside {r['extra']}, tracklets kept {r['tracklets_kept']:.3f}")
+    say(f"  trained: recall {r['recall']:.5f}, outside {r['extra']}, tracklets kept {r['tracklets_kept']:.3f}, "
+        f"road cells kept {r['road_cells_kept']:.5f}")
     assert r["extra"] == 0
     assert r["recall"] >= 0.995
+    assert r["road_cells_kept"] >= 0.995
     assert r["tracklets_kept"] < 0.3
+    everything = ft.evaluate(tracker, evs, -3.0e38, plain=r["plain"])  # a cut no score fails
+    assert (everything["recall"], everything["road_cells_kept"], everything["cells_kept"]) == (1.0, 1.0, 1.0)
+    tracker.set_filter(True, thr)
     learned_ms, learned_cells = _device_ms(tracker)
     tracker.set_filter(False)
     plain_ms, plain_cells = _device_ms(tracker)

@@ -532,6 +532,21 @@ def road_tracks(tracker: GpuTracker, item: int) -> List[tuple]:
     return list(zip(*[ids[l][cols[l]].tolist() for l in range(LAYERS)]))
 
 
+def cell_keys(tracker: GpuTracker, item: int) -> set:
+    """Cells of one work item as (cell layer, cluster id, cluster id, cluster
+    id): identities independent of the indices of a particular run."""
+    n = tracker.counts(item)
+    out = set()
+    ids = [tracker.download(item, CLUSTERS, l, n.clusters[l], 8)[:, 0] for l in range(LAYERS)]
+    for k in range(CELL_LAYERS):
+        if n.cells[k] == 0:
+            continue
+        c = tracker.download(item, CELLS, k, n.cells[k], 10)
+        out.update(zip([k] * len(c), ids[k][c[:, 0]].tolist(), ids[k + 1][c[:, 1]].tolist(),
+                       ids[k + 2][c[:, 2]].tolist()))
+    return out
+
+
 def _fptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 

@@ -20,7 +20,7 @@ from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 
-from . import MLP_PARAMS, GpuTracker, generate_events, road_tracks
+from . import MLP_PARAMS, GpuTracker, cell_keys, generate_events, road_tracks
 
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "filter_weights.json")
 
@@ -86,15 +86,18 @@ def train(tracker: GpuTracker, epochs: int, batch: int = 4096, lr: float = 3e-3,
 def evaluate(tracker: GpuTracker, events, threshold: Optional[float], plain=None) -> Dict:
     """Runs `events` plain (unless their plain tracks are given) and with the
     filter at `threshold`; returns recall of the plain tracks (with
-    multiplicity), tracks outside the plain set, and the tracklet and cell
-    counts of both runs."""
+    multiplicity), tracks outside the plain set, the tracklets and cells kept,
+    and the road cells kept: of the distinct cells of the plain roads (the
+    cells the training labels mark), the share the filtered run still builds."""
     tracker.set_events(events)
     items = None
     if plain is None:
         tracker.set_filter(False)
         tracker.run()
         items = tracker.num_items()
-        plain = {"tracks": [Counter(road_tracks(tracker, i)) for i in range(items)],
+        tracks = [Counter(road_tracks(tracker, i)) for i in range(items)]
+        plain = {"tracks": tracks,
+                 "road_cells": [{(k,) + t[k:k + 3] for t in tr for k in range(5)} for tr in tracks],
                  "tracklets": sum(sum(tracker.counts(i).tracklets) for i in range(items)),
                  "cells": sum(sum(tracker.counts(i).cells) for i in range(items))}
     tracker.set_filter(True, threshold)
@@ -108,8 +111,14 @@ def evaluate(tracker: GpuTracker, events, threshold: Optional[float], plain=None
         total += sum(plain["tracks"][i].values())
     tl = sum(sum(tracker.counts(i).tracklets) for i in range(items))
     ce = sum(sum(tracker.counts(i).cells) for i in range(items))
+    rc = rc_kept = 0
+    for i in range(items):
+        want = plain["road_cells"][i]
+        rc += len(want)
+        rc_kept += len(want & cell_keys(tracker, i)) if want else 0
     return {"plain": plain, "recall": found / max(total, 1), "plain_tracks": total, "found": found, "extra": extra,
-            "tracklets_kept": tl / max(plain["tracklets"], 1), "cells_kept": ce / max(plain["cells"], 1)}
+            "tracklets_kept": tl / max(plain["tracklets"], 1), "cells_kept": ce / max(plain["cells"], 1),
+            "road_cells_kept": rc_kept / max(rc, 1)}
 
 
 def tune_threshold(tracker: GpuTracker, events, target: float, scores_pos: np.ndarray) -> Dict:
@@ -135,8 +144,60 @@ def tune_threshold(tracker: GpuTracker, events, target: float, scores_pos: np.nd
     return best
 
 
+def _replay_ms(tracker: GpuTracker, steps: int = 10) -> float:
+    """Mean ms per run of the uploaded batch, replayed as a HIP graph."""
+    tracker.set_graph(False)
+    tracker.run()
+    tracker.set_graph(True)
+    tracker.run_async()
+    tracker.finish()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tracker.run_async()
+    tracker.finish()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    tracker.set_graph(False)
+    return ms
+
+
+def sweep(events, params: np.ndarray, thresholds: Sequence[float]) -> List[Dict]:
+    """The trade-off of the threshold on `events`: per threshold, the road
+    recall of the plain tracks, tracks outside them, the tracklets and cells
+    kept, and the time per run (graph replay, block 256, 8 workgroups per CU,
+    one z sub-bin), with the plain form's time for reference."""
+    tracker = GpuTracker(device=0)
+    tracker.set_mlp_params(params)
+    tracker.set_tuning(256, 0, 1)
+    tracker.set_events(events)
+    tracker.set_filter(False)
+    plain_ms = _replay_ms(tracker)
+    plain = None
+    out = []
+    for thr in sorted(thresholds):
+        r = evaluate(tracker, events, thr, plain)
+        plain = r.pop("plain")
+        tracker.set_filter(True, thr)
+        ms = _replay_ms(tracker)
+        row = dict(r, threshold=thr, ms=ms, plain_ms=plain_ms, speedup=plain_ms / ms)
+        _log("  threshold %8.4f: recall %.5f, outside %d, tracklets kept %.4f, cells kept %.4f, road cells kept "
+             "%.5f, %.3f ms (plain %.3f ms, %.3fx)" % (thr, r["recall"], r["extra"], r["tracklets_kept"],
+                                                      r["cells_kept"], r["road_cells_kept"], ms, plain_ms,
+                                                      plain_ms / ms))
+        out.append(row)
+    tracker.close()
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--sweep", action="store_true",
+                    help="instead of training: the threshold trade-off of the weights file (--out) on held-out "
+                         "events (--sweep-first-seed, --sweep-events)")
+    ap.add_argument("--sweep-first-seed", type=int, default=20000)
+    ap.add_argument("--sweep-events", type=int, default=256)
+    ap.add_argument("--sweep-offsets", type=float, nargs="+",
+                    default=[-40.0, -30.0, -24.0, -18.0, -12.0, -8.0, -4.0, -2.0, -1.0, 0.0, 0.5, 1.0, 2.0],
+                    help="thresholds of the sweep, relative to the file's")
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--preset", type=int, default=2, help="event preset of the training events (2: central)")
     ap.add_argument("--first-seed", type=int, default=10000)
@@ -152,6 +213,16 @@ def main(argv=None):
     ap.add_argument("--init-seed", type=int, default=7)
     ap.add_argument("--target-recall", type=float, default=0.999, help="road recall on the validation events")
     args = ap.parse_args(argv)
+
+    if args.sweep:
+        params, thr, _ = load_filter(args.out)
+        events = generate_events(args.preset, args.sweep_first_seed, args.sweep_events)
+        thresholds = [thr + d for d in args.sweep_offsets]
+        rows = sweep(events, params, thresholds)
+        print(json.dumps({"events": "preset %d seeds %d..%d" % (args.preset, args.sweep_first_seed,
+                                                                  args.sweep_first_seed + args.sweep_events - 1),
+                          "shipped_threshold": thr, "rows": rows}))
+        return
 
     t0 = time.time()
     tracker = GpuTracker(device=0)
