@@ -319,6 +319,41 @@ def test_trained_filter_gain_over_plain_and_untrained(tracker):
     assert kept_u > 2.5 * kept_t
 
 
+def test_training_end_to_end(tracker):
+    """The whole training at a small size: labels from plain runs over 32
+    central events, four AdamW epochs from a fresh initialisation, the
+    threshold tuned on 8 validation events; then, on 8 events neither set
+    holds, the trained filter keeps the plain roads (recall >= 99 %, none
+    outside) while cutting most tracklets, and its training rows separate far
+    better than the initial network's."""
+    from itsca import filter_training as ft
+    say("gpu training end to end: collect, train, tune, evaluate on held-out events")
+    params, thr, meta = ft.train_filter(tracker, 2, range(11000, 11032), 2e6, range(11500, 11508), epochs=4,
+                                        target_recall=0.995)
+    say(f"  {meta['rows']} rows, final loss {meta['final_loss']:.5f}, threshold {thr:.4f}, "
+        f"validation recall {meta['validation_road_recall']:.5f}")
+    assert meta["validation_road_recall"] >= 0.995 and thr > -3.0e38
+    n = meta["rows"]
+    _, y = tracker.train_get(n)
+    z_trained = tracker.train_scores(n)
+    tracker.mlp_init(7)
+    z_initial = tracker.train_scores(n)
+    recall = float(np.mean(z_trained[y > 0] >= thr))
+    kept_t = float(np.mean(z_trained >= thr))
+    kept_i = float(np.mean(z_initial >= np.quantile(z_initial[y > 0], 1.0 - recall)))
+    say(f"  training rows at positive recall {recall:.5f}: trained keeps {kept_t:.3f}, initial keeps {kept_i:.3f}")
+    assert kept_i > 2.0 * kept_t
+
+    tracker.set_mlp_params(params)
+    r = ft.evaluate(tracker, itsca.generate_events(2, 31000, 8), thr)
+    say(f"  held out: recall {r['recall']:.5f}, outside {r['extra']}, tracklets kept {r['tracklets_kept']:.3f}, "
+        f"road cells kept {r['road_cells_kept']:.5f}")
+    assert r["extra"] == 0
+    assert r["recall"] >= 0.99
+    assert r["tracklets_kept"] < 0.4
+    tracker.set_filter(False)
+
+
 def test_cli_plain_and_with_learned_filter(golden_dir):
     """The reference-style CLI (bin/itsca-track) finds the fixture's road count
     plain, and a subset of it with --filter and the shipped weights."""

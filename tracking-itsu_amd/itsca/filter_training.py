@@ -188,6 +188,47 @@ def sweep(events, params: np.ndarray, thresholds: Sequence[float]) -> List[Dict]
     return out
 
 
+def train_filter(tracker: GpuTracker, preset: int, seeds: Sequence[int], samples: float, val_seeds: Sequence[int],
+                 epochs: int = 12, batch: int = 4096, lr: float = 3e-3, weight_decay: float = 1e-4,
+                 pos_weight: float = 4.0, init_seed: int = 7, target_recall: float = 0.999):
+    """The whole training: labels from plain runs over the events of `seeds`
+    (about `samples` rows), AdamW epochs from a fresh initialisation, and the
+    threshold tuned to `target_recall` of the plain roads of the `val_seeds`
+    events.  Returns (params, threshold, meta); the tracker keeps the trained
+    weights, filter off."""
+    seeds, val_seeds = list(seeds), list(val_seeds)
+    t0 = time.time()
+    tracker.set_filter(False)
+    tracker.set_events(generate_events(preset, seeds[0], 1))
+    tracker.run()
+    per_event = sum(sum(tracker.counts(i).tracklets) for i in range(tracker.num_items()))
+    stride = max(1, int(round(per_event * len(seeds) / samples)))
+    cap = int(per_event * 1.1 * len(seeds) / stride) + 1024
+    n = collect(tracker, preset, seeds, stride, cap)
+    X, Y = tracker.train_get(n)
+    _log("collected %d rows (stride %d, %d tracklets in the first event), %.4f positive, %.1f s"
+         % (n, stride, per_event, Y.mean(), time.time() - t0))
+
+    tracker.mlp_init(init_seed)
+    losses = train(tracker, epochs, batch, lr, weight_decay=weight_decay, pos_weight=pos_weight, seed=init_seed)
+    _log("epoch losses " + " ".join("%.5f" % v for v in losses) + "  (%.1f s)" % (time.time() - t0))
+    z = tracker.train_scores(n)
+    params = tracker.mlp_params()
+
+    val = [e for s in val_seeds for e in generate_events(preset, s, 1)]
+    best = tune_threshold(tracker, val, target_recall, z[Y > 0])
+    meta = {"training_events": "preset %d seeds %d..%d" % (preset, seeds[0], seeds[-1]),
+            "rows": int(n), "stride": stride, "positive_fraction": float(Y.mean()), "epochs": epochs,
+            "batch": batch, "lr": lr, "weight_decay": weight_decay, "pos_weight": pos_weight,
+            "final_loss": losses[-1], "validation_events": "preset %d seeds %d..%d" % (
+                preset, val_seeds[0], val_seeds[-1]),
+            "validation_road_recall": best["recall"], "validation_tracklets_kept": best["tracklets_kept"],
+            "validation_cells_kept": best["cells_kept"],
+            "training_row_recall": float(np.mean(z[Y > 0] >= best["threshold"])),
+            "training_rows_kept": float(np.mean(z >= best["threshold"]))}
+    return params, best["threshold"], meta
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--sweep", action="store_true",
@@ -226,40 +267,13 @@ def main(argv=None):
 
     t0 = time.time()
     tracker = GpuTracker(device=0)
-    probe = generate_events(args.preset, args.first_seed, 1)
-    tracker.set_events(probe)
-    tracker.run()
-    per_event = sum(sum(tracker.counts(i).tracklets) for i in range(tracker.num_items()))
-    stride = max(1, int(round(per_event * args.events / args.samples)))
-    cap = int(per_event * 1.1 * args.events / stride) + 1024
-    n = collect(tracker, args.preset, range(args.first_seed, args.first_seed + args.events), stride, cap)
-    X, Y = tracker.train_get(n)
-    _log("collected %d rows (stride %d, %d tracklets in the first event), %.4f positive, %.1f s"
-         % (n, stride, per_event, Y.mean(), time.time() - t0))
-
-    tracker.mlp_init(args.init_seed)
-    losses = train(tracker, args.epochs, args.batch, args.lr, weight_decay=args.weight_decay,
-                   pos_weight=args.pos_weight, seed=args.init_seed)
-    _log("epoch losses " + " ".join("%.5f" % v for v in losses) + "  (%.1f s)" % (time.time() - t0))
-    z = tracker.train_scores(n)
-    params = tracker.mlp_params()
-
-    val = [e for s in range(args.val_first_seed, args.val_first_seed + args.val_events)
-           for e in generate_events(args.preset, s, 1)]
-    best = tune_threshold(tracker, val, args.target_recall, z[Y > 0])
-    row_recall = float(np.mean(z[Y > 0] >= best["threshold"]))
-    row_kept = float(np.mean(z >= best["threshold"]))
-    meta = {"training_events": "preset %d seeds %d..%d" % (args.preset, args.first_seed,
-                                                              args.first_seed + args.events - 1),
-            "rows": int(n), "stride": stride, "positive_fraction": float(Y.mean()), "epochs": args.epochs,
-            "batch": args.batch, "lr": args.lr, "weight_decay": args.weight_decay, "pos_weight": args.pos_weight,
-            "final_loss": losses[-1], "validation_events": "preset %d seeds %d..%d" % (
-                args.preset, args.val_first_seed, args.val_first_seed + args.val_events - 1),
-            "validation_road_recall": best["recall"], "validation_tracklets_kept": best["tracklets_kept"],
-            "validation_cells_kept": best["cells_kept"], "training_row_recall": row_recall,
-            "training_rows_kept": row_kept}
-    save_filter(args.out, params, best["threshold"], meta)
-    _log(json.dumps(dict(meta, threshold=best["threshold"], seconds=round(time.time() - t0, 1))))
+    params, threshold, meta = train_filter(
+        tracker, args.preset, range(args.first_seed, args.first_seed + args.events), args.samples,
+        range(args.val_first_seed, args.val_first_seed + args.val_events), epochs=args.epochs, batch=args.batch,
+        lr=args.lr, weight_decay=args.weight_decay, pos_weight=args.pos_weight, init_seed=args.init_seed,
+        target_recall=args.target_recall)
+    save_filter(args.out, params, threshold, meta)
+    _log(json.dumps(dict(meta, threshold=threshold, seconds=round(time.time() - t0, 1))))
     tracker.close()
 
 
