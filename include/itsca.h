@@ -203,7 +203,12 @@ itsca_status itsca_run_noalloc(itsca_ctx* ctx, const itsca_device_event* ev, int
 itsca_status itsca_mlp_init(itsca_ctx* ctx, uint64_t seed); /* He initialisation */
 itsca_status itsca_mlp_set_params(itsca_ctx* ctx, const float* params);
 itsca_status itsca_mlp_get_params(itsca_ctx* ctx, float* params);
-/* Turns the filter on for later runs; tracklets with logit < threshold are dropped. */
+/* Turns the filter on for later runs; tracklets with logit < threshold are dropped.
+ * enable: 0 off; 1 on, scored by a kernel of its own after the search of every
+ * tracklet layer; 2 on, scored inside the search (one launch per tracklet
+ * layer, no candidate round trip through memory; slower on MI355X).  Both give
+ * the same tracklets; form 2 falls back to form 1 for batches searched with
+ * more than one z sub-bin or with a layer of more than 65536 clusters. */
 itsca_status itsca_set_filter(itsca_ctx* ctx, int enable, float threshold);
 
 /* Training set on the device: rows of ITSCA_MLP_FEATURES BF16 values and a

@@ -2,7 +2,8 @@
 compiled for the host against the HIP shim of tests/emu/ (one fiber per GPU
 thread, the wavefront exchanges and the MFMA instructions emulated), with
 AddressSanitizer.  Checks the kernels' logic and bounds without a GPU: golden
-fixtures bit for bit under two tunings, the learned filter's paths, an event
+fixtures bit for bit under two tunings, the learned filter's paths (scored
+after the search and inside it), an event
 of 1831 clusters per layer and an event whose tracklets overflow the count's
 slots, against the oracle."""
 import glob
@@ -44,8 +45,12 @@ for name in ("small_s1", "sparse_s6", "small_s5_nolayer3"):
         t.run()
         assert itsca.dump_item(t, 0, ev.vertices[0]) == fixture(name, "stable", 0)
         thr = float(np.quantile(z, 0.5))
+        t.set_filter(True, thr, fused=True)  # the search and the filter in one kernel: the same tracklets
+        t.run()
+        fused = itsca.dump_item(t, 0, ev.vertices[0])
         t.set_filter(True, thr)
         t.run()
+        assert itsca.dump_item(t, 0, ev.vertices[0]) == fused, name
         assert not (Counter(itsca.road_tracks(t, 0)) - plain)
         k = 0
         for l in range(6):
@@ -102,6 +107,13 @@ t.run()
 n = t.counts(0)
 assert np.bincount(t.download(0, itsca.TRACKLETS, 0, n.tracklets[0], 4)[:, 0]).max() > itsca.TRACKLET_SLOTS
 assert itsca.dump_item(t, 0, ev.vertices[0]) == pyoracle.OracleVertex(ev, 0, stable=True).dump()
+t.mlp_init(4)
+t.set_filter(True, 0.0, fused=True)
+t.run()
+fused = itsca.dump_item(t, 0, ev.vertices[0])
+t.set_filter(True, 0.0)
+t.run()
+assert itsca.dump_item(t, 0, ev.vertices[0]) == fused
 t.close()
 print("slot-overflow event vs oracle ok", flush=True)
 print("ALL OK")

@@ -276,6 +276,37 @@ def _device_ms(tracker, runs=3):
     return total, cells
 
 
+def test_fused_filter_equals_separate_filter(tracker):
+    """The filter scored inside the tracklet search (one launch per layer)
+    keeps exactly the tracklets of the filter scored by its own kernel: every
+    array of every item, on central events and the fixtures, at the shipped
+    threshold and at the median score of a random network."""
+    from itsca import filter_training as ft
+    say("gpu fused filter vs separate filter")
+    evs = itsca.generate_events(2, 30100, 6) + [itsca.load_events(p)[0] for _, p, _ in CASES]
+    tracker.set_events(evs)
+    items = tracker.num_items()
+    params, thr, _ = ft.load_filter()
+    for p, t in ((params, thr), (_random_params(41), None)):
+        tracker.set_mlp_params(p)
+        if t is None:
+            tracker.set_filter(False)
+            tracker.run()
+            total = sum(sum(tracker.counts(i).tracklets) for i in range(items))
+            tracker.train_clear(total)
+            tracker.train_collect(1, 0)
+            t = float(np.median(tracker.train_scores(total)))
+        outs = []
+        for fused in (False, True):
+            tracker.set_filter(True, t, fused=fused)
+            tracker.run()
+            outs.append([itsca.dump_item(tracker, i, (0.0, 0.0, 0.0)) for i in range(items)])
+        kept = sum(sum(tracker.counts(i).tracklets) for i in range(items))
+        say(f"  threshold {t:.4f}: {kept} tracklets kept, fused == separate: {outs[0] == outs[1]}")
+        assert outs[0] == outs[1]
+    tracker.set_filter(False)
+
+
 def test_trained_filter_gain_over_plain_and_untrained(tracker):
     """Ablation on held-out central events (seeds the training never saw): the
     shipped network keeps >= 99.5 % of the plain tracks, finds none outside

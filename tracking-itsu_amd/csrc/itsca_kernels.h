@@ -38,6 +38,7 @@ struct LaunchConfig {
   int maxLayerClusters;   // largest cluster count of any (item, layer)
   bool filter;            // learned tracklet filter after the tracklet count
   float threshold;        // its logit threshold
+  bool fusedFilter;       // the filter inside the tracklet search (kTrackletsSearchScore), one launch per layer
 };
 
 void launchPipeline(const DevState& s, const LaunchConfig& cfg, hipStream_t stream, KernelTimer* timer);

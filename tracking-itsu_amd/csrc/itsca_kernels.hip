@@ -15,6 +15,7 @@
 
 #include "itsca_device.h"
 #include "itsca_kernels.h"
+#include "itsca_mlp_device.h"
 #include "itsca_mlp_kernels.h"
 
 namespace itsca {
@@ -824,6 +825,105 @@ __device__ __forceinline__ void forEachTracklet(const DevState& s, int rw, int l
 constexpr int kStageCap = 1984;  // next-layer clusters per chunk (31 KB: five workgroups of 256 per CU)
 constexpr int kStageRows = 8;    // phi rows of the copy
 
+// The LDS copy of one chunk: the union of its windows, its rows in the copy.
+struct StageLds {
+  int uni[5];  // min row offset, max row end, min z bin, max z bin, no-copy flag
+  int rowLo[kStageRows], rowOff[kStageRows], rowStart[kStageRows + 1];
+  int copyRows, copyRef;  // rows of the LDS copy (0: none) and the window row of its first row
+};
+
+// Copies the next-layer records (`src`, the next layer's segment, in the order
+// the search reads) of the union of the chunk's windows to `stage`, at most
+// `cap` of them, and returns the number of rows copied (0: none fit; the
+// search then reads global memory).  Called by every thread of the block; t
+// is the thread's search (empty past the chunk), rowRef the window row before
+// the chunk's first cluster.  Rows are taken relative to rowRef: the chunk's
+// clusters sit in consecutive rows, and a window starts at most one row
+// before its cluster's row.
+__device__ __forceinline__ int stageUnion(const TrackletSearch& t, int rowRef, const int* table, int nNext,
+                                          const float4* src, float4* stage, int cap, StageLds& L)
+{
+  if (threadIdx.x == 0) {
+    L.uni[0] = kPhiBins;
+    L.uni[1] = -kPhiBins;
+    L.uni[2] = kZBins;
+    L.uni[3] = -1;
+    L.uni[4] = 0;
+  }
+  __syncthreads();
+  int dMin = kPhiBins, dMax = -kPhiBins, zMin = kZBins, zMax = -1, noCopy = 0;
+  if (!t.win.empty) {
+    if (t.win.y >= kPhiBins || t.win.rows > kStageRows) noCopy = 1;  // the phi == 2pi row, or a wide window
+    int d = t.win.y - rowRef;
+    if (d < 0) d += kPhiBins;
+    if (d >= kPhiBins / 2) d -= kPhiBins;
+    dMin = d;
+    dMax = d + t.win.rows - 1;
+    if (t.zFirst <= t.zLast) {
+      zMin = t.zFirst;
+      zMax = t.zLast;
+    }
+  }
+  for (int m = 32; m >= 1; m >>= 1) {
+    dMin = min(dMin, __shfl_xor(dMin, m, 64));
+    dMax = max(dMax, __shfl_xor(dMax, m, 64));
+    zMin = min(zMin, __shfl_xor(zMin, m, 64));
+    zMax = max(zMax, __shfl_xor(zMax, m, 64));
+    noCopy = max(noCopy, __shfl_xor(noCopy, m, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(&L.uni[0], dMin);
+    atomicMax(&L.uni[1], dMax);
+    atomicMin(&L.uni[2], zMin);
+    atomicMax(&L.uni[3], zMax);
+    atomicMax(&L.uni[4], noCopy);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {  // the copy: rows [dMin, dMax] of the union, z bins [zMin, zMax] of each
+    const int nr = L.uni[1] - L.uni[0] + 1;
+    int rows = 0, ref = 0;
+    if (L.uni[4] == 0 && nr >= 1 && nr <= kStageRows) {
+      ref = ((rowRef + L.uni[0]) % kPhiBins + kPhiBins) % kPhiBins;
+      int total = 0;
+      for (int ri = 0; ri < nr; ++ri) {
+        const int row = (ref + ri) % kPhiBins;
+        int lo = 0, hi = 0;
+        if (L.uni[2] <= L.uni[3]) {
+          lo = table[binIndex(L.uni[2], row)];
+          hi = max(lo, min(table[binIndex(L.uni[3], row) + 1], nNext));
+        }
+        L.rowLo[ri] = lo;
+        L.rowStart[ri] = total;
+        L.rowOff[ri] = total - lo;
+        total += hi - lo;
+      }
+      L.rowStart[nr] = total;
+      if (total <= cap) rows = nr;
+    }
+    L.copyRows = rows;
+    L.copyRef = ref;
+  }
+  __syncthreads();
+  const int nrows = L.copyRows;
+  if (nrows > 0) {
+    const int total = L.rowStart[nrows];
+    for (int k = threadIdx.x; k < total; k += blockDim.x) {
+      int ri = 0;
+      while (ri + 1 < nrows && L.rowStart[ri + 1] <= k) ++ri;
+      stage[k] = src[L.rowLo[ri] + (k - L.rowStart[ri])];
+    }
+    __syncthreads();
+  }
+  return nrows;
+}
+
+// The window row before the chunk's first cluster (sorted position `first`
+// of the records at `cbin`).
+__device__ __forceinline__ int chunkRowRef(const int* cbin, int first)
+{
+  return (max(cbin[first], 0) / kPhiBins + kPhiBins - 1) % kPhiBins;
+}
+
 // Count: each cluster's accepted candidates, in emission order, are kept in
 // kTrkSlots scratch slots so that the fill does not repeat the window search.
 // Five waves per SIMD (the LDS copy allows five workgroups of 256 per CU).
@@ -834,9 +934,7 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) kTr
 {
   __shared__ int scratch[B / 64 + 1];
   __shared__ float4 stage[kStageCap];
-  __shared__ int uni[5];  // min row offset, max row end, min z bin, max z bin, no-copy flag
-  __shared__ int rowLo[kStageRows], rowHi[kStageRows], rowOff[kStageRows], rowStart[kStageRows + 1];
-  __shared__ int copyRows, copyRef;  // rows of the LDS copy (0: none) and the window row of its first row
+  __shared__ StageLds L;
   const int S = s.W * kTrackletLayers;
   const int* chunkStart = s.chunkStart + kStTracklets * (s.maxSegs + 1);
   const int nChunks = chunkStart[S];
@@ -861,83 +959,9 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) kTr
       phi = s.cphi[rcur.base + i];
       t = trackletSearch(s.segGeo[rw * kLayers + l + 1], z, r, phi, s.itemVertex[3 * w + 2], l);
     }
-    // Union of the windows.  Rows are taken relative to the row before the
-    // chunk's first cluster: the chunk's clusters sit in consecutive rows, and
-    // a window starts at most one row before its cluster's row.
-    const int rowRef = (max(s.cbin[rcur.base + first], 0) / kPhiBins + kPhiBins - 1) % kPhiBins;
-    if (threadIdx.x == 0) {
-      uni[0] = kPhiBins;
-      uni[1] = -kPhiBins;
-      uni[2] = kZBins;
-      uni[3] = -1;
-      uni[4] = 0;
-    }
-    __syncthreads();
-    int dMin = kPhiBins, dMax = -kPhiBins, zMin = kZBins, zMax = -1, noCopy = 0;
-    if (!t.win.empty) {
-      if (t.win.y >= kPhiBins || t.win.rows > kStageRows) noCopy = 1;  // the phi == 2pi row, or a wide window
-      int d = t.win.y - rowRef;
-      if (d < 0) d += kPhiBins;
-      if (d >= kPhiBins / 2) d -= kPhiBins;
-      dMin = d;
-      dMax = d + t.win.rows - 1;
-      if (t.zFirst <= t.zLast) {
-        zMin = t.zFirst;
-        zMax = t.zLast;
-      }
-    }
-    for (int m = 32; m >= 1; m >>= 1) {
-      dMin = min(dMin, __shfl_xor(dMin, m, 64));
-      dMax = max(dMax, __shfl_xor(dMax, m, 64));
-      zMin = min(zMin, __shfl_xor(zMin, m, 64));
-      zMax = max(zMax, __shfl_xor(zMax, m, 64));
-      noCopy = max(noCopy, __shfl_xor(noCopy, m, 64));
-    }
-    if ((threadIdx.x & 63) == 0) {
-      atomicMin(&uni[0], dMin);
-      atomicMax(&uni[1], dMax);
-      atomicMin(&uni[2], zMin);
-      atomicMax(&uni[3], zMax);
-      atomicMax(&uni[4], noCopy);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {  // the copy: rows [dMin, dMax] of the union, z bins [zMin, zMax] of each
-      const int nr = uni[1] - uni[0] + 1;
-      int rows = 0, ref = 0;
-      if (uni[4] == 0 && nr >= 1 && nr <= kStageRows) {
-        ref = ((rowRef + uni[0]) % kPhiBins + kPhiBins) % kPhiBins;
-        int total = 0;
-        for (int ri = 0; ri < nr; ++ri) {
-          const int row = (ref + ri) % kPhiBins;
-          int lo = 0, hi = 0;
-          if (uni[2] <= uni[3]) {
-            lo = table[binIndex(uni[2], row)];
-            hi = max(lo, min(table[binIndex(uni[3], row) + 1], next.count));
-          }
-          rowLo[ri] = lo;
-          rowHi[ri] = hi;
-          rowStart[ri] = total;
-          rowOff[ri] = total - lo;
-          total += hi - lo;
-        }
-        rowStart[nr] = total;
-        if (total <= kStageCap) rows = nr;
-      }
-      copyRows = rows;
-      copyRef = ref;
-    }
-    __syncthreads();
-    const int nrows = copyRows;
     const float4* copySrc = s.zSub > 1 ? s.crzpS : s.crzp;
-    if (nrows > 0) {
-      const int total = rowStart[nrows];
-      for (int k = threadIdx.x; k < total; k += B) {
-        int ri = 0;
-        while (ri + 1 < nrows && rowStart[ri + 1] <= k) ++ri;
-        stage[k] = copySrc[rnext.base + rowLo[ri] + (k - rowStart[ri])];
-      }
-      __syncthreads();
-    }
+    const int nrows = stageUnion(t, chunkRowRef(s.cbin + rcur.base, first), table, next.count, copySrc + rnext.base,
+                                 stage, kStageCap, L);
     int cnt = 0;
     if (i < cur.count) {
       int* slots = s.trkCand + (size_t)p * kTrkSlots;
@@ -950,8 +974,8 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) kTr
                                 s.subTab + (size_t)(rw * kLayers + l + 1) * kBins,
                                 s.zSub,
                                 stage,
-                                rowOff,
-                                copyRef};
+                                L.rowOff,
+                                L.copyRef};
       if (nrows > 0 && (t.sub || s.zSub == 1))  // with sub-bins the copy serves only the bin-by-bin search
         searchRows<true, 8, kSub>(t, table, next.count, l, z, r, phi, src, emit);
       else
@@ -960,6 +984,175 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) kTr
     }
     const int total = blockSum(cnt, scratch);  // its barriers also end this chunk's use of the copy
     if (threadIdx.x == 0) s.chunkSum[(size_t)kStTracklets * s.maxChunks + chunk] = total;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Learned form, second variant (itsca_set_filter mode 2): the tracklet search
+// and the filter fused, one tracklet layer per launch, from the last layer
+// down (a candidate's features read the raw tracklet count of its next-layer
+// cluster, which the previous launch wrote).  Per chunk: the search of
+// kTrackletsCount, with each cluster's accepted candidates kept in LDS as
+// 16-bit positions (the first kCandLds of them; the rest up to kTrkSlots go to
+// the cluster's slot row); then, lane by lane, step k scores the k-th
+// candidate of every lane's cluster (features, BF16 MFMA network) and appends
+// a kept one to the lane's slot row.  The raw count goes to trkCnt, the kept
+// count to trkKept (kCommitKept moves it after the last layer), the chunk's
+// kept sum to the chunk sums.  Clusters with more than kTrkSlots candidates
+// are kept whole (the fill recomputes them).  Needs one z sub-bin and next
+// layers of at most 65536 clusters.  The same tracklets as kTrackletsCount +
+// kTrackletsScore, with no candidate round trip through memory, but slower on
+// MI355X: the search and the scoring alternate within a workgroup at four
+// waves per SIMD, and the six launches each pay the chunk set-up.
+//
+// The workgroup walks the chunks of its stride sequence (xcdBlockIndex + k *
+// gridDim.x, as every stage kernel) that belong to the launch's layer, listed
+// in LDS B at a time.  The MFMA tiles alias the copy of the next layer, which
+// the search no longer reads by then; the LDS of a workgroup of 256 (27.6 KB
+// copy, 8 KB candidates) lets four of them share a CU.
+// ---------------------------------------------------------------------------
+constexpr int kCandLds = 16;           // candidates per cluster kept in LDS
+constexpr int kFusedStageCap = 1728;   // next-layer clusters of the copy (central events need at most ~1720)
+
+template <int B>
+__global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4)))
+kTrackletsSearchScore(DevState s, int layer, float threshold)
+{
+  constexpr int kWaves = B / 64;
+  struct ScoreLds {
+    uint16_t tiles[kWaves][64 * kMlpInPad];  // 16-byte aligned rows
+  };
+  static_assert(sizeof(ScoreLds) <= kFusedStageCap * sizeof(float4), "the scoring arrays alias the copy");
+  __shared__ float4 stage[kFusedStageCap];
+  __shared__ uint16_t cand[kCandLds * B];
+  __shared__ StageLds L;
+  __shared__ int scratch[B / 64 + 1];
+  __shared__ int list[B];
+  __shared__ int waveKept[kWaves];
+  ScoreLds& sc = *reinterpret_cast<ScoreLds*>(stage);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int l = layer;
+  const int S = s.W * kTrackletLayers;
+  const int nChunks = s.chunkStart[kStTracklets * (s.maxSegs + 1) + S];
+  const int4* info = s.chunkInfo + (size_t)kStTracklets * s.maxChunks;
+  const int G = gridDim.x, b0 = xcdBlockIndex();
+  const int mine = b0 < nChunks ? (nChunks - 1 - b0) / G + 1 : 0;
+  for (int r0 = 0; r0 < mine; r0 += B) {
+    const int kk = r0 + (int)threadIdx.x;
+    const int c = kk < mine ? b0 + kk * G : -1;
+    const int on = c >= 0 && info[c].x % kTrackletLayers == l ? 1 : 0;
+    int n;
+    const int at = blockExclusiveScan(on, scratch, n);
+    if (on) list[at] = c;
+    __syncthreads();
+    for (int q = 0; q < n; ++q) {
+      const int chunk = list[q];
+      const int4 ci = info[chunk];
+      const int w = ci.x / kTrackletLayers;
+      const Seg cur = s.clSeg[w * kLayers + l], next = s.clSeg[w * kLayers + l + 1];
+      const int rw = s.recItem[w];
+      const Seg rcur = s.clSeg[rw * kLayers + l], rnext = s.clSeg[rw * kLayers + l + 1];
+      const int first = ci.y;
+      const int i = first + threadIdx.x;
+      const int p = cur.base + i;
+      const int* table = s.table + (size_t)(rw * kLayers + l + 1) * kTableSize;
+      const float vx = s.itemVertex[3 * w], vy = s.itemVertex[3 * w + 1], vz = s.itemVertex[3 * w + 2];
+      TrackletSearch t;
+      float z = 0.0f, r = 0.0f, phi = 0.0f;
+      t.win.empty = true;
+      if (i < cur.count) {
+        z = s.cz[rcur.base + i];
+        r = s.cr[rcur.base + i];
+        phi = s.cphi[rcur.base + i];
+        t = trackletSearch(s.segGeo[rw * kLayers + l + 1], z, r, phi, vz, l);
+      }
+      const int nrows = stageUnion(t, chunkRowRef(s.cbin + rcur.base, first), table, next.count,
+                                   s.crzp + rnext.base, stage, kFusedStageCap, L);
+      int* myRow = s.trkCand + (size_t)p * kTrkSlots;
+      int raw = 0;
+      if (i < cur.count) {
+        auto emit = [&](int j) {
+          if (raw < kCandLds)
+            cand[raw * B + threadIdx.x] = (uint16_t)j;
+          else if (raw < kTrkSlots)
+            myRow[raw] = j;
+          ++raw;
+        };
+        const CandidateSource src{reinterpret_cast<const char*>(s.crzp + rnext.base), nullptr, nullptr, 1, stage,
+                                  L.rowOff, L.copyRef};
+        if (nrows > 0)
+          searchRows<true, 8, false>(t, table, next.count, l, z, r, phi, src, emit);
+        else
+          searchRows<false, 8, false>(t, table, next.count, l, z, r, phi, src, emit);
+        s.trkCnt[p] = raw;
+      }
+      __syncthreads();  // the copy is no longer read: its LDS holds the scoring arrays from here
+
+      // Scoring, lane by lane: step k scores the k-th candidate of every
+      // lane's cluster (one 64-sample tile per step; lanes whose cluster has
+      // fewer candidates idle), and a kept candidate is appended to the
+      // lane's slot row at once.  The weights are loaded per chunk (cache
+      // hits) so that they hold no registers during the search.
+      const MlpWave m = loadMlp(s.mlpW);
+      const int scored = raw <= kTrkSlots ? raw : 0;
+      int steps = scored;
+      for (int d = 32; d >= 1; d >>= 1) steps = max(steps, __shfl_xor(steps, d, 64));
+      TrackletSeed ts{};
+      if (scored > 0) {
+        const float4 a = s.cxyrz[rcur.base + i];  // (x, y, r, z)
+        ts = trackletSeed(a.x, a.y, a.w, a.z, phi, vx, vy, vz, raw);
+      }
+      // candidate k of this lane's cluster, -1 past its last; a slot-row entry
+      // is read two steps before any kept candidate can be written over it
+      auto candAt = [&](int k) {
+        if (k >= scored) return -1;
+        return k < kCandLds ? (int)cand[k * B + threadIdx.x] : myRow[k];
+      };
+      auto recOf = [&](int j) {
+        CandRec rec{make_float4(0.0f, 0.0f, 0.0f, 0.0f), 0.0f, 0};
+        if (j >= 0) {
+          rec.b = s.cxyrz[rnext.base + j];
+          rec.phi = s.cphi[rnext.base + j];
+          rec.count = l + 1 < kTrackletLayers ? s.trkCnt[next.base + j] : 0;
+        }
+        return rec;
+      };
+      int kept = 0;
+      int jCur = candAt(0), jNext = candAt(1);
+      CandRec qCur = recOf(jCur);
+      for (int k = 0; k < steps; ++k) {
+        const int jAfter = candAt(k + 2);
+        const CandRec qNext = recOf(jNext);
+        float f[kMlpIn];
+        if (jCur >= 0)
+          candidateFeatures(ts, l, qCur.b.x - vx, qCur.b.y - vy, qCur.b.w, qCur.b.z, qCur.phi, vz, qCur.count, f);
+        else
+          for (int e = 0; e < kMlpIn; ++e) f[e] = 0.0f;
+        uint4 lo, hi;
+        packFeatureRow(f, lo, hi);
+        storeTileRow(sc.tiles[wave], lane, lo, hi);
+        waveSync();
+        const float zl = mlpForward64(m, sc.tiles[wave]);
+        waveSync();  // the tile is rewritten by the next step
+        if (jCur >= 0 && zl >= threshold) myRow[kept++] = jCur;
+        jCur = jNext;
+        jNext = jAfter;
+        qCur = qNext;
+      }
+      const int keptCount = scored == raw ? kept : raw;
+      if (i < cur.count) s.trkKept[p] = keptCount;
+      int sum = i < cur.count ? keptCount : 0;
+      for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
+      if (lane == 0) waveKept[wave] = sum;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        int tk = 0;
+        for (int k = 0; k < kWaves; ++k) tk += waveKept[k];
+        s.chunkSum[(size_t)kStTracklets * s.maxChunks + chunk] = tk;
+      }
+      __syncthreads();  // the next chunk's copy overwrites the scoring arrays
+    }
+    __syncthreads();  // the list is rebuilt by the next round
   }
 }
 
@@ -1785,14 +1978,24 @@ void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t str
   hipLaunchKernelGGL(kChunkSegs, dim3(segBlocks((long)s.W * kTrackletLayers * cfg.maxLayerClusters)), dim3(256), 0, stream, s,
                      (int)kStTracklets, s.W * kTrackletLayers, 0, 0);
   done("kPlanTracklets", 1);
-  if (s.zSub > 1)
-    hipLaunchKernelGGL((kTrackletsCount<B, true>), dim3(G), dim3(B), 0, stream, s);
-  else
-    hipLaunchKernelGGL((kTrackletsCount<B, false>), dim3(G), dim3(B), 0, stream, s);
-  done("kTrackletsCount", 1);
-  if (cfg.filter) {
-    launchTrackletFilter(s, B, G, cfg.threshold, stream);
-    done("kTrackletsScore", 1);
+  // the fused search and filter keeps candidates as 16-bit positions and reads the copy in sorted order
+  if (cfg.filter && cfg.fusedFilter && s.zSub == 1 && cfg.maxLayerClusters <= 65536) {
+    for (int l = kTrackletLayers - 1; l >= 0; --l) {
+      hipLaunchKernelGGL(kTrackletsSearchScore<B>, dim3(G), dim3(B), 0, stream, s, l, cfg.threshold);
+      done("kTrackletsSearchScore", 1);
+    }
+    launchCommitKept(s, stream);
+    done("kCommitKept", 1);
+  } else {
+    if (s.zSub > 1)
+      hipLaunchKernelGGL((kTrackletsCount<B, true>), dim3(G), dim3(B), 0, stream, s);
+    else
+      hipLaunchKernelGGL((kTrackletsCount<B, false>), dim3(G), dim3(B), 0, stream, s);
+    done("kTrackletsCount", 1);
+    if (cfg.filter) {
+      launchTrackletFilter(s, B, G, cfg.threshold, stream);
+      done("kTrackletsScore", 1);
+    }
   }
   hipLaunchKernelGGL(kScanChunks, dim3(1), dim3(1024), 0, stream, s, (int)kStTracklets, s.W * kTrackletLayers,
                      s.trkSeg, s.tCap, (int)kStatusOverflowTracklets);

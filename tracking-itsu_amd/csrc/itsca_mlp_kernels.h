@@ -18,6 +18,8 @@ struct MlpBuffers {
 // Scores the tracklet candidates left by the count kernel and keeps those
 // whose logit is >= threshold (B = block size of the stage kernels).
 void launchTrackletFilter(const DevState& s, int block, int blocks, float threshold, hipStream_t stream);
+// Moves the kept counts of the filter (trkKept) into trkCnt after the last layer.
+void launchCommitKept(const DevState& s, hipStream_t stream);
 // Labels every tracklet of the last run (on a final road or not) and appends
 // every stride-th one, from `phase`, to a training set at row `base`.
 void launchLabelAndCollect(const DevState& s, int totalTracklets, uint8_t* label, int stride, int phase,

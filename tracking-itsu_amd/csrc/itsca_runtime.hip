@@ -115,6 +115,7 @@ struct itsca_ctx {
   // learned tracklet filter
   bool filterOn = false;
   float threshold = 0.0f;
+  bool filterFused = false;  // itsca_set_filter(ctx, 2, ...)
   int mlpStep = 0;
   int trainN = 0, trainCap = 0;
   DBuf<float> mlpMaster, mlpMom1, mlpMom2, mlpWq, mlpPartial, mlpGrad, mlpLoss, trainScores;
@@ -424,7 +425,7 @@ struct itsca_ctx {
   {
     int maxN = 0;
     for (const Seg& g : clSeg) maxN = std::max(maxN, g.count);
-    LaunchConfig cfg{persistentBlocks, planScratch.p, maxN, filterOn, threshold};
+    LaunchConfig cfg{persistentBlocks, planScratch.p, maxN, filterOn, threshold, filterFused};
     if (profiling && !timerReady) {
       for (auto& e : timer.ev) HIP_TRY(hipEventCreate(&e));
       timerReady = true;
@@ -919,7 +920,7 @@ itsca_status itsca_run_noalloc(itsca_ctx* ctx, const itsca_device_event* ev, int
     HIP_TRY(hipMemcpyAsync(const_cast<int*>(s.itemEvent), ctx->naZero, 4, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(const_cast<int*>(s.recItem), ctx->naZero, 4, hipMemcpyHostToDevice, ctx->stream));
     LaunchConfig cfg{ctx->persistentBlocks, planScratch, maxLayer, ctx->filterOn && ctx->mlpWbf.p != nullptr,
-                     ctx->threshold};
+                     ctx->threshold, ctx->filterFused};
     launchPipeline(s, cfg, ctx->stream, nullptr);
     if (roads_cap > 0) launchRoadsOut(s, 0, roads, roads_cap, ctx->stream);
     HIP_TRY(hipGetLastError());
@@ -1254,8 +1255,10 @@ itsca_status itsca_set_filter(itsca_ctx* ctx, int enable, float threshold)
 {
   if (!ctx) return ITSCA_ERR_ARGUMENT;
   return guarded(ctx, "itsca_set_filter", [&]() -> itsca_status {
+    if (enable < 0 || enable > 2) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_set_filter: enable must be 0, 1 or 2");
     if (enable) ctx->ensureMlp();
     ctx->filterOn = enable != 0;
+    ctx->filterFused = enable == 2;
     ctx->threshold = threshold;
     ctx->dropGraph();
     return ITSCA_OK;

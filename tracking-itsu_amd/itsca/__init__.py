@@ -367,8 +367,11 @@ class GpuTracker:
         assert p.size == MLP_PARAMS
         self._check(self._lib.itsca_mlp_set_params(self.ctx, _fptr(p)), "itsca_mlp_set_params")
 
-    def set_filter(self, on: bool, threshold: float = 0.0) -> None:
-        self._check(self._lib.itsca_set_filter(self.ctx, 1 if on else 0, float(threshold)), "itsca_set_filter")
+    def set_filter(self, on: bool, threshold: float = 0.0, fused: bool = False) -> None:
+        """Learned filter on or off; `fused` scores inside the tracklet search
+        instead of in a kernel of its own after it (same tracklets)."""
+        mode = (2 if fused else 1) if on else 0
+        self._check(self._lib.itsca_set_filter(self.ctx, mode, float(threshold)), "itsca_set_filter")
 
     def train_clear(self, capacity: int) -> None:
         self._check(self._lib.itsca_train_clear(self.ctx, int(capacity)), "itsca_train_clear")
