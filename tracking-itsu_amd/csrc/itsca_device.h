@@ -40,6 +40,9 @@ constexpr int kTrkSlots = 64;  // per cluster (next-layer cluster positions)
 // one in a third of the layer-1 chunks with 4 slots, and none has more than 8.
 constexpr int kCellSlots = 8;
 constexpr int kEdgeSlots = 8;  // per cell (next-layer cell indices)
+// Records allocated past the tracklet capacity (tTP): the cell count reads its
+// candidates in blocks whose last one may run past the end of the range.
+constexpr int kTrackletPad = 8;
 
 // Tracklet search index: inside every z-phi bin of at most kSubMaxBin
 // clusters, the clusters ordered by z sub-bin (zSubIndex, S = DevState::zSub

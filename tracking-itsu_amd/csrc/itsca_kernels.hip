@@ -693,7 +693,18 @@ __device__ __forceinline__ int subStart(uint4 e, int t)
 // candidates whose loads are issued together (more registers, fewer partial
 // blocks).  kSub: the bin-by-bin sub-bin search is compiled in (without it the
 // caller must have src.zSub == 1).
-template <bool kStaged, int kScanBlock, bool kSub, typename F>
+//
+// kWrap: the second phi comparison (|dphi - 2pi| < cut, the window across
+// phi = 0) is evaluated.  Callers may leave it out for a cluster whose phi
+// lies in (kWrapFreeLo, kWrapFreeHi): every candidate phi lies in [0, 2pi]
+// (normalized, Cluster.cxx:35-45), so |phi - phi_j| <= 5.79 and
+// |dphi - 2pi| >= 0.49 > kPhiCut -- that comparison is false for every
+// candidate, and the result is unchanged.
+constexpr float kWrapFreeLo = 0.5f, kWrapFreeHi = 5.7f;
+
+__device__ __forceinline__ bool phiWrapFree(float phi) { return phi > kWrapFreeLo && phi < kWrapFreeHi; }
+
+template <bool kStaged, int kScanBlock, bool kSub, bool kWrap = true, typename F>
 __device__ __forceinline__ void searchRows(const TrackletSearch& t, const int* table, int nNext, int l, float z,
                                            float r, float phi, const CandidateSource& src, F&& emit)
 {
@@ -703,6 +714,7 @@ __device__ __forceinline__ void searchRows(const TrackletSearch& t, const int* t
   auto passes = [&](const float4& q) {
     const float deltaZ = fabsf(tanLambda * (q.x - r) + z - q.y);
     const float deltaPhi = fabsf(phi - q.z);
+    if (!kWrap) return (deltaZ < maxDeltaZ) & (deltaPhi < kPhiCut);
     return (deltaZ < maxDeltaZ) & ((deltaPhi < kPhiCut) | (fabsf(deltaPhi - kTwoPi) < kPhiCut));
   };
   auto loadGlobal = [](const char* g, int j) { return *reinterpret_cast<const float4*>(g + 16u * (unsigned)j); };
@@ -879,36 +891,49 @@ __device__ __forceinline__ int stageUnion(const TrackletSearch& t, int rowRef, c
     atomicMax(&L.uni[4], noCopy);
   }
   __syncthreads();
-  if (threadIdx.x == 0) {  // the copy: rows [dMin, dMax] of the union, z bins [zMin, zMax] of each
-    const int nr = L.uni[1] - L.uni[0] + 1;
-    int rows = 0, ref = 0;
-    if (L.uni[4] == 0 && nr >= 1 && nr <= kStageRows) {
-      ref = ((rowRef + L.uni[0]) % kPhiBins + kPhiBins) % kPhiBins;
-      int total = 0;
-      for (int ri = 0; ri < nr; ++ri) {
-        const int row = (ref + ri) % kPhiBins;
-        int lo = 0, hi = 0;
-        if (L.uni[2] <= L.uni[3]) {
-          lo = table[binIndex(L.uni[2], row)];
-          hi = max(lo, min(table[binIndex(L.uni[3], row) + 1], nNext));
-        }
-        L.rowLo[ri] = lo;
-        L.rowStart[ri] = total;
-        L.rowOff[ri] = total - lo;
-        total += hi - lo;
-      }
-      L.rowStart[nr] = total;
-      if (total <= cap) rows = nr;
+  // The copy: rows [dMin, dMax] of the union, z bins [zMin, zMax] of each.
+  // The first wavefront reads the rows' table entries in parallel (lane ri:
+  // row ri) and numbers them with a shuffle scan.
+  const int nr = L.uni[1] - L.uni[0] + 1;
+  const bool fitsRows = L.uni[4] == 0 && nr >= 1 && nr <= kStageRows;
+  if (threadIdx.x < 64 && fitsRows) {
+    const int ri = threadIdx.x;
+    const int ref = ((rowRef + L.uni[0]) % kPhiBins + kPhiBins) % kPhiBins;
+    int lo = 0, hi = 0;
+    if (ri < nr && L.uni[2] <= L.uni[3]) {
+      const int row = (ref + ri) % kPhiBins;
+      lo = table[binIndex(L.uni[2], row)];
+      hi = max(lo, min(table[binIndex(L.uni[3], row) + 1], nNext));
     }
-    L.copyRows = rows;
-    L.copyRef = ref;
+    const int len = hi - lo;
+    int inc = len;  // inclusive scan over the rows (at most kStageRows lanes carry a length)
+#pragma unroll
+    for (int d = 1; d < kStageRows; d <<= 1) {
+      const int t = __shfl_up(inc, d, 64);
+      if (ri >= d) inc += t;
+    }
+    const int start = inc - len;
+    if (ri < nr) {
+      L.rowLo[ri] = lo;
+      L.rowStart[ri] = start;
+      L.rowOff[ri] = start - lo;
+    }
+    const int total = __shfl(inc, nr - 1, 64);
+    if (ri == 0) {
+      L.rowStart[nr] = total;
+      L.copyRows = total <= cap ? nr : 0;
+      L.copyRef = ref;
+    }
+  } else if (threadIdx.x == 0 && !fitsRows) {
+    L.copyRows = 0;
+    L.copyRef = 0;
   }
   __syncthreads();
   const int nrows = L.copyRows;
   if (nrows > 0) {
     const int total = L.rowStart[nrows];
+    int ri = 0;  // k only grows, so the row index only moves forward
     for (int k = threadIdx.x; k < total; k += blockDim.x) {
-      int ri = 0;
       while (ri + 1 < nrows && L.rowStart[ri + 1] <= k) ++ri;
       stage[k] = src[L.rowLo[ri] + (k - L.rowStart[ri])];
     }
@@ -962,6 +987,8 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) kTr
     const float4* copySrc = s.zSub > 1 ? s.crzpS : s.crzp;
     const int nrows = stageUnion(t, chunkRowRef(s.cbin + rcur.base, first), table, next.count, copySrc + rnext.base,
                                  stage, kStageCap, L);
+    // every cluster of the wavefront away from phi = 0: the wrap comparison is left out
+    const bool waveWrapFree = __ballot(i < cur.count && !phiWrapFree(phi)) == 0ull;
     int cnt = 0;
     if (i < cur.count) {
       int* slots = s.trkCand + (size_t)p * kTrkSlots;
@@ -976,10 +1003,14 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) kTr
                                 stage,
                                 L.rowOff,
                                 L.copyRef};
-      if (nrows > 0 && (t.sub || s.zSub == 1))  // with sub-bins the copy serves only the bin-by-bin search
-        searchRows<true, 8, kSub>(t, table, next.count, l, z, r, phi, src, emit);
-      else
+      if (nrows > 0 && (t.sub || s.zSub == 1)) {  // with sub-bins the copy serves only the bin-by-bin search
+        if (waveWrapFree)
+          searchRows<true, 8, kSub, false>(t, table, next.count, l, z, r, phi, src, emit);
+        else
+          searchRows<true, 8, kSub, true>(t, table, next.count, l, z, r, phi, src, emit);
+      } else {
         searchRows<false, 8, kSub>(t, table, next.count, l, z, r, phi, src, emit);
+      }
       s.trkCnt[p] = cnt;
     }
     const int total = blockSum(cnt, scratch);  // its barriers also end this chunk's use of the copy
@@ -1380,7 +1411,20 @@ __device__ __forceinline__ bool cellFit(const DevState& s, const CellSeed& c, in
 // events 0.2 % of the layer-1 tracklets have 9-13 survivors.
 constexpr int kCellList = 16;
 constexpr int kFitBatch = 4;  // survivors whose fit loads are issued together
-constexpr int kCandBlock = 4;  // candidates whose loads are issued together
+constexpr int kCandBlock = 4;  // candidates whose loads are issued together (two 16-byte loads)
+static_assert(kCandBlock % 2 == 0 && kCandBlock - 1 <= kTrackletPad, "the last block reads past the range");
+
+// Two consecutive (tanLambda, phi) records in one 16-byte load (8-byte
+// aligned).  The last block of a candidate range reads up to kCandBlock - 1
+// records past it -- masked out, and inside the kTrackletPad records that
+// every tracklet array is allocated beyond its capacity.
+__device__ __forceinline__ void loadPair(const float2* p, float2& a, float2& b)
+{
+  typedef float Float4A8 __attribute__((ext_vector_type(4), aligned(8)));
+  const Float4A8 v = *reinterpret_cast<const Float4A8*>(p);
+  a = make_float2(v.x, v.y);
+  b = make_float2(v.z, v.w);
+}
 
 // Count: the cheap cuts run over every candidate; survivors (usually 0-2) are
 // fitted in a second loop, so the expensive fit does not sit inside the
@@ -1449,7 +1493,7 @@ __global__ void __launch_bounds__(B) kCellsCount(DevState s)
       for (int ib = c.uFirst; ib < uEnd; ib += kCandBlock) {
         float2 up[kCandBlock];
 #pragma unroll
-        for (int k = 0; k < kCandBlock; ++k) up[k] = s.tTP[c.tnBase + min(ib + k, uEnd - 1)];
+        for (int k = 0; k < kCandBlock; k += 2) loadPair(s.tTP + c.tnBase + ib + k, up[k], up[k + 1]);
         uint32_t mask = 0;
 #pragma unroll
         for (int k = 0; k < kCandBlock; ++k) mask |= ((cellCheapCuts(c, up[k]) & (ib + k < uEnd)) ? 1u : 0u) << k;

@@ -259,7 +259,7 @@ struct itsca_ctx {
     HIP_TRY(table.reserve((size_t)W * kLayers * kTableSize));
     HIP_TRY(segGeo.reserve((size_t)std::max(W, 1) * kLayers));
     HIP_TRY(tFS.reserve(tCap));
-    HIP_TRY(tTP.reserve(tCap));
+    HIP_TRY(tTP.reserve((size_t)tCap + kTrackletPad));
     HIP_TRY(cellCnt.reserve(tCap));
     HIP_TRY(cellOff.reserve(tCap));
     HIP_TRY(cellCand.reserve((size_t)tCap * kCellSlots));
@@ -838,7 +838,7 @@ bool carveNoAlloc(unsigned char* base, size_t bytes, int n, DevState& s, int*& p
   s.chunkOff = static_cast<int*>(take((size_t)kNumStages * s.maxChunks, 4));
   s.chunkInfo = static_cast<int4*>(take((size_t)kNumStages * s.maxChunks, sizeof(int4)));
   s.tFS = static_cast<int2*>(take(s.tCap, sizeof(int2)));
-  s.tTP = static_cast<float2*>(take(s.tCap, sizeof(float2)));
+  s.tTP = static_cast<float2*>(take((size_t)s.tCap + kTrackletPad, sizeof(float2)));
   s.cellCnt = static_cast<int*>(take(s.tCap, 4));
   s.cellOff = static_cast<int*>(take(s.tCap, 4));
   s.cellCand = static_cast<int*>(take((size_t)s.tCap * kCellSlots, 4));
