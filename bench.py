@@ -153,6 +153,11 @@ def cpu_baseline(itsca, tmp, n_events, seed, per_core):
             "sample": "%d central events, single pass, one tracker per thread" % len(evs)}
 
 
+def tuning_dict(t):
+    return {"block_size": t.block_size, "persistent_blocks": t.persistent_blocks, "z_subbins": t.z_subbins,
+            "phi_subrows": t.phi_subrows}
+
+
 def time_form(tracker, steps, warmup):
     """Events are already resident; the pipeline is replayed as a HIP graph."""
     tracker.set_graph(False)
@@ -279,8 +284,7 @@ def single_events(itsca, steps=50):
         t.close()
         out[key] = {"ms_per_event": round(1e3 * s / steps, 4), "events_per_s": round(steps / s, 1),
                     "form": "plain, one event per launch, hipGraph replay", "seed": seed,
-                    "tuning": {"block_size": tuning.block_size, "persistent_blocks": tuning.persistent_blocks,
-                               "z_subbins": tuning.z_subbins}}
+                    "tuning": tuning_dict(tuning)}
     return out
 
 
@@ -303,11 +307,40 @@ def mixed_batch(itsca, steps, tuning_fixed=None):
     out = {"workload": "256 events, multiplicity log-uniform in [50, 8000] tracks (seeds 1000..1255), one launch",
            "form": "plain, hipGraph replay", "events_per_s": round(256 * steps / s, 2),
            "ms_per_step": round(1e3 * s / steps, 4),
-           "tuning": {"block_size": tuning.block_size, "persistent_blocks": tuning.persistent_blocks,
-                      "z_subbins": tuning.z_subbins},
+           "tuning": tuning_dict(tuning),
            "kernel_ms": {k: round(v, 4) for k, v in sorted(kernel_ms.items(), key=lambda kv: -kv[1])},
            "stage_bandwidth": stage_bandwidth(counts, kernel_ms),
            "bytes_alg_per_step": sum(bytes_alg(c) for c in counts)}
+    t.close()
+    return out
+
+
+BINNINGS = [(1, 1), (2, 1), (4, 1), (8, 1), (1, 2), (2, 2), (4, 2), (1, 4), (2, 4), (3, 4)]
+
+
+def binning_sweep(itsca, tracker, threshold, steps=10):
+    """The search-index binnings (z sub-bins x phi sub-rows) on the uploaded
+    central batch, both forms, and on the mixed-multiplicity batch (plain), at
+    block 256 and 16 workgroups per CU: ms per step (graph replay) and the
+    tracklet count's ms (HIP events)."""
+    def sweep(t, forms):
+        rows = []
+        for form in forms:
+            if form != "mixed":
+                t.set_filter(form == "learned", threshold)
+            for zs, ps in BINNINGS:
+                t.set_tuning(256, 0, zs, ps)
+                s = time_form(t, steps, 2)
+                km, _, _ = kernel_profile(t, 3)
+                rows.append({"form": form, "z_subbins": zs, "phi_subrows": ps, "ms_per_step": round(1e3 * s / steps, 4),
+                             "kTrackletsCount_ms": round(km.get("kTrackletsCount", 0.0), 4),
+                             "kPrep_ms": round(km.get("kPrep", 0.0), 4)})
+        return rows
+    out = {"central": sweep(tracker, ("plain", "learned"))}
+    evs = itsca.generate_events(3, 1000, 256)
+    t = itsca.GpuTracker(device=0)
+    t.set_events(evs)
+    out["mixed"] = sweep(t, ("mixed",))
     t.close()
     return out
 
@@ -353,13 +386,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed-multiplicity and pile-up batches")
     ap.add_argument("--weights", default=None, help="learned filter weights (default: the shipped file)")
-    ap.add_argument("--tuning-plain", default=None, metavar="B,P,Z",
-                    help="fixed block size, persistent workgroups and z sub-bins of the plain form instead of the "
-                         "autotuner's choice (for profiler runs)")
-    ap.add_argument("--tuning-learned", default=None, metavar="B,P,Z", help="the same for the learned form")
+    ap.add_argument("--tuning-plain", default=None, metavar="B,P,Z[,F]",
+                    help="fixed block size, persistent workgroups, z sub-bins and phi sub-rows of the plain form "
+                         "instead of the autotuner's choice (for profiler runs)")
+    ap.add_argument("--tuning-learned", default=None, metavar="B,P,Z[,F]", help="the same for the learned form")
     ap.add_argument("--only", choices=["central", "mixed"], default=None,
                     help="run one workload alone (for profiler runs): the central batch (both forms) or the "
                          "mixed-multiplicity batch")
+    ap.add_argument("--binning-sweep", action="store_true",
+                    help="also time every search-index binning (z sub-bins x phi sub-rows) on the central and the "
+                         "mixed batches")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy kernel of this many bytes (a reference dispatch of "
                          "known traffic for hardware-counter runs)")
@@ -403,9 +439,9 @@ def main():
             t = tracker.tuning()
         else:
             t, _ = tracker.autotune(2)
-        print("# %s: tuning block %d, %d workgroups, %d z sub-bins" % (form, t.block_size, t.persistent_blocks,
-                                                                       t.z_subbins), file=sys.stderr, flush=True)
-        return {"block_size": t.block_size, "persistent_blocks": t.persistent_blocks, "z_subbins": t.z_subbins}
+        print("# %s: tuning block %d, %d workgroups, %d z sub-bins, %d phi sub-rows"
+              % (form, t.block_size, t.persistent_blocks, t.z_subbins, t.phi_subrows), file=sys.stderr, flush=True)
+        return tuning_dict(t)
 
     tracker.set_filter(False)
     plain_tuning = tuned("plain")
@@ -480,6 +516,9 @@ def main():
         "counts_event0": {"tracklets": list(counts[0].tracklets), "cells": list(counts[0].cells),
                           "roads": counts[0].roads},
     }
+    if args.binning_sweep:
+        line["binning_sweep"] = binning_sweep(itsca, tracker, threshold)
+        print("# binning_sweep done", file=sys.stderr, flush=True)
     tracker.close()
     if args.only == "central":
         print(json.dumps(line), flush=True)

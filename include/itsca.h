@@ -134,19 +134,26 @@ itsca_status itsca_download(itsca_ctx* ctx, int item, int what, int layer, void*
  *   persistent_blocks: workgroups of each persistent stage kernel (0 = 8 per CU)
  *   z_subbins:         z sub-bins per z bin of the search index, 1..8 (0 = 8);
  *                      the search tests 20 * z_subbins z slices per layer instead
- *                      of the reference's 20 bins.  The reference's own 20 x 20
- *                      binning is kept for the results: it decides which
- *                      candidate its inclusive bound adds (Tracker.cxx:84-89). */
+ *                      of the reference's 20 bins
+ *   phi_subrows:       phi sub-rows per phi row of the search index, 1, 2 or 4
+ *                      (0 = 1); the search sees 20 * phi_subrows phi slices per
+ *                      layer instead of the reference's 20 rows.
+ *                      z_subbins * phi_subrows is at most 15.
+ * The reference's own 20 x 20 binning (Constants.h:77-79) is kept for the
+ * results: it decides which candidate its inclusive bound adds
+ * (Tracker.cxx:84-89), so the sub-bins only index the search. */
 typedef struct itsca_tuning {
   int block_size;
   int persistent_blocks;
   int z_subbins;
+  int phi_subrows;
 } itsca_tuning;
 itsca_status itsca_set_tuning(itsca_ctx* ctx, const itsca_tuning* tuning);
 itsca_status itsca_get_tuning(itsca_ctx* ctx, itsca_tuning* tuning);
 
 /* Autotuner: times every tuning (block size 64/128/256 x 4/8/16 workgroups
- * per CU x 1/2/4/8 z sub-bins) over `repeats` runs of the uploaded batch,
+ * per CU x search-index binnings: z sub-bins 1/2/4/8 x phi sub-rows 1/2/4,
+ * at most 15 sub-bins per bin) over `repeats` runs of the uploaded batch,
  * keeps the fastest, and leaves the results of one run with it.  *best_ms gets
  * its time per run. */
 itsca_status itsca_autotune(itsca_ctx* ctx, int repeats, itsca_tuning* chosen, float* best_ms);

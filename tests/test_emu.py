@@ -2,7 +2,8 @@
 compiled for the host against the HIP shim of tests/emu/ (one fiber per GPU
 thread, the wavefront exchanges and the MFMA instructions emulated), with
 AddressSanitizer.  Checks the kernels' logic and bounds without a GPU: golden
-fixtures bit for bit under two tunings, the learned filter's paths (scored
+fixtures bit for bit under several tunings (launch shapes, z sub-bins and phi
+sub-rows of the search index), the learned filter's paths (scored
 after the search and inside it), an event
 of 1831 clusters per layer and an event whose tracklets overflow the count's
 slots, against the oracle."""
@@ -76,37 +77,46 @@ for name in ("small_s1", "sparse_s6", "small_s5_nolayer3"):
         print("mfma forward vs float64 ok", flush=True)
     t.close()
     print(name, "ok", flush=True)
-# another launch shape and an odd number of z sub-bins of the search index
+# other launch shapes and search-index binnings: an odd number of z sub-bins,
+# phi sub-rows alone and together with z sub-bins
 p = cases["pileup2_s7"]
 ev = itsca.load_events(p)[0]
-t = itsca.GpuTracker(persistent_blocks=4, lib=lib)
-t.set_tuning(128, 4, 3)
-t.set_events([ev])
-t.run()
-for v in range(len(ev.vertices)):
-    assert itsca.dump_item(t, v, ev.vertices[v]) == fixture("pileup2_s7", "stable", v), v
-t.close()
-print("pileup2_s7 block 128, 3 z sub-bins ok", flush=True)
+for tuning in ((128, 4, 3, 1), (64, 4, 1, 4), (256, 4, 3, 4)):
+    t = itsca.GpuTracker(persistent_blocks=4, lib=lib)
+    t.set_tuning(*tuning)
+    t.set_events([ev])
+    t.run()
+    for v in range(len(ev.vertices)):
+        assert itsca.dump_item(t, v, ev.vertices[v]) == fixture("pileup2_s7", "stable", v), (tuning, v)
+    t.close()
+print("pileup2_s7 under three search-index binnings ok", flush=True)
 # 1831 clusters per layer: two tiles of the stable sort, dense bins searched by z sub-bin, against the oracle
 import pyoracle
 ev = itsca.generate_events(3, 3006, 1)[0]
 assert int(np.diff(ev.layer_offsets).min()) > 1024
-t = itsca.GpuTracker(persistent_blocks=8, lib=lib)
-t.set_tuning(256, 8, 8)
-t.set_events([ev])
-t.run()
-assert itsca.dump_item(t, 0, ev.vertices[0]) == pyoracle.OracleVertex(ev, 0, stable=True).dump()
-t.close()
-print("mixed event 3006 vs oracle ok", flush=True)
+want = pyoracle.OracleVertex(ev, 0, stable=True).dump()
+for tuning in ((256, 8, 8, 1), (256, 8, 2, 4)):
+    t = itsca.GpuTracker(persistent_blocks=8, lib=lib)
+    t.set_tuning(*tuning)
+    t.set_events([ev])
+    t.run()
+    assert itsca.dump_item(t, 0, ev.vertices[0]) == want, tuning
+    t.close()
+print("mixed event 3006 vs oracle ok (z sub-bins; z sub-bins and phi sub-rows)", flush=True)
 # a cluster with more tracklets than the count's slots: the fill's own search, against the oracle
 from golden_cases import dense_spot_event
 ev = dense_spot_event()
+want = pyoracle.OracleVertex(ev, 0, stable=True).dump()
 t = itsca.GpuTracker(persistent_blocks=4, lib=lib)
+t.set_tuning(256, 4, 4, 2)  # the fill's own search through the search index with phi sub-rows
 t.set_events([ev])
+t.run()
+assert itsca.dump_item(t, 0, ev.vertices[0]) == want
+t.set_tuning(256, 4, 8, 1)
 t.run()
 n = t.counts(0)
 assert np.bincount(t.download(0, itsca.TRACKLETS, 0, n.tracklets[0], 4)[:, 0]).max() > itsca.TRACKLET_SLOTS
-assert itsca.dump_item(t, 0, ev.vertices[0]) == pyoracle.OracleVertex(ev, 0, stable=True).dump()
+assert itsca.dump_item(t, 0, ev.vertices[0]) == want
 t.mlp_init(4)
 t.set_filter(True, 0.0, fused=True)
 t.run()
@@ -193,6 +203,18 @@ except itsca.ItscaError as e:
     assert "non-finite" in str(e), e
 t.close()
 print("non-finite input refused ok", flush=True)
+# search-index binnings outside the 16-byte sub-bin entries are refused
+t = itsca.GpuTracker(persistent_blocks=4, lib=lib)
+for bad_tuning in ((256, 0, 8, 2), (256, 0, 4, 4), (256, 0, 2, 3)):
+    try:
+        t.set_tuning(*bad_tuning)
+        raise SystemExit("tuning %r was accepted" % (bad_tuning,))
+    except itsca.ItscaError as e:
+        assert "phi_subrows" in str(e), e
+t.set_tuning(256, 0, 3, 4)
+assert (t.tuning().z_subbins, t.tuning().phi_subrows) == (3, 4)
+t.close()
+print("search-index binning bounds ok", flush=True)
 # roads per root cell bounded at 3 in this build: events within the bound give
 # the fixture, an event with a root of 4 roads fails with a capacity error
 cap = itsca.bind_hip(ctypes.CDLL(os.path.join(ROOT, "tests", "emu", "libitsca_emu_cap3.so")))

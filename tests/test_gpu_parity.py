@@ -236,12 +236,13 @@ def test_capacity_growth_keeps_results(golden_dir):
     t.close()
 
 
-@pytest.mark.parametrize("block,zsub", [(64, 1), (128, 2), (256, 8), (256, 3)])
-def test_launch_shapes_do_not_change_results(tracker, golden_dir, block, zsub):
-    say(f"gpu tuning block={block} z_subbins={zsub}")
+@pytest.mark.parametrize("block,zsub,phisub", [(64, 1, 1), (128, 2, 1), (256, 8, 1), (256, 3, 1), (64, 1, 4),
+                                               (128, 3, 4), (256, 4, 2)])
+def test_launch_shapes_do_not_change_results(tracker, golden_dir, block, zsub, phisub):
+    say(f"gpu tuning block={block} z_subbins={zsub} phi_subrows={phisub}")
     evs = [itsca.load_events(os.path.join(golden_dir, n + ".events.txt"))[0] for n in ("small_s3", "sparse_s6")]
     t = itsca.GpuTracker(device=0)
-    t.set_tuning(block, 0, zsub)
+    t.set_tuning(block, 0, zsub, phisub)
     t.set_events(evs)
     t.run()
     for i, name in enumerate(("small_s3", "sparse_s6")):
@@ -249,19 +250,20 @@ def test_launch_shapes_do_not_change_results(tracker, golden_dir, block, zsub):
     t.close()
 
 
-@pytest.mark.parametrize("zsub", [1, 2, 4, 8])
-def test_search_index_binning_keeps_central_event(zsub):
-    """The z sub-bins of the search index only decide which candidates are
-    skipped untested: the central event's full dump stays bit-exact."""
-    say(f"gpu central event z_subbins={zsub}")
+@pytest.mark.parametrize("zsub,phisub", [(1, 1), (2, 1), (4, 1), (8, 1), (1, 2), (1, 4), (2, 4), (4, 2), (3, 4)])
+def test_search_index_binning_keeps_central_event(zsub, phisub):
+    """The z sub-bins and phi sub-rows of the search index only decide which
+    candidates are skipped untested: the central event's full dump stays
+    bit-exact."""
+    say(f"gpu central event z_subbins={zsub} phi_subrows={phisub}")
     info = central()
     with tempfile.TemporaryDirectory() as tmp:
         p = os.path.join(tmp, "c.txt")
         itsca.write_events(info["preset"], info["seed"], 1, p)
         ev = itsca.load_events(p)[0]
     t = itsca.GpuTracker(device=0)
-    t.set_tuning(256, 0, zsub)
-    assert t.tuning().z_subbins == zsub
+    t.set_tuning(256, 0, zsub, phisub)
+    assert (t.tuning().z_subbins, t.tuning().phi_subrows) == (zsub, phisub)
     t.set_events([ev])
     t.run()
     text = itsca.dump_item(t, 0, ev.vertices[0])

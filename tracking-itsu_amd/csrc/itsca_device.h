@@ -45,9 +45,14 @@ constexpr int kEdgeSlots = 8;  // per cell (next-layer cell indices)
 constexpr int kTrackletPad = 8;
 
 // Tracklet search index: inside every z-phi bin of at most kSubMaxBin
-// clusters, the clusters ordered by z sub-bin (zSubIndex, S = DevState::zSub
-// parts per bin, at most kMaxZSub), with the start of each sub-bin as a byte.
+// clusters, the clusters ordered by sub-bin -- phi sub-row (phiSubIndex,
+// P = DevState::phiSub parts per row, at most kMaxPhiSub), then z sub-bin
+// (zSubIndex, S = DevState::zSub parts per bin, at most kMaxZSub): sub-bin
+// phiSub * S + zSub -- with the start of each sub-bin as a byte of a 16-byte
+// entry, so S * P <= kMaxSubBins.
 constexpr int kMaxZSub = 8;
+constexpr int kMaxPhiSub = 4;
+constexpr int kMaxSubBins = 15;
 constexpr int kSubMaxBin = 128;
 constexpr int kDefaultZSub = 8;
 
@@ -96,8 +101,9 @@ struct DevState {
   float *cz, *cphi, *cr;
   float4* crzp;  // (r, z, phi, 0) per sorted cluster: one load per tracklet candidate
   float4* crzpS;  // the same records, z-sub-bin order inside each bin; w = offset in the bin (int bits)
-  uint4* subTab;  // per (item, layer) and bin: byte t = offset of sub-bin t in crzpS (t <= zSub)
+  uint4* subTab;  // per (item, layer) and bin: byte t = offset of sub-bin t in crzpS (t <= zSub * phiSub)
   int zSub;       // z sub-bins per bin of the search index (1..kMaxZSub)
+  int phiSub;     // phi sub-rows per row of the search index (1..kMaxPhiSub)
   float4* segGeo;  // per (item, layer): (min r, max r, 1 if every z bin index lies in [0, kZBins) else 0, 0)
   int *cid, *cmc, *cbin;
   float *sPhi, *sR;  // prep scratch, original order

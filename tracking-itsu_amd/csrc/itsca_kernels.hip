@@ -348,26 +348,30 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
     }
   }
   // The tracklet search index (no counterpart in the reference; only with
-  // more than one z sub-bin per bin).  Inside every bin of at most kSubMaxBin
-  // clusters the clusters are grouped by z sub-bin (a counting sort with LDS
-  // counters; the order inside a sub-bin is arbitrary and never matters) and
-  // written to crzpS with their offset in the bin; subTab gets the start of
-  // every sub-bin.  Larger bins and the phi == 2pi slot keep the sorted order.
-  // Every sorted position p is visited by the thread that wrote its record.
-  const int S = s.zSub;
-  if (S <= 1) return;
+  // more than one sub-bin per bin).  Inside every bin of at most kSubMaxBin
+  // clusters the clusters are grouped by sub-bin (phi sub-row, then z
+  // sub-bin; a counting sort with LDS counters; the order inside a sub-bin is
+  // arbitrary and never matters) and written to crzpS with their offset in the
+  // bin; subTab gets the start of every sub-bin.  Larger bins and the
+  // phi == 2pi slot keep the sorted order.  Every sorted position p is visited
+  // by the thread that wrote its record.
+  const int S = s.zSub, P = s.phiSub, NS = S * P;
+  if (NS <= 1) return;
   __threadfence_block();
   __syncthreads();
-  int* subCnt = lBucket;  // kBins * S counters, then cursors
-  for (int k = threadIdx.x; k < kBins * S; k += blockDim.x) subCnt[k] = 0;
+  int* subCnt = lBucket;  // kBins * NS counters, then cursors
+  for (int k = threadIdx.x; k < kBins * NS; k += blockDim.x) subCnt[k] = 0;
   __syncthreads();
   auto smallBin = [&](int b) { return b >= 0 && b < kBins && tab[b + 1] - tab[b] <= kSubMaxBin; };
+  auto subOf = [&](int b, int p) {  // sub-bin of sorted position p in bin b (row b / kPhiBins, z bin b % kPhiBins)
+    return phiSubIndex(s.cphi[base + p], b / kPhiBins, P) * S + zSubIndex(l, s.cz[base + p], b % kPhiBins, S);
+  };
   for (int p = threadIdx.x; p < n; p += blockDim.x) {
     const int b = s.cbin[base + p];
-    if (smallBin(b)) atomicAdd(&subCnt[b * S + zSubIndex(l, s.cz[base + p], b % kZBins, S)], 1);
+    if (smallBin(b)) atomicAdd(&subCnt[b * NS + subOf(b, p)], 1);
   }
   __syncthreads();
-  // Byte t of a bin's entry: members with sub-bin < t (t = S and above: all).
+  // Byte t of a bin's entry: members with sub-bin < t (t = NS and above: all).
   uint4* subTab = s.subTab + (size_t)(w * kLayers + l) * kBins;
   for (int b = threadIdx.x; b < kBins; b += blockDim.x) {
     uint32_t e[4] = {0u, 0u, 0u, 0u};
@@ -376,9 +380,9 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
 #pragma unroll
       for (int t = 0; t < 16; ++t) {
         e[t >> 2] |= (uint32_t)acc << (8 * (t & 3));
-        if (t < S) {
-          const int c = subCnt[b * S + t];
-          subCnt[b * S + t] = acc;
+        if (t < NS) {
+          const int c = subCnt[b * NS + t];
+          subCnt[b * NS + t] = acc;
           acc += c;
         }
       }
@@ -390,7 +394,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
     const int b = s.cbin[base + p];
     const int lo = b < 0 ? 0 : (b < kBins ? tab[b] : tab[kBins]);
     int q = p;
-    if (smallBin(b)) q = lo + atomicAdd(&subCnt[b * S + zSubIndex(l, s.cz[base + p], b % kZBins, S)], 1);
+    if (smallBin(b)) q = lo + atomicAdd(&subCnt[b * NS + subOf(b, p)], 1);
     const float4 rec = s.crzp[base + p];
     s.crzpS[base + q] = make_float4(rec.x, rec.y, rec.z, __int_as_float(p - lo));
   }
@@ -661,7 +665,7 @@ struct CandidateSource {
   const char* global;
   const char* globalS;
   const uint4* subTab;
-  int zSub;
+  int zSub, phiSub;
   const float4* lds;
   const int* off;
   int rowRef;
@@ -674,25 +678,54 @@ __device__ __forceinline__ int subStart(uint4 e, int t)
   return (int)((d >> (8 * (t & 3))) & 0xffu);
 }
 
+// The phi sub-rows [lo, hi] of window row `row` (P sub-rows per row) that can
+// hold a candidate passing the phi cut of a cluster at `phi`; lo > hi: none.
+// A candidate passes when |phi - phi_j| < kPhiCut or ||phi - phi_j| - 2pi| <
+// kPhiCut, so phi_j lies within kPhiCut (plus kPhiSkipMargin, far above the
+// rounding of those differences at |phi| < 7) of phi + 2pi k for some k in
+// {-1, 0, 1}.  The candidates of row `row` have phi_j in the row's range,
+// widened here by 0.01 rad; for every k whose interval meets it, phiSubIndex
+// (monotone in phi) bounds the candidate's sub-row by those of the interval's
+// ends.
+constexpr float kPhiSkipMargin = 1.0e-3f;
+
+__device__ __forceinline__ void phiSubRange(float phi, int row, int P, int& lo, int& hi)
+{
+  constexpr float kRowWidth = kTwoPi / (float)kPhiBins;
+  const float rowLo = (float)row * kRowWidth - 0.01f, rowHi = (float)(row + 1) * kRowWidth + 0.01f;
+  lo = P;
+  hi = -1;
+#pragma unroll
+  for (int k = -1; k <= 1; ++k) {
+    const float a = phi - (kPhiCut + kPhiSkipMargin) + (float)k * kTwoPi;
+    const float b = phi + (kPhiCut + kPhiSkipMargin) + (float)k * kTwoPi;
+    if (b >= rowLo && a <= rowHi) {
+      lo = min(lo, phiSubIndex(a, row, P));
+      hi = max(hi, phiSubIndex(b, row, P));
+    }
+  }
+}
+
 // Visits every accepted candidate of the search, in the reference's order
 // (window row order, then ascending candidate position).
 //
-// With the z range known (t.sub) and more than one z sub-bin per bin, each
-// row is searched bin by bin over the
-// bins [zFirst, zLast].  In a bin of at most kSubMaxBin clusters only the z
-// sub-bins [zSubIndex(zLo), zSubIndex(zHi)] of the search index are tested: a
-// candidate that passes has z in [zLo, zHi], hence (zSubIndex being monotone)
-// a sub-bin in that range.  The accepted ones are collected as bits of their
-// offsets in the bin and emitted in ascending order, which is the sorted
-// order.  Larger bins keep the sorted order in the search index and are
-// tested whole.  Otherwise the row's bins [zFirst, zLast] are tested in
-// sorted order, as one range.  The reference's extra candidate (the inclusive
+// With the z range known (t.sub) and more than one sub-bin per bin, each row
+// is searched bin by bin over the bins [zFirst, zLast].  In a bin of at most
+// kSubMaxBin clusters only the sub-bins whose z sub-bin lies in
+// [zSubIndex(zLo), zSubIndex(zHi)] and whose phi sub-row lies in the row's
+// phiSubRange are tested: a candidate that passes has z in [zLo, zHi] and phi
+// in that range, hence (both indices being monotone) a sub-bin among them.
+// The accepted ones are collected as bits of their offsets in the bin and
+// emitted in ascending order, which is the sorted order.  Larger bins keep the
+// sorted order in the search index and are tested whole; a row whose phi
+// range holds no sub-row is skipped.  Otherwise the row's bins [zFirst, zLast]
+// are tested in sorted order, as one range.  The reference's extra candidate (the inclusive
 // upper bound) is tested last, from the sorted records in global memory.
 // kStaged: the records of the tested bins are in the LDS copy (search-index
-// order with more than one z sub-bin, else sorted order).  kScanBlock: window
+// order with more than one sub-bin, else sorted order).  kScanBlock: window
 // candidates whose loads are issued together (more registers, fewer partial
 // blocks).  kSub: the bin-by-bin sub-bin search is compiled in (without it the
-// caller must have src.zSub == 1).
+// caller must have src.zSub * src.phiSub == 1).
 //
 // kWrap: the second phi comparison (|dphi - 2pi| < cut, the window across
 // phi = 0) is evaluated.  Callers may leave it out for a cluster whose phi
@@ -770,19 +803,22 @@ __device__ __forceinline__ void searchRows(const TrackletSearch& t, const int* t
       if (ri < 0) ri += kPhiBins;
       off = src.off[ri];
     }
-    if (kSub && t.sub && src.zSub > 1) {
+    if (kSub && t.sub && src.zSub * src.phiSub > 1) {
       auto loadS = [&](int j) { return kStaged ? src.lds[j + off] : loadGlobal(src.globalS, j); };
-      for (int zb = t.zFirst; zb <= t.zLast; ++zb) {  // table positions clamped like maxBin
+      const int S = src.zSub, P = src.phiSub;
+      int psLo = 0, psHi = 0;
+      if (P > 1) phiSubRange(phi, row, P, psLo, psHi);
+      for (int zb = t.zFirst; psLo <= psHi && zb <= t.zLast; ++zb) {  // table positions clamped like maxBin
         const int b = min(firstBin + (zb - win.x), maxBin);
         const int bS = table[b];
         const int bE = min(table[min(b + 1, maxBin)], nNext);
         if (bE <= bS) continue;
         if (bE - bS <= kSubMaxBin) {
           const uint4 e = src.subTab[b];
-          const int j0 = bS + subStart(e, zSubIndex(l + 1, t.zLo, zb, src.zSub));
-          const int j1 = bS + subStart(e, zSubIndex(l + 1, t.zHi, zb, src.zSub) + 1);
+          const int zsLo = zSubIndex(l + 1, t.zLo, zb, S), zsHi = zSubIndex(l + 1, t.zHi, zb, S);
           bits0 = bits1 = 0;
-          scan(j0, j1, loadS, std::true_type{});
+          for (int ps = psLo; ps <= psHi; ++ps)
+            scan(bS + subStart(e, ps * S + zsLo), bS + subStart(e, ps * S + zsHi + 1), loadS, std::true_type{});
           while (bits0 != 0ull) {
             const int k = __builtin_ctzll(bits0);
             bits0 &= bits0 - 1ull;
@@ -824,6 +860,7 @@ __device__ __forceinline__ void forEachTracklet(const DevState& s, int rw, int l
                             reinterpret_cast<const char*>(s.crzpS + baseNext),
                             s.subTab + (size_t)(rw * kLayers + l + 1) * kBins,
                             s.zSub,
+                            s.phiSub,
                             nullptr,
                             nullptr,
                             0};
@@ -952,8 +989,9 @@ __device__ __forceinline__ int chunkRowRef(const int* cbin, int first)
 // Count: each cluster's accepted candidates, in emission order, are kept in
 // kTrkSlots scratch slots so that the fill does not repeat the window search.
 // Five waves per SIMD (the LDS copy allows five workgroups of 256 per CU).
-// kSub: launched with more than one z sub-bin; the instance for one z sub-bin
-// leaves the sub-bin search out and fits five waves without spills.
+// kSub: launched with a search index (more than one sub-bin per bin); the
+// instance without one leaves the sub-bin search out and fits five waves
+// without spills.
 template <int B, bool kSub>
 __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) kTrackletsCount(DevState s)
 {
@@ -984,7 +1022,8 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) kTr
       phi = s.cphi[rcur.base + i];
       t = trackletSearch(s.segGeo[rw * kLayers + l + 1], z, r, phi, s.itemVertex[3 * w + 2], l);
     }
-    const float4* copySrc = s.zSub > 1 ? s.crzpS : s.crzp;
+    const bool indexed = s.zSub * s.phiSub > 1;  // the search index exists (crzpS)
+    const float4* copySrc = indexed ? s.crzpS : s.crzp;
     const int nrows = stageUnion(t, chunkRowRef(s.cbin + rcur.base, first), table, next.count, copySrc + rnext.base,
                                  stage, kStageCap, L);
     // every cluster of the wavefront away from phi = 0: the wrap comparison is left out
@@ -1000,10 +1039,11 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) kTr
                                 reinterpret_cast<const char*>(s.crzpS + rnext.base),
                                 s.subTab + (size_t)(rw * kLayers + l + 1) * kBins,
                                 s.zSub,
+                                s.phiSub,
                                 stage,
                                 L.rowOff,
                                 L.copyRef};
-      if (nrows > 0 && (t.sub || s.zSub == 1)) {  // with sub-bins the copy serves only the bin-by-bin search
+      if (nrows > 0 && (t.sub || !indexed)) {  // with sub-bins the copy serves only the bin-by-bin search
         if (waveWrapFree)
           searchRows<true, 8, kSub, false>(t, table, next.count, l, z, r, phi, src, emit);
         else
@@ -1030,8 +1070,8 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) kTr
 // a kept one to the lane's slot row.  The raw count goes to trkCnt, the kept
 // count to trkKept (kCommitKept moves it after the last layer), the chunk's
 // kept sum to the chunk sums.  Clusters with more than kTrkSlots candidates
-// are kept whole (the fill recomputes them).  Needs one z sub-bin and next
-// layers of at most 65536 clusters.  The same tracklets as kTrackletsCount +
+// are kept whole (the fill recomputes them).  Needs one sub-bin per bin (no
+// search index) and next layers of at most 65536 clusters.  The same tracklets as kTrackletsCount +
 // kTrackletsScore, with no candidate round trip through memory, but slower on
 // MI355X: the search and the scoring alternate within a workgroup at four
 // waves per SIMD, and the six launches each pay the chunk set-up.
@@ -1109,7 +1149,7 @@ kTrackletsSearchScore(DevState s, int layer, float threshold)
             myRow[raw] = j;
           ++raw;
         };
-        const CandidateSource src{reinterpret_cast<const char*>(s.crzp + rnext.base), nullptr, nullptr, 1, stage,
+        const CandidateSource src{reinterpret_cast<const char*>(s.crzp + rnext.base), nullptr, nullptr, 1, 1, stage,
                                   L.rowOff, L.copyRef};
         if (nrows > 0)
           searchRows<true, 8, false>(t, table, next.count, l, z, r, phi, src, emit);
@@ -2023,7 +2063,7 @@ void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t str
                      (int)kStTracklets, s.W * kTrackletLayers, 0, 0);
   done("kPlanTracklets", 1);
   // the fused search and filter keeps candidates as 16-bit positions and reads the copy in sorted order
-  if (cfg.filter && cfg.fusedFilter && s.zSub == 1 && cfg.maxLayerClusters <= 65536) {
+  if (cfg.filter && cfg.fusedFilter && s.zSub * s.phiSub == 1 && cfg.maxLayerClusters <= 65536) {
     for (int l = kTrackletLayers - 1; l >= 0; --l) {
       hipLaunchKernelGGL(kTrackletsSearchScore<B>, dim3(G), dim3(B), 0, stream, s, l, cfg.threshold);
       done("kTrackletsSearchScore", 1);
@@ -2031,7 +2071,7 @@ void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t str
     launchCommitKept(s, stream);
     done("kCommitKept", 1);
   } else {
-    if (s.zSub > 1)
+    if (s.zSub * s.phiSub > 1)
       hipLaunchKernelGGL((kTrackletsCount<B, true>), dim3(G), dim3(B), 0, stream, s);
     else
       hipLaunchKernelGGL((kTrackletsCount<B, false>), dim3(G), dim3(B), 0, stream, s);

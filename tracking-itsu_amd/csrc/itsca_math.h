@@ -262,6 +262,16 @@ ITSCA_HD int zSubIndex(int layer, float z, int zb, int S)
 }
 // IndexTableUtils.h:53-56
 ITSCA_HD int phiBinIndex(float phi) { return truncToIntX86(phi * kInversePhiBinSize); }
+// phi sub-row of the tracklet search index (no counterpart in the reference):
+// phi row `row` split into P parts.  For a fixed row it is monotone in phi
+// (truncation of a monotone float expression, then a clamp into [0, P)); phi
+// stays within (-8, 14) wherever it is called.
+ITSCA_HD int phiSubIndex(float phi, int row, int P)
+{
+  const int f = (int)(phi * (kInversePhiBinSize * (float)P));
+  const int sub = f - row * P;
+  return sub < 0 ? 0 : (sub >= P ? P - 1 : sub);
+}
 // IndexTableUtils.h:58-62 (row stride PhiBins, clamped to the 401st slot; a
 // negative z index in phi row 0 gives a negative bin, which the reference's
 // sort puts before bin 0 and its index table counts inside bin 0's range).

@@ -105,6 +105,7 @@ struct itsca_ctx {
   int persistentBlocks = 0;
   int blockSize = 256;  // workgroup (and chunk) size of the stage kernels
   int zSub = kDefaultZSub;  // z sub-bins per bin of the tracklet search index
+  int phiSub = 1;           // phi sub-rows per row of the tracklet search index
   double capacityScale = 1.0;
   hipStream_t stream = nullptr;
   std::string err;
@@ -365,6 +366,7 @@ struct itsca_ctx {
     s.crzpS = crzpS.p;
     s.subTab = subTab.p;
     s.zSub = zSub;
+    s.phiSub = phiSub;
     s.segGeo = segGeo.p;
     s.cid = cid.p;
     s.cmc = cmc.p;
@@ -900,6 +902,7 @@ itsca_status itsca_run_noalloc(itsca_ctx* ctx, const itsca_device_event* ev, int
     s.W = 1;
     s.chunk = ctx->blockSize;
     s.zSub = ctx->zSub;
+    s.phiSub = ctx->phiSub;
     s.rx = ev->x;
     s.ry = ev->y;
     s.rz = ev->z;
@@ -1120,9 +1123,15 @@ itsca_status itsca_set_tuning(itsca_ctx* ctx, const itsca_tuning* t)
   if (t->persistent_blocks < 0) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_set_tuning: negative persistent_blocks");
   if (t->z_subbins < 0 || t->z_subbins > kMaxZSub)
     return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_set_tuning: z_subbins must be 0..8");
+  if (t->phi_subrows != 0 && t->phi_subrows != 1 && t->phi_subrows != 2 && t->phi_subrows != kMaxPhiSub)
+    return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_set_tuning: phi_subrows must be 0, 1, 2 or 4");
+  const int zSub = t->z_subbins > 0 ? t->z_subbins : kDefaultZSub, phiSub = t->phi_subrows > 0 ? t->phi_subrows : 1;
+  if (zSub * phiSub > kMaxSubBins)
+    return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_set_tuning: z_subbins * phi_subrows must be at most 15");
   ctx->blockSize = t->block_size;
   ctx->persistentBlocks = t->persistent_blocks > 0 ? t->persistent_blocks : 8 * ctx->cus;
-  ctx->zSub = t->z_subbins > 0 ? t->z_subbins : kDefaultZSub;
+  ctx->zSub = zSub;
+  ctx->phiSub = phiSub;
   ctx->dropGraph();
   return ITSCA_OK;
 }
@@ -1133,13 +1142,15 @@ itsca_status itsca_get_tuning(itsca_ctx* ctx, itsca_tuning* t)
   t->block_size = ctx->blockSize;
   t->persistent_blocks = ctx->persistentBlocks;
   t->z_subbins = ctx->zSub;
+  t->phi_subrows = ctx->phiSub;
   return ITSCA_OK;
 }
 
 // Autotuner: times every tuning on the uploaded batch and keeps the fastest.
 // No tuning changes a result: the chunk plans only decide which workgroup
-// computes which element, and the z sub-bins only decide which candidates the
-// tracklet search can skip without testing, so every candidate is valid.
+// computes which element, and the z sub-bins and phi sub-rows only decide
+// which candidates the tracklet search can skip without testing, so every
+// candidate is valid.
 itsca_status itsca_autotune(itsca_ctx* ctx, int repeats, itsca_tuning* chosen, float* best_ms)
 {
   return guarded(ctx, "itsca_autotune", [&]() -> itsca_status {
@@ -1151,14 +1162,17 @@ itsca_status itsca_autotune(itsca_ctx* ctx, int repeats, itsca_tuning* chosen, f
     hipEvent_t a, b;
     HIP_TRY(hipEventCreate(&a));
     HIP_TRY(hipEventCreate(&b));
-    int bestBlock = ctx->blockSize, bestBlocks = ctx->persistentBlocks, bestSub = ctx->zSub;
+    int bestBlock = ctx->blockSize, bestBlocks = ctx->persistentBlocks, bestSub = ctx->zSub, bestPhi = ctx->phiSub;
     float best = -1.0f;
     for (int block : {64, 128, 256}) {
       for (int perCu : {4, 8, 16}) {
        for (int zSub : {1, 2, 4, 8}) {
+        for (int phiSub : {1, 2, 4}) {
+        if (zSub * phiSub > kMaxSubBins) continue;
         ctx->blockSize = block;
         ctx->persistentBlocks = perCu * ctx->cus;
         ctx->zSub = zSub;
+        ctx->phiSub = phiSub;
         ctx->dropGraph();
         const itsca_status st = itsca_run(ctx);  // sizes the buffers, captures the graph if enabled
         if (st != ITSCA_OK) {
@@ -1178,6 +1192,8 @@ itsca_status itsca_autotune(itsca_ctx* ctx, int repeats, itsca_tuning* chosen, f
           bestBlock = block;
           bestBlocks = ctx->persistentBlocks;
           bestSub = zSub;
+          bestPhi = phiSub;
+        }
         }
        }
       }
@@ -1187,6 +1203,7 @@ itsca_status itsca_autotune(itsca_ctx* ctx, int repeats, itsca_tuning* chosen, f
     ctx->blockSize = bestBlock;
     ctx->persistentBlocks = bestBlocks;
     ctx->zSub = bestSub;
+    ctx->phiSub = bestPhi;
     ctx->profiling = prof;
     ctx->useGraph = graph;
     ctx->dropGraph();
@@ -1195,6 +1212,7 @@ itsca_status itsca_autotune(itsca_ctx* ctx, int repeats, itsca_tuning* chosen, f
       chosen->block_size = bestBlock;
       chosen->persistent_blocks = bestBlocks;
       chosen->z_subbins = bestSub;
+      chosen->phi_subrows = bestPhi;
     }
     if (best_ms) *best_ms = best;
     return st;

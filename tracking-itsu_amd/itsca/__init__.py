@@ -40,7 +40,8 @@ class _Config(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("block_size", ctypes.c_int), ("persistent_blocks", ctypes.c_int), ("z_subbins", ctypes.c_int)]
+    _fields_ = [("block_size", ctypes.c_int), ("persistent_blocks", ctypes.c_int), ("z_subbins", ctypes.c_int),
+                ("phi_subrows", ctypes.c_int)]
 
 
 class Capacities(ctypes.Structure):
@@ -334,8 +335,9 @@ class GpuTracker:
     def finish(self) -> None:
         self._check(self._lib.itsca_finish(self.ctx), "itsca_finish")
 
-    def set_tuning(self, block_size: int, persistent_blocks: int = 0, z_subbins: int = 0) -> None:
-        t = Tuning(block_size, persistent_blocks, z_subbins)
+    def set_tuning(self, block_size: int, persistent_blocks: int = 0, z_subbins: int = 0,
+                   phi_subrows: int = 0) -> None:
+        t = Tuning(block_size, persistent_blocks, z_subbins, phi_subrows)
         self._check(self._lib.itsca_set_tuning(self.ctx, ctypes.byref(t)), "itsca_set_tuning")
 
     def tuning(self) -> Tuning:
@@ -344,9 +346,9 @@ class GpuTracker:
         return t
 
     def autotune(self, repeats: int = 3):
-        """Picks the fastest tuning (launch shape and z sub-bins of the
-        tracklet search index) for the uploaded batch; returns (Tuning, ms per
-        run)."""
+        """Picks the fastest tuning (launch shape, and z sub-bins and phi
+        sub-rows of the tracklet search index) for the uploaded batch; returns
+        (Tuning, ms per run)."""
         t = Tuning()
         ms = ctypes.c_float(0)
         self._check(self._lib.itsca_autotune(self.ctx, int(repeats), ctypes.byref(t), ctypes.byref(ms)),
