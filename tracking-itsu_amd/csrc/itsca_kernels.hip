@@ -1451,7 +1451,7 @@ __device__ __forceinline__ bool cellFit(const DevState& s, const CellSeed& c, in
 // events 0.2 % of the layer-1 tracklets have 9-13 survivors.
 constexpr int kCellList = 16;
 constexpr int kFitBatch = 4;  // survivors whose fit loads are issued together
-constexpr int kCandBlock = 4;  // candidates whose loads are issued together (two 16-byte loads)
+constexpr int kCandBlock = 8;  // candidates whose loads are issued together (four 16-byte loads)
 static_assert(kCandBlock % 2 == 0 && kCandBlock - 1 <= kTrackletPad, "the last block reads past the range");
 
 // Two consecutive (tanLambda, phi) records in one 16-byte load (8-byte

@@ -91,8 +91,13 @@ ITSCA_HD void candidateFeatures(const TrackletSeed& t, int layer, float bx, floa
   f[7] = tanL * (1.0f / 1.5f);
   f[8] = (float)(countJ < 32 ? countJ : 32) * (1.0f / 16.0f);
   f[9] = t.countI;
-  for (int k = 0; k < kMlpIn; ++k)
-    if (!(f[k] == f[k])) f[k] = 0.0f;  // NaN guard (degenerate geometry)
+  // NaN guard (degenerate geometry: r_i = r_j, or a cluster at the vertex's x
+  // and y).  With finite coordinates only the features through tanL or the
+  // pointing can be NaN: the curvature is guarded above, and every other
+  // feature is a finite difference, product or count.
+  if (!(f[2] == f[2])) f[2] = 0.0f;
+  if (!(f[6] == f[6])) f[6] = 0.0f;
+  if (!(f[7] == f[7])) f[7] = 0.0f;
 }
 
 ITSCA_HD void trackletFeatures(const TrackletFeatureInput& in, float f[kMlpIn])
