@@ -207,7 +207,7 @@ def road_report(itsca, tracker, events, events_path, labels_path):
             "fake": sum(c[2] for c in counts)}
 
 
-def end_to_end(itsca, events, params, threshold, batches=6):
+def end_to_end(itsca, events, params, threshold, gates, batches=6):
     """The whole path a caller of the reference entry sees, per batch of
     events held in host memory: events in (itsca_add_event + itsca_upload),
     the pipeline, roads out to host memory (itsca_download_roads), no graph.
@@ -220,6 +220,7 @@ def end_to_end(itsca, events, params, threshold, batches=6):
         roads = [0, 0]
         for t in trackers:
             t.set_mlp_params(params)
+            t.set_filter_gates(gates)
             t.set_filter(form == "learned", threshold)
             t.set_events(events)  # sizes the buffers
             t.run()
@@ -345,7 +346,7 @@ def binning_sweep(itsca, tracker, threshold, steps=10):
     return out
 
 
-def pileup_batch(itsca, ft, params, threshold, steps, events=64):
+def pileup_batch(itsca, ft, params, threshold, gates, steps, events=64):
     """Pile-up events (4 collisions, z ~ U(-10, 10) cm, 2000-8000 primaries
     each, 3 % noise), every vertex its own work item, plain and learned."""
     with tempfile.TemporaryDirectory() as tmp:
@@ -355,6 +356,7 @@ def pileup_batch(itsca, ft, params, threshold, steps, events=64):
         t = itsca.GpuTracker(device=0)
         t.set_events(evs)
         t.set_mlp_params(params)
+        t.set_filter_gates(gates)
         t.set_filter(False)
         s_plain = time_form(t, steps, 1)
         q_plain = road_report(itsca, t, evs, ev_path, lab_path)
@@ -428,7 +430,9 @@ def main():
     tracker = itsca.GpuTracker(device=0)
     tracker.set_events(events)
     params, threshold, meta = ft.load_filter(args.weights)
+    gates = ft.load_gates(args.weights)
     tracker.set_mlp_params(params)
+    tracker.set_filter_gates(gates)
     print("# generated and uploaded %d events in %.1f s" % (len(events), time.time() - t_gen), file=sys.stderr,
           flush=True)
 
@@ -491,6 +495,7 @@ def main():
                    "events_per_step": args.batch,
                    "form": "batched CA + learned tracklet filter (10-32-32-1, BF16 MFMA, FP32 accumulate)",
                    "filter_threshold": threshold,
+                   "filter_phi_gates": gates,
                    "filter_training": meta.get("training_events"),
                    "tuning": learned_tuning},
         "plain": {"value": round(plain_eps, 2), "unit": "events/s", "ms_per_step": round(1e3 * plain_s / args.steps, 4),
@@ -527,8 +532,8 @@ def main():
         tmpdir = tempfile.TemporaryDirectory()
         for key, fn in (("single_event", lambda: single_events(itsca)),
                         ("mixed_batch", lambda: mixed_batch(itsca, 5)),
-                        ("pileup_batch", lambda: pileup_batch(itsca, ft, params, threshold, 3)),
-                        ("end_to_end", lambda: end_to_end(itsca, events, params, threshold)),
+                        ("pileup_batch", lambda: pileup_batch(itsca, ft, params, threshold, gates, 3)),
+                        ("end_to_end", lambda: end_to_end(itsca, events, params, threshold, gates)),
                         ("cpp_api_per_event", lambda: cpp_api_latency(itsca, tmpdir.name, args.seed))):
             try:
                 line[key] = fn()

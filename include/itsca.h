@@ -217,11 +217,22 @@ itsca_status itsca_mlp_get_params(itsca_ctx* ctx, float* params);
  * the same tracklets; form 2 falls back to form 1 for batches searched with
  * more than one z sub-bin or with a layer of more than 65536 clusters. */
 itsca_status itsca_set_filter(itsca_ctx* ctx, int enable, float threshold);
+/* phi gates of the filtered runs: per tracklet layer l (ITSCA_TRACKLET_LAYERS
+ * of them), the tracklet search of a run with the filter on keeps only the
+ * candidates with |dphi| < gates[l] (0 < gates[l] <= 0.3, the reference's phi
+ * cut: no gate) -- the plain candidates that pass the gate, in the plain
+ * order -- and searches only the window rows that can hold them.  The network
+ * scores what is left, its count features being the gated counts.  gates ==
+ * NULL: no gates (the default).  Plain runs never use them. */
+#define ITSCA_TRACKLET_LAYERS 6
+itsca_status itsca_set_filter_gates(itsca_ctx* ctx, const float* gates);
 
 /* Training set on the device: rows of ITSCA_MLP_FEATURES BF16 values and a
  * 0/1 label (the tracklet belongs to a cell of a final road of the plain CA). */
 itsca_status itsca_train_clear(itsca_ctx* ctx, int capacity);
-/* After a plain itsca_run: labels every tracklet of the batch and appends every
+/* After a plain itsca_run, or a filtered one whose threshold keeps every
+ * candidate (<= -1e38: the gated candidates): labels every tracklet of the
+ * batch (a tracklet of a cell of a final road of that run) and appends every
  * stride-th one (starting at `phase`) to the training set. */
 itsca_status itsca_train_collect(itsca_ctx* ctx, int stride, int phase, int* added);
 itsca_status itsca_train_set(itsca_ctx* ctx, const uint16_t* features, const uint8_t* labels, int n);

@@ -64,6 +64,11 @@ int itsca_events_copy(const itsca_events* ev, int event, float* x, float* y, flo
  * threshold under "threshold").  params receives n_params floats; returns 0 on
  * success, -1 if the file cannot be read or does not hold n_params values. */
 int itsca_filter_load(const char* path, float* params, int n_params, float* threshold);
+/* The phi gates of the same file ("phi_gates": one per tracklet layer, radians;
+ * see itsca_set_filter_gates): gates receives n_gates floats, 0.3 each (no
+ * gate) when the file has none; returns 0 on success, -1 if the file cannot be
+ * read or its gate list is short. */
+int itsca_filter_load_gates(const char* path, float* gates, int n_gates);
 
 #ifdef __cplusplus
 }

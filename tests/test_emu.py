@@ -77,6 +77,35 @@ for name in ("small_s1", "sparse_s6", "small_s5_nolayer3"):
         print("mfma forward vs float64 ok", flush=True)
     t.close()
     print(name, "ok", flush=True)
+# phi gates: a filtered run whose threshold keeps every candidate builds exactly
+# the plain tracklets inside the gates, in the plain order -- scored after the
+# search and inside it, with and without the search index (phi sub-rows)
+from gpu_items import gated_tracklets
+for name in ("small_s1", "pileup2_s7"):
+    ev = itsca.load_events(cases[name])[0]
+    t = itsca.GpuTracker(persistent_blocks=4, lib=lib)
+    t.set_events([ev])
+    t.run()
+    items = t.num_items()
+    plain = [[t.download(w, itsca.TRACKLETS, l, t.counts(w).tracklets[l], 4) for l in range(6)] for w in range(items)]
+    t.mlp_init(3)
+    for gates, tuning, fused in (([0.02, 0.02, 0.15, 0.05, 0.1, 0.05], (256, 4, 1, 1), False),
+                                 ([0.004] * 6, (128, 4, 2, 4), False),
+                                 ([0.02, 0.3, 0.15, 0.05, 0.1, 0.05], (256, 4, 1, 1), True)):
+        t.set_tuning(*tuning)
+        t.set_filter_gates(gates)
+        t.set_filter(True, -3.0e38, fused=fused)
+        t.run()
+        for w in range(items):
+            want = gated_tracklets(t, w, plain[w], gates)
+            n = t.counts(w)
+            for l in range(6):
+                got = t.download(w, itsca.TRACKLETS, l, n.tracklets[l], 4)[:, :2]
+                assert np.array_equal(got, want[l]), (name, gates, tuning, w, l)
+    t.set_filter(False)
+    t.set_filter_gates(None)
+    t.close()
+    print(name, "phi gates ok", flush=True)
 # other launch shapes and search-index binnings: an odd number of z sub-bins,
 # phi sub-rows alone and together with z sub-bins
 p = cases["pileup2_s7"]

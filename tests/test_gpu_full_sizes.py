@@ -59,8 +59,7 @@ def test_central_256_event_batch_learned_within_plain():
     say("gpu full size: 256 held-out central events, learned vs plain")
     evs = itsca.generate_events(2, 20000, 256)
     t = itsca.GpuTracker(device=0)
-    params, threshold, _ = ft.load_filter(None)
-    t.set_mlp_params(params)
+    threshold = ft.prepare(t)  # the shipped weights and phi gates
     q = ft.evaluate(t, evs, threshold)
     t.close()
     say("  recall %.6f of %d plain tracks, %d outside" % (q["recall"], q["plain_tracks"], q["extra"]))
@@ -80,8 +79,7 @@ def test_pileup_64_event_batch():
     t.run()
     for v in range(4):
         assert itsca.dump_item(t, v, evs[big].vertices[v]) == pyoracle.OracleVertex(evs[big], v, stable=True).dump(), v
-    params, threshold, _ = ft.load_filter(None)
-    t.set_mlp_params(params)
+    threshold = ft.prepare(t)
     q = ft.evaluate(t, evs, threshold)
     t.close()
     say("  recall %.6f of %d plain tracks, %d outside" % (q["recall"], q["plain_tracks"], q["extra"]))

@@ -13,6 +13,7 @@ import pytest
 import itsca
 import mlp_ref as M
 from golden_cases import dense_spot_event, fixture, small_cases
+from gpu_items import gated_tracklets
 
 pytestmark = pytest.mark.gpu
 CASES = small_cases()
@@ -263,6 +264,59 @@ def test_filter_keeps_exactly_the_tracklets_scored_above_threshold(tracker, quan
     tracker.set_filter(False)
 
 
+@pytest.mark.parametrize("quantile", [0.3, 0.9])
+def test_gated_filter_keeps_exactly_the_gated_tracklets_scored_above_threshold(tracker, quantile):
+    """With phi gates, a filtered run that keeps every candidate builds exactly
+    the plain tracklets inside the gates, in the plain order (restated on the
+    host from the downloaded cluster phis), and a threshold then keeps exactly
+    those of them whose training row (gated counts) scores at or above it;
+    clusters whose gated candidates overflowed their slots are kept whole."""
+    from itsca import filter_training as ft
+    say(f"gpu gated filter keeps exactly the gated rows above the {quantile} quantile")
+    gates = [0.02, 0.02, 0.15, 0.05, 0.1, 0.05]
+    evs = itsca.generate_events(2, 33, 1) + itsca.generate_events(5, 34, 1) + [dense_spot_event()]
+    tracker.set_filter_gates(None)
+    tracker.set_filter(False)
+    tracker.set_events(evs)
+    tracker.run()
+    items = tracker.num_items()
+    plain = [[tracker.download(i, itsca.TRACKLETS, l, tracker.counts(i).tracklets[l], 4)
+              for l in range(itsca.TRACKLET_LAYERS)] for i in range(items)]
+    tracker.set_mlp_params(_random_params(6))
+    tracker.set_filter_gates(gates)
+    tracker.set_filter(True, ft.KEEP_ALL)
+    tracker.run()
+    gated = [gated_tracklets(tracker, i, plain[i], gates) for i in range(items)]
+    for i in range(items):
+        n = tracker.counts(i)
+        for l in range(itsca.TRACKLET_LAYERS):
+            assert np.array_equal(tracker.download(i, itsca.TRACKLETS, l, n.tracklets[l], 4)[:, :2], gated[i][l]), (i, l)
+    total = sum(len(t) for it in gated for t in it)
+    n_plain = sum(len(t) for it in plain for t in it)
+    say(f"  gated candidates {total} of {n_plain} plain tracklets")
+    assert 0 < total < 0.7 * n_plain
+    tracker.train_clear(total)
+    assert tracker.train_collect(1, 0) == total
+    z = tracker.train_scores(total)
+    threshold = float(np.quantile(z, quantile))
+    tracker.set_filter(True, threshold)
+    tracker.run()
+    k = dropped = 0
+    for i in range(items):
+        n = tracker.counts(i)
+        for l in range(itsca.TRACKLET_LAYERS):
+            t = gated[i][l]
+            raw = np.bincount(t[:, 0], minlength=n.clusters[l]) if len(t) else np.zeros(n.clusters[l], int)
+            keep = (z[k:k + len(t)] >= threshold) | (raw[t[:, 0]] > itsca.TRACKLET_SLOTS)
+            k += len(t)
+            got = tracker.download(i, itsca.TRACKLETS, l, n.tracklets[l], 4)[:, :2]
+            assert np.array_equal(got, t[keep]), (i, l)
+            dropped += int((~keep).sum())
+    assert k == total and 0 < dropped < total
+    tracker.set_filter(False)
+    tracker.set_filter_gates(None)
+
+
 def _device_ms(tracker, runs=3):
     """(all kernels, cell-building kernels) in ms per run, from HIP events."""
     tracker.set_profiling(True)
@@ -287,6 +341,7 @@ def test_fused_filter_equals_separate_filter(tracker):
     tracker.set_events(evs)
     items = tracker.num_items()
     params, thr, _ = ft.load_filter()
+    tracker.set_filter_gates(ft.load_gates())  # both forms search inside the shipped phi gates
     for p, t in ((params, thr), (_random_params(41), None)):
         tracker.set_mlp_params(p)
         if t is None:
@@ -305,18 +360,19 @@ def test_fused_filter_equals_separate_filter(tracker):
         say(f"  threshold {t:.4f}: {kept} tracklets kept, fused == separate: {outs[0] == outs[1]}")
         assert outs[0] == outs[1]
     tracker.set_filter(False)
+    tracker.set_filter_gates(None)
 
 
 def test_trained_filter_gain_over_plain_and_untrained(tracker):
     """Ablation on held-out central events (seeds the training never saw): the
     shipped network keeps >= 99.5 % of the plain tracks, finds none outside
     them, and cuts the device time; an untrained network held to the same
-    recall of positive rows keeps far more tracklets."""
+    recall of positive rows keeps far more of the negative rows."""
     from itsca import filter_training as ft
     say("gpu ablation: trained filter vs plain and vs untrained")
     evs = itsca.generate_events(2, 30000, 24)
-    params, thr, _ = ft.load_filter()
-    tracker.set_mlp_params(params)
+    thr = ft.prepare(tracker)  # the shipped weights and phi gates
+    gates = ft.load_gates()
     r = ft.evaluate(tracker, evs, thr)
     say(f"  trained: recall {r['recall']:.5f}, outside {r['extra']}, tracklets kept {r['tracklets_kept']:.3f}, "
         f"road cells kept {r['road_cells_kept']:.5f}")
@@ -324,8 +380,10 @@ def test_trained_filter_gain_over_plain_and_untrained(tracker):
     assert r["recall"] >= 0.995
     assert r["road_cells_kept"] >= 0.995
     assert r["tracklets_kept"] < 0.3
-    everything = ft.evaluate(tracker, evs, -3.0e38, plain=r["plain"])  # a cut no score fails
+    tracker.set_filter_gates(None)
+    everything = ft.evaluate(tracker, evs, -3.0e38, plain=r["plain"])  # no gates, a cut no score fails
     assert (everything["recall"], everything["road_cells_kept"], everything["cells_kept"]) == (1.0, 1.0, 1.0)
+    tracker.set_filter_gates(gates)
     tracker.set_filter(True, thr)
     learned_ms, learned_cells = _device_ms(tracker)
     tracker.set_filter(False)
@@ -335,19 +393,24 @@ def test_trained_filter_gain_over_plain_and_untrained(tracker):
     assert learned_cells < 0.5 * plain_cells
     assert learned_ms < 0.9 * plain_ms
 
+    # rows as the network sees them: the gated candidates, from a run that keeps all of them
+    tracker.set_filter(True, ft.KEEP_ALL)
+    tracker.run()
     total = sum(sum(tracker.counts(i).tracklets) for i in range(tracker.num_items()))
     tracker.train_clear(total)
     tracker.train_collect(1, 0)
+    tracker.set_filter(False)
+    tracker.set_filter_gates(None)
     _, y = tracker.train_get(total)
     z_trained = tracker.train_scores(total)
     row_recall = float(np.mean(z_trained[y > 0] >= thr))
     tracker.mlp_init(123)
     z_untrained = tracker.train_scores(total)
     thr_u = float(np.quantile(z_untrained[y > 0], 1.0 - row_recall))
-    kept_u = float(np.mean(z_untrained >= thr_u))
-    kept_t = float(np.mean(z_trained >= thr))
-    say(f"  positive-row recall {row_recall:.5f}: trained keeps {kept_t:.3f}, untrained keeps {kept_u:.3f}")
-    assert kept_u > 2.5 * kept_t
+    kept_u = float(np.mean(z_untrained[y == 0] >= thr_u))
+    kept_t = float(np.mean(z_trained[y == 0] >= thr))
+    say(f"  positive-row recall {row_recall:.5f}: negative rows kept: trained {kept_t:.3f}, untrained {kept_u:.3f}")
+    assert kept_u > 2.0 * kept_t
 
 
 def test_training_end_to_end(tracker):
@@ -356,7 +419,8 @@ def test_training_end_to_end(tracker):
     threshold tuned on 8 validation events; then, on 8 events neither set
     holds, the trained filter keeps the plain roads (recall >= 99 %, none
     outside) while cutting most tracklets, and its training rows separate far
-    better than the initial network's."""
+    better than the initial network's (negative rows kept at equal recall of
+    the positive ones)."""
     from itsca import filter_training as ft
     say("gpu training end to end: collect, train, tune, evaluate on held-out events")
     params, thr, meta = ft.train_filter(tracker, 2, range(11000, 11032), 2e6, range(11500, 11508), epochs=4,
@@ -370,10 +434,12 @@ def test_training_end_to_end(tracker):
     tracker.mlp_init(7)
     z_initial = tracker.train_scores(n)
     recall = float(np.mean(z_trained[y > 0] >= thr))
-    kept_t = float(np.mean(z_trained >= thr))
-    kept_i = float(np.mean(z_initial >= np.quantile(z_initial[y > 0], 1.0 - recall)))
-    say(f"  training rows at positive recall {recall:.5f}: trained keeps {kept_t:.3f}, initial keeps {kept_i:.3f}")
+    kept_t = float(np.mean(z_trained[y == 0] >= thr))
+    kept_i = float(np.mean(z_initial[y == 0] >= np.quantile(z_initial[y > 0], 1.0 - recall)))
+    say(f"  training rows at positive recall {recall:.5f}: negative rows kept: trained {kept_t:.3f}, "
+        f"initial {kept_i:.3f}")
     assert kept_i > 2.0 * kept_t
+    assert len(meta["phi_gates"]) == itsca.TRACKLET_LAYERS and max(meta["phi_gates"]) <= 0.3
 
     tracker.set_mlp_params(params)
     r = ft.evaluate(tracker, itsca.generate_events(2, 31000, 8), thr)
@@ -383,6 +449,7 @@ def test_training_end_to_end(tracker):
     assert r["recall"] >= 0.99
     assert r["tracklets_kept"] < 0.4
     tracker.set_filter(False)
+    tracker.set_filter_gates(None)
 
 
 def test_cli_plain_and_with_learned_filter(golden_dir):

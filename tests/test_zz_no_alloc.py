@@ -41,8 +41,7 @@ def test_noalloc_equals_itsca_run():
         assert len(want) > 50
         assert np.array_equal(got, want)
     # the learned filter, same weights and threshold on both entry points
-    params, threshold, _ = ft.load_filter(None)
-    t.set_mlp_params(params)
+    threshold = ft.prepare(t)  # the shipped weights and phi gates
     t.set_filter(True, threshold)
     for ev in (big, small):
         t.set_events([ev])

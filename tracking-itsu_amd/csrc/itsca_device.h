@@ -113,6 +113,10 @@ struct DevState {
   int *trkCnt, *trkOff;  // per sorted cluster
   int* trkKept;          // per sorted cluster: candidates kept by the learned filter
   const uint16_t* mlpW;  // learned filter parameters, BF16 bits
+  // learned form: per tracklet layer, the phi gate of the tracklet search (the
+  // phi cut becomes |dphi| < phiGate[l] <= kPhiCut); gated = 0: plain search
+  float phiGate[kTrackletLayers];
+  int gated;
   int* trkCand;          // kTrkSlots per sorted cluster
   Seg* trkSeg;           // W*6
   int2* tFS;    // (first cluster, second cluster) positions

@@ -623,15 +623,38 @@ constexpr float kZSkipMargin = 0.01f;
 
 // The search of one cluster: its window, and the z bins of each window row
 // whose clusters are tested.
+//
+// phi gate (the learned form only, no counterpart in the reference): with
+// phiCut < kPhiCut the phi cut becomes |dphi| < phiCut (the same comparisons,
+// the smaller bound), so the accepted candidates are the plain ones that pass
+// the gate, in the plain order.  The window rows are the plain ones, but the
+// bins of a row are scanned only when the row can hold a candidate inside the
+// gate: the rows of [phi - phiCut, phi + phiCut] widened by kPhiSkipMargin
+// (phiBinIndex is monotone), a cyclic sub-range of the plain rows.  The
+// clusters of the other rows' bins lie in those rows (every z bin index of the
+// next layer in [0, kZBins), else nothing is narrowed), so outside the gate.
+// Every row's extra candidate (the first cluster after its bins, which can sit
+// in any later row when bins are empty) is still tested.
+constexpr float kPhiSkipMargin = 1.0e-3f;
+
 struct TrackletSearch {
   TrackletWindow win;
   float tanLambda, maxDeltaZ;
   int zFirst, zLast;
   float zLo, zHi;  // every candidate that can pass has z in [zLo, zHi] (when sub)
   bool sub;        // the z range holds: bins, then z sub-bins, outside it are skipped
+  float phiCut;    // the phi cut: kPhiCut, or the learned form's gate
+  // the window rows whose bins are scanned: scanRows rows from scanFirst
+  // (narrow: a gated sub-range of the window's rows, else all of them)
+  int scanFirst, scanRows;
+  bool narrow;
 };
 
-__device__ __forceinline__ TrackletSearch trackletSearch(float4 geoNext, float z, float r, float phi, float vz, int l)
+// The learned form's phi gate of tracklet layer l (kPhiCut: none).
+__device__ __forceinline__ float trackletPhiCut(const DevState& s, int l) { return s.gated ? s.phiGate[l] : kPhiCut; }
+
+__device__ __forceinline__ TrackletSearch trackletSearch(float4 geoNext, float z, float r, float phi, float vz, int l,
+                                                         float phiCut)
 {
   TrackletSearch t;
   t.win = trackletWindow(z, r, phi, vz, l);
@@ -641,6 +664,23 @@ __device__ __forceinline__ TrackletSearch trackletSearch(float4 geoNext, float z
   t.zLast = t.win.z;
   t.zLo = t.zHi = 0.0f;
   t.sub = false;
+  t.phiCut = phiCut < kPhiCut ? phiCut : kPhiCut;
+  t.scanFirst = t.win.y;
+  t.scanRows = t.win.rows;
+  t.narrow = false;
+  if (t.phiCut < kPhiCut && !t.win.empty && geoNext.z != 0.0f) {
+    const int y = phiBinIndex(normalizedPhi(phi - (t.phiCut + kPhiSkipMargin)));
+    const int w = phiBinIndex(normalizedPhi(phi + (t.phiCut + kPhiSkipMargin)));
+    const bool inRange = y >= 0 && y < kPhiBins && w >= 0 && w < kPhiBins && t.win.y >= 0 && t.win.y < kPhiBins &&
+                         t.win.w >= 0 && t.win.w < kPhiBins;
+    if (inRange) {
+      int rows = w - y + 1;
+      if (rows < 0) rows += kPhiBins;
+      t.scanFirst = y;
+      t.scanRows = rows;
+      t.narrow = true;
+    }
+  }
   if (!t.win.empty && geoNext.z != 0.0f) {
     const float zA = t.tanLambda * (geoNext.x - r) + z, zB = t.tanLambda * (geoNext.y - r) + z;
     const float zLo = fminf(zA, zB) - t.maxDeltaZ - kZSkipMargin, zHi = fmaxf(zA, zB) + t.maxDeltaZ + kZSkipMargin;
@@ -679,17 +719,16 @@ __device__ __forceinline__ int subStart(uint4 e, int t)
 }
 
 // The phi sub-rows [lo, hi] of window row `row` (P sub-rows per row) that can
-// hold a candidate passing the phi cut of a cluster at `phi`; lo > hi: none.
-// A candidate passes when |phi - phi_j| < kPhiCut or ||phi - phi_j| - 2pi| <
-// kPhiCut, so phi_j lies within kPhiCut (plus kPhiSkipMargin, far above the
-// rounding of those differences at |phi| < 7) of phi + 2pi k for some k in
-// {-1, 0, 1}.  The candidates of row `row` have phi_j in the row's range,
+// hold a candidate passing the phi cut phiCut (kPhiCut, or the learned form's
+// gate) of a cluster at `phi`; lo > hi: none.  A candidate passes when
+// |phi - phi_j| < phiCut or ||phi - phi_j| - 2pi| < phiCut, so phi_j lies
+// within phiCut (plus kPhiSkipMargin, far above the rounding of those
+// differences at |phi| < 7) of phi + 2pi k for some k in {-1, 0, 1}.  The candidates of row `row` have phi_j in the row's range,
 // widened here by 0.01 rad; for every k whose interval meets it, phiSubIndex
 // (monotone in phi) bounds the candidate's sub-row by those of the interval's
 // ends.
-constexpr float kPhiSkipMargin = 1.0e-3f;
 
-__device__ __forceinline__ void phiSubRange(float phi, int row, int P, int& lo, int& hi)
+__device__ __forceinline__ void phiSubRange(float phi, float phiCut, int row, int P, int& lo, int& hi)
 {
   constexpr float kRowWidth = kTwoPi / (float)kPhiBins;
   const float rowLo = (float)row * kRowWidth - 0.01f, rowHi = (float)(row + 1) * kRowWidth + 0.01f;
@@ -697,8 +736,8 @@ __device__ __forceinline__ void phiSubRange(float phi, int row, int P, int& lo, 
   hi = -1;
 #pragma unroll
   for (int k = -1; k <= 1; ++k) {
-    const float a = phi - (kPhiCut + kPhiSkipMargin) + (float)k * kTwoPi;
-    const float b = phi + (kPhiCut + kPhiSkipMargin) + (float)k * kTwoPi;
+    const float a = phi - (phiCut + kPhiSkipMargin) + (float)k * kTwoPi;
+    const float b = phi + (phiCut + kPhiSkipMargin) + (float)k * kTwoPi;
     if (b >= rowLo && a <= rowHi) {
       lo = min(lo, phiSubIndex(a, row, P));
       hi = max(hi, phiSubIndex(b, row, P));
@@ -744,11 +783,12 @@ __device__ __forceinline__ void searchRows(const TrackletSearch& t, const int* t
   if (t.win.empty) return;
   const TrackletWindow& win = t.win;
   const float tanLambda = t.tanLambda, maxDeltaZ = t.maxDeltaZ;
+  const float phiCut = t.phiCut;
   auto passes = [&](const float4& q) {
     const float deltaZ = fabsf(tanLambda * (q.x - r) + z - q.y);
     const float deltaPhi = fabsf(phi - q.z);
-    if (!kWrap) return (deltaZ < maxDeltaZ) & (deltaPhi < kPhiCut);
-    return (deltaZ < maxDeltaZ) & ((deltaPhi < kPhiCut) | (fabsf(deltaPhi - kTwoPi) < kPhiCut));
+    if (!kWrap) return (deltaZ < maxDeltaZ) & (deltaPhi < phiCut);
+    return (deltaZ < maxDeltaZ) & ((deltaPhi < phiCut) | (fabsf(deltaPhi - kTwoPi) < phiCut));
   };
   auto loadGlobal = [](const char* g, int j) { return *reinterpret_cast<const float4*>(g + 16u * (unsigned)j); };
   uint64_t bits0 = 0, bits1 = 0;  // accepted offsets 0..63 and 64..127 of the current bin
@@ -797,17 +837,25 @@ __device__ __forceinline__ void searchRows(const TrackletSearch& t, const int* t
     // lies within 2 ulp below 0.3).  Clamp to the table end.
     if (maxBin > kBins) maxBin = kBins;
     const int jLast = table[maxBin];  // the reference's inclusive upper bound
+    bool scanned = true;  // the row's bins are scanned (gated: the row meets the gate)
+    if (t.narrow) {
+      int d = row - t.scanFirst;
+      if (d < 0) d += kPhiBins;
+      scanned = d < t.scanRows;
+    }
     int off = 0;
-    if (kStaged) {
+    if (kStaged && scanned) {  // the LDS copy holds the scanned rows
       int ri = row - src.rowRef;
       if (ri < 0) ri += kPhiBins;
       off = src.off[ri];
     }
-    if (kSub && t.sub && src.zSub * src.phiSub > 1) {
+    if (!scanned) {
+      // only the extra candidate below
+    } else if (kSub && t.sub && src.zSub * src.phiSub > 1) {
       auto loadS = [&](int j) { return kStaged ? src.lds[j + off] : loadGlobal(src.globalS, j); };
       const int S = src.zSub, P = src.phiSub;
       int psLo = 0, psHi = 0;
-      if (P > 1) phiSubRange(phi, row, P, psLo, psHi);
+      if (P > 1) phiSubRange(phi, phiCut, row, P, psLo, psHi);
       for (int zb = t.zFirst; psLo <= psHi && zb <= t.zLast; ++zb) {  // table positions clamped like maxBin
         const int b = min(firstBin + (zb - win.x), maxBin);
         const int bS = table[b];
@@ -855,7 +903,7 @@ __device__ __forceinline__ void forEachTracklet(const DevState& s, int rw, int l
                                                 int nNext, float4 geoNext, float z, float r, float phi, float vz,
                                                 F&& emit)
 {
-  const TrackletSearch t = trackletSearch(geoNext, z, r, phi, vz, l);
+  const TrackletSearch t = trackletSearch(geoNext, z, r, phi, vz, l, trackletPhiCut(s, l));
   const CandidateSource src{reinterpret_cast<const char*>(s.crzp + baseNext),
                             reinterpret_cast<const char*>(s.crzpS + baseNext),
                             s.subTab + (size_t)(rw * kLayers + l + 1) * kBins,
@@ -901,13 +949,13 @@ __device__ __forceinline__ int stageUnion(const TrackletSearch& t, int rowRef, c
   }
   __syncthreads();
   int dMin = kPhiBins, dMax = -kPhiBins, zMin = kZBins, zMax = -1, noCopy = 0;
-  if (!t.win.empty) {
-    if (t.win.y >= kPhiBins || t.win.rows > kStageRows) noCopy = 1;  // the phi == 2pi row, or a wide window
-    int d = t.win.y - rowRef;
+  if (!t.win.empty) {  // the rows whose bins the search scans
+    if (t.scanFirst >= kPhiBins || t.scanRows > kStageRows) noCopy = 1;  // the phi == 2pi row, or a wide window
+    int d = t.scanFirst - rowRef;
     if (d < 0) d += kPhiBins;
     if (d >= kPhiBins / 2) d -= kPhiBins;
     dMin = d;
-    dMax = d + t.win.rows - 1;
+    dMax = d + t.scanRows - 1;
     if (t.zFirst <= t.zLast) {
       zMin = t.zFirst;
       zMax = t.zLast;
@@ -1020,7 +1068,7 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) kTr
       z = s.cz[rcur.base + i];
       r = s.cr[rcur.base + i];
       phi = s.cphi[rcur.base + i];
-      t = trackletSearch(s.segGeo[rw * kLayers + l + 1], z, r, phi, s.itemVertex[3 * w + 2], l);
+      t = trackletSearch(s.segGeo[rw * kLayers + l + 1], z, r, phi, s.itemVertex[3 * w + 2], l, trackletPhiCut(s, l));
     }
     const bool indexed = s.zSub * s.phiSub > 1;  // the search index exists (crzpS)
     const float4* copySrc = indexed ? s.crzpS : s.crzp;
@@ -1135,7 +1183,7 @@ kTrackletsSearchScore(DevState s, int layer, float threshold)
         z = s.cz[rcur.base + i];
         r = s.cr[rcur.base + i];
         phi = s.cphi[rcur.base + i];
-        t = trackletSearch(s.segGeo[rw * kLayers + l + 1], z, r, phi, vz, l);
+        t = trackletSearch(s.segGeo[rw * kLayers + l + 1], z, r, phi, vz, l, trackletPhiCut(s, l));
       }
       const int nrows = stageUnion(t, chunkRowRef(s.cbin + rcur.base, first), table, next.count,
                                    s.crzp + rnext.base, stage, kFusedStageCap, L);

@@ -116,6 +116,7 @@ struct itsca_ctx {
   // learned tracklet filter
   bool filterOn = false;
   float threshold = 0.0f;
+  float phiGate[kTrackletLayers] = {kPhiCut, kPhiCut, kPhiCut, kPhiCut, kPhiCut, kPhiCut};
   bool filterFused = false;  // itsca_set_filter(ctx, 2, ...)
   int mlpStep = 0;
   int trainN = 0, trainCap = 0;
@@ -342,6 +343,16 @@ struct itsca_ctx {
     dropGraph();
   }
 
+  // the learned form's phi gates (filtered runs only)
+  void setGates(DevState& s) const
+  {
+    s.gated = 0;
+    for (int l = 0; l < kTrackletLayers; ++l) {
+      s.phiGate[l] = phiGate[l];
+      if (filterOn && phiGate[l] < kPhiCut) s.gated = 1;
+    }
+  }
+
   DevState state() const
   {
     DevState s{};
@@ -380,6 +391,7 @@ struct itsca_ctx {
     s.trkOff = trkOff.p;
     s.trkKept = trkKept.p;
     s.mlpW = mlpWbf.p;
+    setGates(s);
     s.trkCand = trkCand.p;
     s.trkSeg = dTrkSeg.p;
     s.tFS = tFS.p;
@@ -909,6 +921,7 @@ itsca_status itsca_run_noalloc(itsca_ctx* ctx, const itsca_device_event* ev, int
     s.rid = ev->cluster_id;
     s.rmc = ev->mc_id;
     s.mlpW = ctx->mlpWbf.p;
+    ctx->setGates(s);
     // small inputs: one work item, its layer segments, the vertex
     for (int l = 0; l < 8; ++l) ctx->naEvOff[l] = offs[l];
     for (int l = 0; l < kLayers; ++l) ctx->naClSeg[l] = Seg{offs[l] - offs[0], offs[l + 1] - offs[l]};
@@ -1283,6 +1296,20 @@ itsca_status itsca_set_filter(itsca_ctx* ctx, int enable, float threshold)
   });
 }
 
+itsca_status itsca_set_filter_gates(itsca_ctx* ctx, const float* gates)
+{
+  if (!ctx) return ITSCA_ERR_ARGUMENT;
+  float g[kTrackletLayers];
+  for (int l = 0; l < kTrackletLayers; ++l) {
+    g[l] = gates ? gates[l] : kPhiCut;
+    if (!(g[l] > 0.0f && g[l] <= kPhiCut))
+      return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_set_filter_gates: every gate must lie in (0, 0.3]");
+  }
+  for (int l = 0; l < kTrackletLayers; ++l) ctx->phiGate[l] = g[l];
+  ctx->dropGraph();
+  return ITSCA_OK;
+}
+
 itsca_status itsca_train_clear(itsca_ctx* ctx, int capacity)
 {
   if (!ctx || capacity < 0) return ITSCA_ERR_ARGUMENT;
@@ -1301,7 +1328,9 @@ itsca_status itsca_train_collect(itsca_ctx* ctx, int stride, int phase, int* add
   if (!ctx || stride < 1 || phase < 0) return ITSCA_ERR_ARGUMENT;
   return guarded(ctx, "itsca_train_collect", [&]() -> itsca_status {
     if (!ctx->ran) return ctx->fail(ITSCA_ERR_STATE, "itsca_train_collect: no completed run");
-    if (ctx->filterOn) return ctx->fail(ITSCA_ERR_STATE, "itsca_train_collect: labels come from plain runs only");
+    if (ctx->filterOn && ctx->threshold > -1.0e38f)
+      return ctx->fail(ITSCA_ERR_STATE,
+                       "itsca_train_collect: labels come from plain runs, or filtered runs that keep every candidate");
     HIP_TRY(hipSetDevice(ctx->device));
     long total = 0;
     for (const Seg& g : ctx->trkSeg) total += g.count;

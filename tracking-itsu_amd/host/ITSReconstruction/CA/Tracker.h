@@ -61,9 +61,11 @@ class Tracker<true> {
   // candidates whose network logit is below the threshold are dropped before
   // cell building; every later cut is the exact one, so the roads found are a
   // subset of the plain roads.  The weights file is the one written by
-  // `python -m itsca.filter_training` (1441 parameters and a threshold).
+  // `python -m itsca.filter_training` (1441 parameters, a threshold and the
+  // per-layer phi gates of the tracklet search; no gates: an empty vector).
   void useLearnedFilter(const std::string& weightsFile);
-  void useLearnedFilter(const std::vector<float>& params, float threshold);
+  void useLearnedFilter(const std::vector<float>& params, float threshold,
+                        const std::vector<float>& phiGates = std::vector<float>());
   void disableLearnedFilter();
 
  private:

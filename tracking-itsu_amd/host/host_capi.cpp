@@ -173,3 +173,30 @@ int itsca_filter_load(const char* path, float* params, int n_params, float* thre
   *threshold = std::strtof(text.c_str() + pos + 1, nullptr);
   return 0;
 }
+
+// C linkage from the declaration in itsca_host.h.
+int itsca_filter_load_gates(const char* path, float* gates, int n_gates)
+{
+  if (!path || !gates || n_gates < 1) return -1;
+  std::ifstream in(path);
+  if (!in) return -1;
+  std::stringstream ss;
+  ss << in.rdbuf();
+  const std::string text = ss.str();
+  for (int l = 0; l < n_gates; ++l) gates[l] = 0.3f;  // no gate
+  size_t pos = text.find("\"phi_gates\"");
+  if (pos == std::string::npos) return 0;
+  pos = text.find('[', pos);
+  const size_t end = pos == std::string::npos ? std::string::npos : text.find(']', pos);
+  if (end == std::string::npos) return -1;
+  const char* c = text.c_str() + pos + 1;
+  for (int l = 0; l < n_gates; ++l) {
+    char* next = nullptr;
+    const float v = std::strtof(c, &next);
+    if (next == c || (size_t)(next - text.c_str()) > end) return -1;
+    gates[l] = v;
+    c = next;
+    while (*c == ',' || *c == ' ' || *c == '\n') ++c;
+  }
+  return 0;
+}

@@ -72,15 +72,21 @@ void Tracker<true>::useLearnedFilter(const std::string& weightsFile)
 {
   std::vector<float> params(ITSCA_MLP_PARAMS);
   float threshold = 0.0f;
-  if (itsca_filter_load(weightsFile.c_str(), params.data(), ITSCA_MLP_PARAMS, &threshold) != 0)
+  std::vector<float> gates(ITSCA_TRACKLET_LAYERS);
+  if (itsca_filter_load(weightsFile.c_str(), params.data(), ITSCA_MLP_PARAMS, &threshold) != 0 ||
+      itsca_filter_load_gates(weightsFile.c_str(), gates.data(), ITSCA_TRACKLET_LAYERS) != 0)
     throw std::runtime_error("cannot read learned filter weights from " + weightsFile);
-  useLearnedFilter(params, threshold);
+  useLearnedFilter(params, threshold, gates);
 }
 
-void Tracker<true>::useLearnedFilter(const std::vector<float>& params, float threshold)
+void Tracker<true>::useLearnedFilter(const std::vector<float>& params, float threshold,
+                                     const std::vector<float>& phiGates)
 {
   if (params.size() != ITSCA_MLP_PARAMS) throw std::runtime_error("learned filter: expected 1441 parameters");
+  if (!phiGates.empty() && phiGates.size() != ITSCA_TRACKLET_LAYERS)
+    throw std::runtime_error("learned filter: expected one phi gate per tracklet layer");
   check(mCtx, itsca_mlp_set_params(mCtx, params.data()), "itsca_mlp_set_params");
+  check(mCtx, itsca_set_filter_gates(mCtx, phiGates.empty() ? nullptr : phiGates.data()), "itsca_set_filter_gates");
   check(mCtx, itsca_set_filter(mCtx, 1, threshold), "itsca_set_filter");
 }
 

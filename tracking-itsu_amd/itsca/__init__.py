@@ -73,7 +73,8 @@ HIP_SYMBOLS = [
     "itsca_get_capacities", "itsca_run_noalloc", "itsca_download_roads",
     "itsca_set_profiling", "itsca_stage_times", "itsca_kernel_times", "itsca_kernel_name", "itsca_set_graph",
     "itsca_set_tuning", "itsca_get_tuning", "itsca_autotune",
-    "itsca_mlp_init", "itsca_mlp_set_params", "itsca_mlp_get_params", "itsca_set_filter", "itsca_train_clear",
+    "itsca_mlp_init", "itsca_mlp_set_params", "itsca_mlp_get_params", "itsca_set_filter", "itsca_set_filter_gates",
+    "itsca_train_clear",
     "itsca_train_collect", "itsca_train_set", "itsca_train_size", "itsca_train_get", "itsca_train_step",
     "itsca_train_epoch", "itsca_train_scores",
 ]
@@ -83,6 +84,7 @@ MLP_FEATURES = 16
 HOST_SYMBOLS = [
     "itsca_events_generate", "itsca_events_load", "itsca_events_write", "itsca_events_free", "itsca_events_count",
     "itsca_events_num_vertices", "itsca_events_layer_offsets", "itsca_events_copy", "itsca_filter_load",
+    "itsca_filter_load_gates",
     "itsca_events_write_binary", "itsca_events_load_labels", "itsca_events_merge_pileup",
     "itsca_events_roads_report",
 ]
@@ -132,6 +134,7 @@ def bind_hip(lib: ctypes.CDLL) -> ctypes.CDLL:
         lib.itsca_mlp_set_params.argtypes = [P, F]
         lib.itsca_mlp_get_params.argtypes = [P, F]
         lib.itsca_set_filter.argtypes = [P, I, ctypes.c_float]
+        lib.itsca_set_filter_gates.argtypes = [P, F]
         lib.itsca_train_clear.argtypes = [P, I]
         lib.itsca_train_collect.argtypes = [P, I, I, ctypes.POINTER(I)]
         lib.itsca_train_set.argtypes = [P, U16, U8, I]
@@ -167,6 +170,7 @@ def host_lib() -> ctypes.CDLL:
         F, I32 = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32)
         lib.itsca_events_copy.argtypes = [P, I, F, F, F, I32, I32, F]
         lib.itsca_filter_load.argtypes = [ctypes.c_char_p, F, I, F]
+        lib.itsca_filter_load_gates.argtypes = [ctypes.c_char_p, F, I]
         lib.itsca_events_write_binary.argtypes = [P, ctypes.c_char_p]
         lib.itsca_events_load_labels.argtypes = [P, ctypes.c_char_p]
         lib.itsca_events_merge_pileup.argtypes = [P, I]
@@ -374,6 +378,18 @@ class GpuTracker:
         instead of in a kernel of its own after it (same tracklets)."""
         mode = (2 if fused else 1) if on else 0
         self._check(self._lib.itsca_set_filter(self.ctx, mode, float(threshold)), "itsca_set_filter")
+
+    def set_filter_gates(self, gates=None) -> None:
+        """Per tracklet layer, the phi gate of filtered runs (radians, in
+        (0, 0.3]); None: no gates."""
+        if gates is None:
+            self._check(self._lib.itsca_set_filter_gates(self.ctx, None), "itsca_set_filter_gates")
+            return
+        g = np.ascontiguousarray(gates, dtype=np.float32)
+        if g.shape != (TRACKLET_LAYERS,):
+            raise ValueError("one phi gate per tracklet layer")
+        self._check(self._lib.itsca_set_filter_gates(self.ctx, g.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+                    "itsca_set_filter_gates")
 
     def train_clear(self, capacity: int) -> None:
         self._check(self._lib.itsca_train_clear(self.ctx, int(capacity)), "itsca_train_clear")

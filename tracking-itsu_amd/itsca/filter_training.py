@@ -1,11 +1,20 @@
 """Training of the learned tracklet filter on the GPU, and its weights file.
 
-The labels come from the plain CA: after a plain run every tracklet is labelled
-1 when it is one of the two tracklets of a cell on a final road, else 0
-(itsca_train_collect).  Training runs the device's own forward/backward and
-AdamW kernels (itsca_train_epoch).  The logit threshold is then tuned on
-separate validation events by running the filtered pipeline and measuring the
-fraction of plain roads it still finds.
+The filter has two parts, both fitted here:
+- per tracklet layer, a phi gate of the tracklet search: twice the largest
+  |dphi| of the positive tracklets of the first gate-sample events (capped at
+  the reference's 0.3 rad cut).  Filtered runs search and keep only the
+  candidates inside it;
+- the network, which scores the gated candidates (its count features are the
+  gated counts).
+
+The labels come from CA runs: after a run every tracklet is labelled 1 when it
+is one of the two tracklets of a cell on a final road, else 0
+(itsca_train_collect).  The gates are fitted on plain runs; the training rows
+then come from gated runs whose threshold keeps every candidate.  Training runs
+the device's own forward/backward and AdamW kernels (itsca_train_epoch).  The
+logit threshold is then tuned on separate validation events by running the
+filtered pipeline and measuring the fraction of plain roads it still finds.
 
     python -m itsca.filter_training --out weights.json
 """
@@ -25,10 +34,12 @@ from . import MLP_PARAMS, GpuTracker, cell_keys, generate_events, road_tracks
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "filter_weights.json")
 
 
-def save_filter(path: str, params: np.ndarray, threshold: float, meta: Dict) -> None:
+def save_filter(path: str, params: np.ndarray, threshold: float, meta: Dict, gates=None) -> None:
     p = np.ascontiguousarray(params, dtype=np.float32)
     doc = {"format": "itsca tracklet filter 10-32-32-1", "params_f32_bits": ["%08x" % v for v in p.view(np.uint32)],
            "threshold": float(threshold), "meta": meta}
+    if gates is not None:
+        doc["phi_gates"] = [float(np.float32(g)) for g in gates]
     with open(path, "w") as f:
         json.dump(doc, f, indent=1)
 
@@ -44,10 +55,27 @@ def load_filter(path: Optional[str] = None):
     return p, float(doc["threshold"]), doc.get("meta", {})
 
 
-def use_filter(tracker: GpuTracker, path: Optional[str] = None) -> float:
-    """Loads a weights file into the tracker and turns the filter on."""
+def load_gates(path: Optional[str] = None):
+    """The per-layer phi gates of a weights file (float32 values), or None
+    when it has none."""
+    with open(path or DEFAULT_WEIGHTS) as f:
+        doc = json.load(f)
+    g = doc.get("phi_gates")
+    return None if g is None else [float(np.float32(v)) for v in g]
+
+
+def prepare(tracker: GpuTracker, path: Optional[str] = None) -> float:
+    """Loads a weights file's parameters and phi gates into the tracker (the
+    filter stays as it is); returns its threshold."""
     p, thr, _ = load_filter(path)
     tracker.set_mlp_params(p)
+    tracker.set_filter_gates(load_gates(path))
+    return thr
+
+
+def use_filter(tracker: GpuTracker, path: Optional[str] = None) -> float:
+    """Loads a weights file into the tracker and turns the filter on."""
+    thr = prepare(tracker, path)
     tracker.set_filter(True, thr)
     return thr
 
@@ -56,10 +84,21 @@ def _log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def collect(tracker: GpuTracker, preset: int, seeds: Sequence[int], stride: int, capacity: int, batch: int = 25) -> int:
-    """Plain runs over the events of `seeds` (in batches); every stride-th
-    tracklet becomes a training row (the phase rotates between batches)."""
-    tracker.set_filter(False)
+KEEP_ALL = -3.0e38  # a threshold that keeps every candidate the gates let through
+
+
+def collect(tracker: GpuTracker, preset: int, seeds: Sequence[int], stride: int, capacity: int, batch: int = 25,
+            gates=None) -> int:
+    """Runs over the events of `seeds` (in batches): plain, or with the phi
+    gates and a threshold that keeps every gated candidate; every stride-th
+    tracklet becomes a training row (the phase rotates between batches).
+    Gated runs score with the tracker's current parameters, which must be
+    finite."""
+    if gates is None:
+        tracker.set_filter(False)
+    else:
+        tracker.set_filter_gates(gates)
+        tracker.set_filter(True, KEEP_ALL)
     tracker.train_clear(capacity)
     seeds = list(seeds)
     for k in range(0, len(seeds), batch):
@@ -69,7 +108,31 @@ def collect(tracker: GpuTracker, preset: int, seeds: Sequence[int], stride: int,
         tracker.train_collect(stride, (k // batch) % stride)
         _log("  collected events %d..%d: %d rows" % (seeds[k], seeds[min(k + batch, len(seeds)) - 1],
                                                    tracker.train_size()))
+    tracker.set_filter(False)
     return tracker.train_size()
+
+
+def fit_gates(tracker: GpuTracker, preset: int, seeds: Sequence[int], margin: float = 2.0):
+    """Per tracklet layer: margin x the largest |dphi| of the positive rows of
+    plain runs over `seeds`, capped at the reference's 0.3 rad cut.  Read from
+    the rows' features: layer = f0 / 0.2, dphi = f1 * 0.3 (BF16, 2^-9
+    relative)."""
+    tracker.set_filter_gates(None)
+    tracker.set_filter(False)
+    tracker.set_events(generate_events(preset, seeds[0], 1))
+    tracker.run()
+    per_event = sum(sum(tracker.counts(i).tracklets) for i in range(tracker.num_items()))
+    n = collect(tracker, preset, seeds, 1, int(per_event * 1.2 * len(seeds)) + 1024)
+    X, Y = tracker.train_get(n)
+    f = (X[:, :2].astype(np.uint32) << 16).view(np.float32)
+    layer = np.rint(f[:, 0] / np.float32(0.2)).astype(int)
+    dphi = np.abs(f[:, 1]) * np.float32(0.3)
+    gates = []
+    for l in range(6):
+        pos = dphi[(layer == l) & (Y > 0)]
+        g = margin * float(pos.max()) if pos.size else 0.3
+        gates.append(float(np.float32(min(max(g, 1e-3), 0.3))))
+    return gates
 
 
 def train(tracker: GpuTracker, epochs: int, batch: int = 4096, lr: float = 3e-3, lr_min: float = 1e-4,
@@ -160,13 +223,14 @@ def _replay_ms(tracker: GpuTracker, steps: int = 10) -> float:
     return ms
 
 
-def sweep(events, params: np.ndarray, thresholds: Sequence[float]) -> List[Dict]:
+def sweep(events, params: np.ndarray, thresholds: Sequence[float], gates=None) -> List[Dict]:
     """The trade-off of the threshold on `events`: per threshold, the road
     recall of the plain tracks, tracks outside them, the tracklets and cells
     kept, and the time per run (graph replay, block 256, 8 workgroups per CU,
     one z sub-bin), with the plain form's time for reference."""
     tracker = GpuTracker(device=0)
     tracker.set_mlp_params(params)
+    tracker.set_filter_gates(gates)
     tracker.set_tuning(256, 0, 1)
     tracker.set_events(events)
     tracker.set_filter(False)
@@ -190,21 +254,28 @@ def sweep(events, params: np.ndarray, thresholds: Sequence[float]) -> List[Dict]
 
 def train_filter(tracker: GpuTracker, preset: int, seeds: Sequence[int], samples: float, val_seeds: Sequence[int],
                  epochs: int = 12, batch: int = 4096, lr: float = 3e-3, weight_decay: float = 1e-4,
-                 pos_weight: float = 4.0, init_seed: int = 7, target_recall: float = 0.999):
-    """The whole training: labels from plain runs over the events of `seeds`
-    (about `samples` rows), AdamW epochs from a fresh initialisation, and the
-    threshold tuned to `target_recall` of the plain roads of the `val_seeds`
-    events.  Returns (params, threshold, meta); the tracker keeps the trained
-    weights, filter off."""
+                 pos_weight: float = 4.0, init_seed: int = 7, target_recall: float = 0.999, gate_events: int = 50,
+                 gate_margin: float = 2.0):
+    """The whole training: the phi gates from plain runs over the first
+    `gate_events` events of `seeds` (gate_margin <= 0: no gates), labels from
+    gated runs over the events of `seeds` (about `samples` rows), AdamW epochs
+    from a fresh initialisation, and the threshold tuned to `target_recall` of
+    the plain roads of the `val_seeds` events.  Returns (params, threshold,
+    meta); meta["phi_gates"] holds the gates.  The tracker keeps the trained
+    weights and the gates, filter off."""
     seeds, val_seeds = list(seeds), list(val_seeds)
     t0 = time.time()
-    tracker.set_filter(False)
+    gates = fit_gates(tracker, preset, seeds[:max(gate_events, 1)], gate_margin) if gate_margin > 0 else None
+    _log("phi gates: %s (%.1f s)" % (gates, time.time() - t0))
+    tracker.mlp_init(init_seed)  # finite parameters for the gated runs, which keep every candidate
+    tracker.set_filter_gates(gates)
+    tracker.set_filter(gates is not None, KEEP_ALL)
     tracker.set_events(generate_events(preset, seeds[0], 1))
     tracker.run()
     per_event = sum(sum(tracker.counts(i).tracklets) for i in range(tracker.num_items()))
     stride = max(1, int(round(per_event * len(seeds) / samples)))
     cap = int(per_event * 1.1 * len(seeds) / stride) + 1024
-    n = collect(tracker, preset, seeds, stride, cap)
+    n = collect(tracker, preset, seeds, stride, cap, gates=gates)
     X, Y = tracker.train_get(n)
     _log("collected %d rows (stride %d, %d tracklets in the first event), %.4f positive, %.1f s"
          % (n, stride, per_event, Y.mean(), time.time() - t0))
@@ -225,7 +296,9 @@ def train_filter(tracker: GpuTracker, preset: int, seeds: Sequence[int], samples
             "validation_road_recall": best["recall"], "validation_tracklets_kept": best["tracklets_kept"],
             "validation_cells_kept": best["cells_kept"],
             "training_row_recall": float(np.mean(z[Y > 0] >= best["threshold"])),
-            "training_rows_kept": float(np.mean(z >= best["threshold"]))}
+            "training_rows_kept": float(np.mean(z >= best["threshold"])), "phi_gates": gates,
+            "gate_events": "preset %d seeds %d..%d" % (preset, seeds[0], seeds[min(gate_events, len(seeds)) - 1]),
+            "gate_margin": gate_margin}
     return params, best["threshold"], meta
 
 
@@ -253,13 +326,16 @@ def main(argv=None):
     ap.add_argument("--pos-weight", type=float, default=4.0)
     ap.add_argument("--init-seed", type=int, default=7)
     ap.add_argument("--target-recall", type=float, default=0.999, help="road recall on the validation events")
+    ap.add_argument("--gate-events", type=int, default=50, help="events whose positive tracklets set the phi gates")
+    ap.add_argument("--gate-margin", type=float, default=2.0,
+                    help="phi gate = this x the largest |dphi| of a positive tracklet (<= 0: no gates)")
     args = ap.parse_args(argv)
 
     if args.sweep:
         params, thr, _ = load_filter(args.out)
         events = generate_events(args.preset, args.sweep_first_seed, args.sweep_events)
         thresholds = [thr + d for d in args.sweep_offsets]
-        rows = sweep(events, params, thresholds)
+        rows = sweep(events, params, thresholds, load_gates(args.out))
         print(json.dumps({"events": "preset %d seeds %d..%d" % (args.preset, args.sweep_first_seed,
                                                                   args.sweep_first_seed + args.sweep_events - 1),
                           "shipped_threshold": thr, "rows": rows}))
@@ -271,8 +347,8 @@ def main(argv=None):
         tracker, args.preset, range(args.first_seed, args.first_seed + args.events), args.samples,
         range(args.val_first_seed, args.val_first_seed + args.val_events), epochs=args.epochs, batch=args.batch,
         lr=args.lr, weight_decay=args.weight_decay, pos_weight=args.pos_weight, init_seed=args.init_seed,
-        target_recall=args.target_recall)
-    save_filter(args.out, params, threshold, meta)
+        target_recall=args.target_recall, gate_events=args.gate_events, gate_margin=args.gate_margin)
+    save_filter(args.out, params, threshold, meta, meta.get("phi_gates"))
     _log(json.dumps(dict(meta, threshold=threshold, seconds=round(time.time() - t0, 1))))
     tracker.close()
 
