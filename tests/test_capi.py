@@ -69,3 +69,20 @@ def test_native_weights_loader_matches_python():
                                  100, ctypes.byref(t)) == -1
     assert lib.itsca_filter_load(b"/nonexistent/w.json", out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                                  itsca.MLP_PARAMS, ctypes.byref(t)) == -1
+    # the per-layer phi gates: the same float32 values; a file without them gives 0.3 (no gate)
+    gates = np.zeros(itsca.TRACKLET_LAYERS, dtype=np.float32)
+    gp = gates.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    assert lib.itsca_filter_load_gates(ft.DEFAULT_WEIGHTS.encode(), gp, itsca.TRACKLET_LAYERS) == 0
+    want = ft.load_gates()
+    assert want is not None and np.array_equal(gates, np.array(want, dtype=np.float32))
+    assert all(0.0 < g <= 0.3 for g in want)
+    import json
+    import tempfile
+    with tempfile.NamedTemporaryFile("w", suffix=".json", delete=False) as f:
+        doc = json.load(open(ft.DEFAULT_WEIGHTS))
+        doc.pop("phi_gates")
+        json.dump(doc, f)
+    assert lib.itsca_filter_load_gates(f.name.encode(), gp, itsca.TRACKLET_LAYERS) == 0
+    assert np.array_equal(gates, np.full(itsca.TRACKLET_LAYERS, 0.3, dtype=np.float32))
+    assert ft.load_gates(f.name) is None
+    os.unlink(f.name)

@@ -36,6 +36,7 @@ DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "filt
 
 def save_filter(path: str, params: np.ndarray, threshold: float, meta: Dict, gates=None) -> None:
     p = np.ascontiguousarray(params, dtype=np.float32)
+    meta = {k: v for k, v in meta.items() if k != "phi_gates"}  # the gates are kept once, at the top level
     doc = {"format": "itsca tracklet filter 10-32-32-1", "params_f32_bits": ["%08x" % v for v in p.view(np.uint32)],
            "threshold": float(threshold), "meta": meta}
     if gates is not None:
