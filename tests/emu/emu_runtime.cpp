@@ -5,15 +5,23 @@
 // with its own stack, and the fibers are resumed round-robin.  A barrier
 // (workgroup or wavefront) is a counter: a fiber that arrives before the last
 // one yields until the barrier's phase advances.  Stack switches are announced
-// to AddressSanitizer.
+// to AddressSanitizer.  With EMU_THREADS=n (n > 1) a launch deals its
+// workgroups to a pool of n OS threads, each with its own fibers and its own
+// copy of the LDS (thread_local), so workgroups of one launch run at the same
+// time and interleave their global-memory accesses; EMU_ORDER=reverse deals
+// them from the last workgroup down.
 #include <sanitizer/common_interface_defs.h>
 #include <sys/mman.h>
 #include <ucontext.h>
 
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <memory>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "hip/hip_runtime.h"
@@ -45,7 +53,7 @@ struct Sched {
 };
 Sched* sched()
 {
-  static Sched* s = [] {
+  thread_local Sched* s = [] {
     Sched* p = new Sched();
     void* m = mmap(nullptr, kMaxThreads * kStack, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE,
                    -1, 0);
@@ -219,6 +227,72 @@ emu_f32x4 mfma16x16x32bf16(emu_bf16x8 a, emu_bf16x8 b, emu_f32x4 c)
   return d;
 }
 
+namespace {
+int emuThreads()
+{
+  static const int n = [] {
+    const char* e = std::getenv("EMU_THREADS");
+    const int v = e ? std::atoi(e) : 1;
+    return v < 1 ? 1 : (v > 64 ? 64 : v);
+  }();
+  return n;
+}
+
+bool emuReverse()
+{
+  static const bool r = [] {
+    const char* e = std::getenv("EMU_ORDER");
+    return e && std::strcmp(e, "reverse") == 0;
+  }();
+  return r;
+}
+
+// Worker threads that run the workgroups of one launch at a time; they live
+// until the process ends.
+struct Pool {
+  std::mutex m;
+  std::condition_variable go, done;
+  const std::function<void(unsigned)>* job = nullptr;
+  unsigned blocks = 0;
+  std::atomic<unsigned> next{0};
+  int active = 0;
+  unsigned long long gen = 0;
+  explicit Pool(int n)
+  {
+    for (int i = 0; i < n; ++i) std::thread([this] { loop(); }).detach();
+  }
+  void loop()
+  {
+    unsigned long long seen = 0;
+    for (;;) {
+      const std::function<void(unsigned)>* j;
+      unsigned nb;
+      {
+        std::unique_lock<std::mutex> lk(m);
+        go.wait(lk, [&] { return gen != seen; });
+        seen = gen;
+        j = job;
+        nb = blocks;
+      }
+      for (unsigned k = next++; k < nb; k = next++) (*j)(k);
+      std::lock_guard<std::mutex> lk(m);
+      if (--active == 0) done.notify_all();
+    }
+  }
+  void run(unsigned nb, int workers, const std::function<void(unsigned)>& f)
+  {
+    std::unique_lock<std::mutex> lk(m);
+    job = &f;
+    blocks = nb;
+    next = 0;
+    active = workers;
+    ++gen;
+    go.notify_all();
+    done.wait(lk, [&] { return active == 0; });
+  }
+};
+}  // namespace
+
 void launch(dim3 grid, dim3 block, const std::function<void()>& body)
 {
   const unsigned nt = block.x;
@@ -226,9 +300,12 @@ void launch(dim3 grid, dim3 block, const std::function<void()>& body)
     std::fprintf(stderr, "emu: workgroup of %u threads\n", nt);
     std::abort();
   }
-  Sched* s = sched();
-  s->body = &body;
-  for (unsigned b = 0; b < grid.x * grid.y * grid.z; ++b) {
+  const unsigned nb = grid.x * grid.y * grid.z;
+  const bool reverse = emuReverse();
+  auto runBlock = [&](unsigned k) {
+    const unsigned b = reverse ? nb - 1 - k : k;
+    Sched* s = sched();
+    s->body = &body;
     Block blk;
     blk.all.n = (int)nt;
     const unsigned nw = (nt + 63) / 64;
@@ -273,7 +350,14 @@ void launch(dim3 grid, dim3 block, const std::function<void()>& body)
       }
     }
     cur = nullptr;
+  };
+  const int workers = emuThreads();
+  if (workers <= 1 || nb <= 1) {
+    for (unsigned k = 0; k < nb; ++k) runBlock(k);
+    return;
   }
+  static Pool* pool = new Pool(workers);  // never destroyed: its threads live until the process ends
+  pool->run(nb, workers, runBlock);
 }
 }  // namespace emu
 

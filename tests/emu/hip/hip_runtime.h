@@ -4,7 +4,10 @@
 // Compiling the product's .hip files for the host with clang++ against this header runs every
 // kernel on the CPU: one fiber per GPU thread, a cooperative barrier per
 // workgroup for __syncthreads, a per-wavefront exchange for __shfl_up, and
-// plain host memory for device buffers.  Built with AddressSanitizer, an
+// plain host memory for device buffers.  Workgroups run one after another, or
+// with EMU_THREADS=n on n OS threads at once (EMU_ORDER=reverse deals them from
+// the last one), so that work of different workgroups interleaves as it does
+// on the GPU.  Built with AddressSanitizer, an
 // out-of-bounds access in a kernel becomes an ASan report.  Used to check
 // kernel logic on machines without a GPU; never a product path.
 #pragma once
@@ -22,7 +25,9 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define amdgpu_waves_per_eu(...)
-#define __shared__ static
+// LDS: one copy per OS thread, i.e. per workgroup in flight (EMU_THREADS > 1
+// runs workgroups concurrently on several OS threads)
+#define __shared__ static thread_local
 #define __align__(n) __attribute__((aligned(n)))
 
 struct dim3 {

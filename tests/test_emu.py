@@ -6,7 +6,8 @@ fixtures bit for bit under several tunings (launch shapes, z sub-bins and phi
 sub-rows of the search index), the learned filter's paths (scored
 after the search and inside it), an event
 of 1831 clusters per layer and an event whose tracklets overflow the count's
-slots, against the oracle."""
+slots, against the oracle; and the same with the workgroups of each launch
+running concurrently on several OS threads."""
 import glob
 import os
 import subprocess
@@ -174,6 +175,74 @@ def test_kernels_under_cpu_emulation_with_asan():
     env["LD_PRELOAD"] = rt + (":" + env["LD_PRELOAD"] if env.get("LD_PRELOAD") else "")
     env["ASAN_OPTIONS"] = "detect_leaks=0"
     out = subprocess.run([sys.executable, "-c", "ROOT = %r\n" % ROOT + SCRIPT], env=env, capture_output=True,
+                         text=True, timeout=1200)
+    assert out.returncode == 0 and "ALL OK" in out.stdout, out.stdout[-2000:] + out.stderr[-6000:]
+
+
+CONCURRENT_SCRIPT = r"""
+import ctypes, os, sys
+import numpy as np
+sys.path[:0] = [os.path.join(ROOT, "tracking-itsu_amd"), os.path.join(ROOT, "tests"), os.path.join(ROOT, "oracle")]
+import itsca, pyoracle
+from golden_cases import dense_spot_event, fixture, small_cases
+from gpu_items import gated_tracklets
+assert os.environ.get("EMU_THREADS") == "4"
+lib = itsca.bind_hip(ctypes.CDLL(os.path.join(ROOT, "tests", "emu", "libitsca_emu.so")))
+cases = {n: p for n, p, _ in small_cases()}
+# fixtures bit for bit, several workgroups per CU so that chunks of one stage interleave
+for name, tuning in (("small_s1", (256, 16, 1, 1)), ("sparse_s6", (64, 16, 1, 1)), ("pileup2_s7", (128, 16, 3, 2))):
+    ev = itsca.load_events(cases[name])[0]
+    t = itsca.GpuTracker(persistent_blocks=16, lib=lib)
+    t.set_tuning(*tuning)
+    t.set_events([ev])
+    t.run()
+    for v in range(len(ev.vertices)):
+        assert itsca.dump_item(t, v, ev.vertices[v]) == fixture(name, "stable", v), (name, v)
+    if name == "small_s1":  # the learned filter: -inf = plain; gated, keeping all = the gated plain tracklets
+        plain = [t.download(0, itsca.TRACKLETS, l, t.counts(0).tracklets[l], 4) for l in range(6)]
+        t.mlp_init(3)
+        t.set_filter(True, -3.0e38)
+        t.run()
+        assert itsca.dump_item(t, 0, ev.vertices[0]) == fixture(name, "stable", 0)
+        gates = [0.02, 0.02, 0.15, 0.05, 0.1, 0.05]
+        t.set_filter_gates(gates)
+        t.run()
+        want = gated_tracklets(t, 0, plain, gates)
+        n = t.counts(0)
+        for l in range(6):
+            assert np.array_equal(t.download(0, itsca.TRACKLETS, l, n.tracklets[l], 4)[:, :2], want[l]), l
+    t.close()
+    print(name, "ok", flush=True)
+# an event of 1831 clusters per layer and the slot-overflow event, against the oracle
+for ev in (itsca.generate_events(3, 3006, 1)[0], dense_spot_event()):
+    t = itsca.GpuTracker(persistent_blocks=16, lib=lib)
+    t.set_tuning(256, 16, 1, 1)
+    t.set_events([ev])
+    t.run()
+    assert itsca.dump_item(t, 0, ev.vertices[0]) == pyoracle.OracleVertex(ev, 0, stable=True).dump()
+    t.close()
+print("oracle events ok", flush=True)
+print("ALL OK")
+"""
+
+
+def test_concurrent_workgroups_under_cpu_emulation_with_asan():
+    """The same kernels with the workgroups of every launch run at once on four
+    OS threads (each with its own LDS), dealt from the last one down: the
+    fixtures and the oracle still match bit for bit, so no kernel depends on
+    the order in which workgroups run or on another workgroup's writes within
+    one launch."""
+    rt = _asan_runtime()
+    if not os.path.exists(CLANG) or rt is None:
+        pytest.skip("ROCm clang++ with its AddressSanitizer runtime is needed")
+    build = subprocess.run(["make", "-C", EMU], capture_output=True, text=True, timeout=1200)
+    assert build.returncode == 0, build.stdout[-2000:] + build.stderr[-4000:]
+    env = dict(os.environ)
+    env["LD_PRELOAD"] = rt + (":" + env["LD_PRELOAD"] if env.get("LD_PRELOAD") else "")
+    env["ASAN_OPTIONS"] = "detect_leaks=0"
+    env["EMU_THREADS"] = "4"
+    env["EMU_ORDER"] = "reverse"
+    out = subprocess.run([sys.executable, "-c", "ROOT = %r\n" % ROOT + CONCURRENT_SCRIPT], env=env, capture_output=True,
                          text=True, timeout=1200)
     assert out.returncode == 0 and "ALL OK" in out.stdout, out.stdout[-2000:] + out.stderr[-6000:]
 
