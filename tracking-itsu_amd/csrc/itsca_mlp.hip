@@ -293,44 +293,56 @@ __global__ void __launch_bounds__(256) kTrainGrad(const float* wq, const uint16_
   float lossAcc = 0.0f, weightAcc = 0.0f;
   const int per = (count + gridDim.x - 1) / gridDim.x;
   const int begin = blockIdx.x * per, end = min(count, begin + per);
+  // Per tile of kTile samples, the forward and backward passes: sample
+  // s = threadIdx / 4, its four threads each computing a quarter of every
+  // hidden layer's units (unit o by thread (s, o / 8)); the logit, loss and
+  // output gradient by the first of the four.  Every unit's sum runs in the
+  // same order as a one-thread-per-sample pass would run it.
+  constexpr int kParts = 256 / kTile, kUnits = kMlpHidden / kParts;
+  static_assert(kParts * kTile == 256 && kUnits * kParts == kMlpHidden, "four threads per sample");
+  const int sIdx = threadIdx.x / kParts, part = threadIdx.x % kParts;
   for (int t0 = begin; t0 < end; t0 += kTile) {
-    const int sIdx = threadIdx.x;
-    if (sIdx < kTile) {
-      const int k = t0 + sIdx;
+    const int k = t0 + sIdx;
+    const bool live = k < end;
+    const long row = live ? (a * (long)(k0 + k) + b) % n : 0;
+    for (int f = part; f < kMlpIn; f += kParts) xs[sIdx][f] = live ? fromBf16(X[row * kMlpInPad + f]) : 0.0f;
+    __syncthreads();
+    for (int o = part * kUnits; o < (part + 1) * kUnits; ++o) {
+      float v = W[kOffB1 + o];
+      for (int f = 0; f < kMlpIn; ++f) v += W[kOffW1 + o * kMlpIn + f] * xs[sIdx][f];
+      h1[sIdx][o] = live ? roundBf16(v > 0.0f ? v : 0.0f) : 0.0f;
+    }
+    __syncthreads();
+    for (int o = part * kUnits; o < (part + 1) * kUnits; ++o) {
+      float v = W[kOffB2 + o];
+      for (int q = 0; q < kMlpHidden; ++q) v += W[kOffW2 + o * kMlpHidden + q] * h1[sIdx][q];
+      h2[sIdx][o] = live ? roundBf16(v > 0.0f ? v : 0.0f) : 0.0f;
+    }
+    __syncthreads();
+    if (part == 0) {
       float loss = 0.0f, weight = 0.0f, d = 0.0f;
-      for (int o = 0; o < kMlpHidden; ++o) h1[sIdx][o] = h2[sIdx][o] = g1[sIdx][o] = g2[sIdx][o] = 0.0f;
-      for (int f = 0; f < kMlpIn; ++f) xs[sIdx][f] = 0.0f;
-      if (k < end) {
-        const long row = (a * (long)(k0 + k) + b) % n;
-        for (int f = 0; f < kMlpIn; ++f) xs[sIdx][f] = fromBf16(X[row * kMlpInPad + f]);
-        for (int o = 0; o < kMlpHidden; ++o) {
-          float v = W[kOffB1 + o];
-          for (int f = 0; f < kMlpIn; ++f) v += W[kOffW1 + o * kMlpIn + f] * xs[sIdx][f];
-          h1[sIdx][o] = roundBf16(v > 0.0f ? v : 0.0f);
-        }
+      if (live) {
         float z = W[kOffB3];
-        for (int o = 0; o < kMlpHidden; ++o) {
-          float v = W[kOffB2 + o];
-          for (int q = 0; q < kMlpHidden; ++q) v += W[kOffW2 + o * kMlpHidden + q] * h1[sIdx][q];
-          h2[sIdx][o] = roundBf16(v > 0.0f ? v : 0.0f);
-          z += W[kOffW3 + o] * h2[sIdx][o];
-        }
+        for (int o = 0; o < kMlpHidden; ++o) z += W[kOffW3 + o] * h2[sIdx][o];
         const float y = Y[row] ? 1.0f : 0.0f;
         weight = Y[row] ? posWeight : 1.0f;
         const float sp = z > 0.0f ? z + log1pf(expf(-z)) : log1pf(expf(z));  // softplus
         loss = weight * (sp - y * z);
         const float prob = 1.0f / (1.0f + expf(-z));
         d = weight * (prob - y);
-        for (int o = 0; o < kMlpHidden; ++o) g2[sIdx][o] = h2[sIdx][o] > 0.0f ? d * W[kOffW3 + o] : 0.0f;
-        for (int q = 0; q < kMlpHidden; ++q) {
-          float v = 0.0f;
-          for (int o = 0; o < kMlpHidden; ++o) v += g2[sIdx][o] * W[kOffW2 + o * kMlpHidden + q];
-          g1[sIdx][q] = h1[sIdx][q] > 0.0f ? v : 0.0f;
-        }
       }
       dz[sIdx] = d;
       lossPart[sIdx] = loss;
       weightPart[sIdx] = weight;
+    }
+    __syncthreads();
+    const float d = dz[sIdx];
+    for (int o = part * kUnits; o < (part + 1) * kUnits; ++o) g2[sIdx][o] = h2[sIdx][o] > 0.0f ? d * W[kOffW3 + o] : 0.0f;
+    __syncthreads();
+    for (int q = part * kUnits; q < (part + 1) * kUnits; ++q) {
+      float v = 0.0f;
+      for (int o = 0; o < kMlpHidden; ++o) v += g2[sIdx][o] * W[kOffW2 + o * kMlpHidden + q];
+      g1[sIdx][q] = h1[sIdx][q] > 0.0f ? v : 0.0f;
     }
     __syncthreads();
     for (int q = 0; q < (kMlpParams + 255) / 256; ++q) {
