@@ -1395,6 +1395,7 @@ struct CellOut {
 struct CellSeed {
   int l, tnBase, cl2Base, uFirst, nCand;
   float x1, y1, z1, r1, r1sq, x2, y2, r2sq, d1x, d1y, d1z, vx, vy, vz, tTanL, tPhi;
+  float maxDeltaZ, maxDCA;  // the layer's cuts, read once per tracklet
 };
 
 __device__ __forceinline__ CellSeed cellSeedOf(const DevState& s, int w, int l, int2 fs, float2 tp);
@@ -1435,6 +1436,8 @@ __device__ __forceinline__ CellSeed cellSeedOf(const DevState& s, int w, int l, 
   c.d1z = c.r2sq - c.r1sq;
   c.tTanL = tp.x;
   c.tPhi = tp.y;
+  c.maxDeltaZ = cellMaxDeltaZ(l);
+  c.maxDCA = cellMaxDCA(l);
   return c;
 }
 
@@ -1443,17 +1446,27 @@ __device__ __forceinline__ CellSeed cellSeedOf(const DevState& s, int w, int l, 
 // The reference evaluates the z cut only after the first two pass; the cuts
 // have no side effects, so all three are evaluated and combined without
 // branches (same comparisons, same result).
+//
+// kWrap: the second phi comparison (|dphi - 2pi| < cut) is evaluated.
+// Tracklet phis are atan2f values in [-pi, pi] (Tracklet.cxx:36), so for a
+// tracklet with |phi| <= kCellWrapFree, |dphi| <= 5.95 and |dphi - 2pi| >=
+// 0.34 > kCellMaxDeltaPhi: that comparison is false, and callers whose
+// tracklets all lie there may leave it out.
+constexpr float kCellWrapFree = 2.8f;
+
+template <bool kWrap = true>
 __device__ __forceinline__ bool cellCheapCuts(const CellSeed& c, float2 up)
 {
   const float uTanL = up.x, uPhi = up.y;
   const float deltaTanLambda = fabsf(c.tTanL - uTanL);
   const float deltaPhi = fabsf(c.tPhi - uPhi);
-  const bool continuity = (deltaTanLambda < kCellMaxDeltaTanLambda) &
-                          ((deltaPhi < kCellMaxDeltaPhi) | (fabsf(deltaPhi - kTwoPi) < kCellMaxDeltaPhi));
+  const bool phiOk = kWrap ? ((deltaPhi < kCellMaxDeltaPhi) | (fabsf(deltaPhi - kTwoPi) < kCellMaxDeltaPhi))
+                           : (deltaPhi < kCellMaxDeltaPhi);
+  const bool continuity = (deltaTanLambda < kCellMaxDeltaTanLambda) & phiOk;
   const float avgTanLambda = 0.5f * (c.tTanL + uTanL);
   const float dirZ = -avgTanLambda * c.r1 + c.z1;
   const float deltaZ = fabsf(dirZ - c.vz);
-  return continuity & (deltaZ < cellMaxDeltaZ(c.l));
+  return continuity & (deltaZ < c.maxDeltaZ);
 }
 
 // Conformal-plane fit and DCA cut (Tracker.cxx:175-218, MathUtils.h:53-60)
@@ -1475,7 +1488,7 @@ __device__ __forceinline__ bool cellFitAt(const CellSeed& c, float4 q3, CellOut&
   const float radius = sqrtf((1.0f - hz2 - 4.0f * planeDistance * hz) / (4.0f * hz2));
   const float cx = -0.5f * hx / hz, cy = -0.5f * hy / hz;
   const float dca = fabsf(radius - sqrtf(cx * cx + cy * cy));
-  if (dca > cellMaxDCA(c.l)) return false;  // NaN passes, as in the reference CPU path
+  if (dca > c.maxDCA) return false;  // NaN passes, as in the reference CPU path
   o.nx = hx;
   o.ny = hy;
   o.nz = hz;
@@ -1499,11 +1512,11 @@ __device__ __forceinline__ bool cellFit(const DevState& s, const CellSeed& c, in
 // events 0.2 % of the layer-1 tracklets have 9-13 survivors.
 constexpr int kCellList = 16;
 constexpr int kFitBatch = 4;  // survivors whose fit loads are issued together
-constexpr int kCandBlock = 8;  // candidates whose loads are issued together (four 16-byte loads)
-static_assert(kCandBlock % 2 == 0 && kCandBlock - 1 <= kTrackletPad, "the last block reads past the range");
+constexpr int kCandBlockMax = 8;  // candidates whose loads are issued together (up to four 16-byte loads)
+static_assert(kCandBlockMax - 1 <= kTrackletPad, "the last block reads past the range");
 
 // Two consecutive (tanLambda, phi) records in one 16-byte load (8-byte
-// aligned).  The last block of a candidate range reads up to kCandBlock - 1
+// aligned).  The last block of a candidate range reads up to kCandBlockMax - 1
 // records past it -- masked out, and inside the kTrackletPad records that
 // every tracklet array is allocated beyond its capacity.
 __device__ __forceinline__ void loadPair(const float2* p, float2& a, float2& b)
@@ -1517,9 +1530,15 @@ __device__ __forceinline__ void loadPair(const float2* p, float2& a, float2& b)
 // Count: the cheap cuts run over every candidate; survivors (usually 0-2) are
 // fitted in a second loop, so the expensive fit does not sit inside the
 // divergent candidate loop.  Accepted candidates are kept in kCellSlots slots.
-template <int B>
+// kCandBlock: candidates whose loads are issued together; kWrapFree: wavefronts
+// whose tracklets are all away from phi = +-pi run the cuts without the wrap
+// comparison.  The plain form (10 candidates per tracklet on central events)
+// runs <8, true>; the filtered form (about 2) runs <4, false>, whose smaller
+// register file keeps eight waves per SIMD.
+template <int B, int kCandBlock, bool kWrapFree>
 __global__ void __launch_bounds__(B) kCellsCount(DevState s)
 {
+  static_assert(kCandBlock % 2 == 0 && kCandBlock <= kCandBlockMax, "pairs of records, within the padding");
   __shared__ int scratch[B / 64 + 1];
   __shared__ int list[kCellList][B];
   const int S = s.W * kCellLayers;
@@ -1549,6 +1568,8 @@ __global__ void __launch_bounds__(B) kCellsCount(DevState s)
     const int w = sg / kCellLayers, l = sg % kCellLayers;
     const Seg tl = s.trkSeg[w * kTrackletLayers + l];
     const int it = ci.y + threadIdx.x;
+    // every tracklet of the wavefront away from phi = +-pi: the wrap comparison is left out
+    const bool waveWrapFree = kWrapFree && __ballot(it < tl.count && !(fabsf(rec.tp.y) <= kCellWrapFree)) == 0ull;
     int cnt = 0;
     if (it < tl.count) {
       const int t = tl.base + it;
@@ -1578,18 +1599,36 @@ __global__ void __launch_bounds__(B) kCellsCount(DevState s)
       // Blocks of kCandBlock candidates: loads first (indices past the range
       // clamped and masked out), cuts without branches, survivors in
       // ascending order.
-      for (int ib = c.uFirst; ib < uEnd; ib += kCandBlock) {
-        float2 up[kCandBlock];
+      // (full blocks need no range test per candidate; the last one masks
+      // the candidates past the range)
+      auto candidates = [&](auto wrap) {
+        constexpr bool kWrap = decltype(wrap)::value;
+        auto block = [&](int ib, auto full) {
+          constexpr bool kFull = decltype(full)::value;
+          float2 up[kCandBlock];
 #pragma unroll
-        for (int k = 0; k < kCandBlock; k += 2) loadPair(s.tTP + c.tnBase + ib + k, up[k], up[k + 1]);
-        uint32_t mask = 0;
+          for (int k = 0; k < kCandBlock; k += 2) loadPair(s.tTP + c.tnBase + ib + k, up[k], up[k + 1]);
+          uint32_t mask = 0;
 #pragma unroll
-        for (int k = 0; k < kCandBlock; ++k) mask |= ((cellCheapCuts(c, up[k]) & (ib + k < uEnd)) ? 1u : 0u) << k;
-        while (mask != 0u) {
-          const int k = __builtin_ctz(mask);
-          mask &= mask - 1u;
-          survivor(ib + k);
-        }
+          for (int k = 0; k < kCandBlock; ++k)
+            mask |= ((cellCheapCuts<kWrap>(c, up[k]) & (kFull || ib + k < uEnd)) ? 1u : 0u) << k;
+          while (mask != 0u) {
+            const int k = __builtin_ctz(mask);
+            mask &= mask - 1u;
+            survivor(ib + k);
+          }
+        };
+        int ib = c.uFirst;
+        for (; ib + kCandBlock <= uEnd; ib += kCandBlock) block(ib, std::true_type{});
+        if (ib < uEnd) block(ib, std::false_type{});
+      };
+      if constexpr (kWrapFree) {
+        if (waveWrapFree)
+          candidates(std::false_type{});
+        else
+          candidates(std::true_type{});
+      } else {
+        candidates(std::true_type{});
       }
       if (m <= kCellList) {  // survivors fitted kFitBatch at a time, their loads issued together
         for (int q0 = 0; q0 < m; q0 += kFitBatch) {
@@ -2137,7 +2176,10 @@ void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t str
   hipLaunchKernelGGL(kPlanCells, dim3(1), dim3(1024), 0, stream, s, counts);
   hipLaunchKernelGGL(kChunkSegs, dim3(segBlocks(s.tCap)), dim3(256), 0, stream, s, (int)kStCells, s.W * kCellLayers, 1, 0);
   done("kPlanCells", 2);
-  hipLaunchKernelGGL(kCellsCount<B>, dim3(G), dim3(B), 0, stream, s);
+  if (cfg.filter)
+    hipLaunchKernelGGL((kCellsCount<B, 4, false>), dim3(G), dim3(B), 0, stream, s);
+  else
+    hipLaunchKernelGGL((kCellsCount<B, 8, true>), dim3(G), dim3(B), 0, stream, s);
   done("kCellsCount", 2);
   hipLaunchKernelGGL(kScanChunks, dim3(1), dim3(1024), 0, stream, s, (int)kStCells, s.W * kCellLayers, s.celSeg,
                      s.cCap, (int)kStatusOverflowCells);
