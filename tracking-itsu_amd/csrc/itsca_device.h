@@ -118,6 +118,7 @@ struct DevState {
   float phiGate[kTrackletLayers];
   int gated;
   int* trkCand;          // kTrkSlots per sorted cluster
+  int* trkKeep;          // filtered form: per chunk of clusters, their kept candidates as one list (chunk's first cluster * kTrkSlots)
   Seg* trkSeg;           // W*6
   int2* tFS;    // (first cluster, second cluster) positions
   float2* tTP;  // (tanLambda, phi): one load per cell candidate

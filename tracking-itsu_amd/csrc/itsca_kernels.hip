@@ -1260,6 +1260,7 @@ kTrackletsSearchScore(DevState s, int layer, float threshold)
       }
       const int keptCount = scored == raw ? kept : raw;
       if (i < cur.count) s.trkKept[p] = keptCount;
+      writeKeptList(keptListOf(s.trkKeep, cur.base + first, wave), myRow, (i < cur.count && raw <= kTrkSlots) ? kept : 0);
       int sum = i < cur.count ? keptCount : 0;
       for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
       if (lane == 0) waveKept[wave] = sum;
@@ -1317,7 +1318,7 @@ __device__ __forceinline__ void forEachSlotOutput(int capped, int exc, int total
 // recomputed by their own thread.  Six waves per SIMD: the overflow search
 // tests its candidates two at a time so that the kernel fits that many
 // without spills.
-template <int B>
+template <int B, bool kKeptList>
 __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) kTrackletsFill(DevState s)
 {
   __shared__ int scratch[B / 64 + 1];
@@ -1352,7 +1353,11 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) kTr
     const int rw = s.recItem[w];
     const Seg rcur = s.clSeg[rw * kLayers + l], rnext = s.clSeg[rw * kLayers + l + 1];
     forEachSlotOutput<B>(capped, exc, totalCap, exCap, owner, [&](int c, int k) {
-      const int j = s.trkCand[(size_t)(cur.base + first + c) * kTrkSlots + k];
+      // the filtered form reads its wavefront's kept list (the scoring wrote it in
+      // this order: offset = the capped counts' scan over the wavefront), the plain
+      // form the slot rows
+      const int j = kKeptList ? keptListOf(s.trkKeep, cur.base + first, c >> 6)[exCap[c] - exCap[c & ~63] + k]
+                              : s.trkCand[(size_t)(cur.base + first + c) * kTrkSlots + k];
       writeTracklet(s, chunkBase + exFull[c] + k, first + c, j, rcur.base, rnext.base);
     });
     if (cnt > kTrkSlots) {
@@ -2171,7 +2176,10 @@ void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t str
   hipLaunchKernelGGL(kScanChunks, dim3(1), dim3(1024), 0, stream, s, (int)kStTracklets, s.W * kTrackletLayers,
                      s.trkSeg, s.tCap, (int)kStatusOverflowTracklets);
   done("kScanChunks", 1);
-  hipLaunchKernelGGL(kTrackletsFill<B>, dim3(G), dim3(B), 0, stream, s);
+  if (cfg.filter)
+    hipLaunchKernelGGL((kTrackletsFill<B, true>), dim3(G), dim3(B), 0, stream, s);
+  else
+    hipLaunchKernelGGL((kTrackletsFill<B, false>), dim3(G), dim3(B), 0, stream, s);
   done("kTrackletsFill", 1);
   hipLaunchKernelGGL(kPlanCells, dim3(1), dim3(1024), 0, stream, s, counts);
   hipLaunchKernelGGL(kChunkSegs, dim3(segBlocks(s.tCap)), dim3(256), 0, stream, s, (int)kStCells, s.W * kCellLayers, 1, 0);

@@ -19,12 +19,12 @@ namespace {
 // o); the slots are taken in rounds of kWaveRound: (1) an LDS map o -> owning
 // lane; (2) 64 candidates at a time are scored (features, BF16 MFMA network),
 // the candidate values kept in LDS and the keep decisions as ballot words;
-// (3) each lane writes the kept candidates of its own cluster back into the
-// cluster's slot row, in order (a kept position never exceeds the candidate's
-// own position, and the round's slots were all read before, so nothing still
-// to be read is overwritten).  Clusters whose candidates overflowed their
-// slots are kept whole.  Kept counts go to trkKept (trkCnt stays the raw count
-// that the features read) and the chunk sums are rewritten.  The LDS arrays
+// (3) each lane counts the kept candidates of its own cluster, and the round's
+// kept candidates are appended, in slot order (cluster after cluster), to the
+// wavefront's kept list (keptListOf), 64 slots per ballot: coalesced stores,
+// read in order by the tracklet fill.  Clusters whose candidates overflowed
+// their slots are kept whole.  Kept counts go to trkKept (trkCnt stays the raw
+// count that the features read) and the chunk sums are rewritten.  The LDS arrays
 // allow four workgroups of 256 per CU; the waves-per-SIMD hint holds the
 // registers to that occupancy.
 // ---------------------------------------------------------------------------
@@ -84,8 +84,8 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) kTr
       const float* tv = reinterpret_cast<const float*>(&t);
       for (int k = 0; k < kSeedFloats; ++k) seeds[k][threadIdx.x] = tv[k];
     }
-    int kept = 0;
-    int* myRow = s.trkCand + (size_t)p * kTrkSlots;
+    int kept = 0, listed = 0;
+    int* waveList = keptListOf(s.trkKeep, cur.base + first, wave);
     for (int r0 = 0; r0 < total; r0 += kWaveRound) {
       const int rn = min(kWaveRound, total - r0);
       for (int k = max(0, r0 - ex); k < scored && ex + k < r0 + rn; ++k) owner[wave][ex + k - r0] = (uint8_t)lane;
@@ -141,18 +141,23 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) kTr
         qCur = qNext;
         jNext = jAfter;
       }
-      // this lane's candidates in the round: [a, b); kept ones go to the row in order
+      // this lane's kept candidates in the round: the set bits of its slots [a, b)
       const int a = max(ex - r0, 0), b = min(ex + scored - r0, rn);
       for (int wd = a >> 5; a < b && wd <= (b - 1) >> 5; ++wd) {
         uint32_t bits = keep[wave][wd];
         const int lo = wd * 32 < a ? a - wd * 32 : 0, hi = (wd + 1) * 32 > b ? b - wd * 32 : 32;
         if (lo > 0) bits &= ~((1u << lo) - 1u);
         if (hi < 32) bits &= (1u << hi) - 1u;
-        while (bits != 0u) {
-          const int o = wd * 32 + __builtin_ctz(bits);
-          bits &= bits - 1u;
-          myRow[kept++] = candv[wave][o];
-        }
+        kept += __builtin_popcount(bits);
+      }
+      // the round's kept candidates, in slot order (cluster after cluster),
+      // appended to the wavefront's list: 64 slots at a time, ranked by a ballot
+      for (int t0 = 0; t0 < rn; t0 += 64) {
+        const int o = t0 + lane;
+        const bool k = o < rn && ((keep[wave][o >> 5] >> (o & 31)) & 1u);
+        const unsigned long long kb = __ballot(k);
+        if (k) waveList[listed + __builtin_popcountll(kb & ((1ull << lane) - 1ull))] = candv[wave][o];
+        listed += __builtin_popcountll(kb);
       }
       waveSync();  // owner, candv and keep are rebuilt by the next round
     }

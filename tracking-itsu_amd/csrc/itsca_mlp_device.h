@@ -214,6 +214,33 @@ struct CandRec {
   int count;
 };
 
+// The filtered form's kept candidates, as one list per wavefront of a chunk:
+// the list of wavefront w (lanes = clusters first + 64w ...) starts at
+// trkKeep[(chunk's first cluster + 64 w) * kTrkSlots] and holds the kept
+// positions of its clusters, cluster after cluster.  Clusters whose
+// candidates overflowed their slots have none (the fill recomputes them).
+// The tracklet fill finds cluster c's at the exclusive scan of the capped
+// kept counts over its wavefront.
+__device__ __forceinline__ int* keptListOf(int* trkKeep, int chunkFirst, int wave)
+{
+  return trkKeep + (size_t)(chunkFirst + 64 * wave) * kTrkSlots;
+}
+
+// Appends a cluster's `capped` kept positions (the start of its slot row
+// `row`) to its wavefront's list, at the exclusive scan of `capped` over the
+// wavefront.  Called by every lane of the wavefront.
+__device__ __forceinline__ void writeKeptList(int* waveList, const int* row, int capped)
+{
+  const int lane = threadIdx.x & 63;
+  int inc = capped;
+  for (int d = 1; d < 64; d <<= 1) {
+    const int t = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += t;
+  }
+  int* dst = waveList + inc - capped;
+  for (int n = 0; n < capped; ++n) dst[n] = row[n];
+}
+
 // Candidate slots a wavefront scores per round (an LDS map slot -> owning lane).
 constexpr int kWaveRound = 1024;
 constexpr int kSeedFloats = 8;  // TrackletSeed, staged in LDS per cluster

@@ -166,7 +166,7 @@ struct itsca_ctx {
   DBuf<int> cid, cmc, cbin, sBin, sInv, trkCnt, trkOff, trkKept, table;
   DBuf<float4> crzp, crzpS, segGeo;
   DBuf<uint4> subTab;
-  DBuf<int> cellCnt, cellOff, trkCand, cellCand;
+  DBuf<int> cellCnt, cellOff, trkCand, trkKeep, cellCand;
   DBuf<int2> tFS;
   DBuf<float2> tTP;
   DBuf<int> succCnt, succCand, eFrom, eTo, eTmp;
@@ -258,6 +258,7 @@ struct itsca_ctx {
     HIP_TRY(crzpS.reserve(SC));
     HIP_TRY(subTab.reserve((size_t)std::max(W, 1) * kLayers * kBins));
     HIP_TRY(trkCand.reserve(SC * kTrkSlots));
+    HIP_TRY(trkKeep.reserve(SC * kTrkSlots));
     HIP_TRY(table.reserve((size_t)W * kLayers * kTableSize));
     HIP_TRY(segGeo.reserve((size_t)std::max(W, 1) * kLayers));
     HIP_TRY(tFS.reserve(tCap));
@@ -393,6 +394,7 @@ struct itsca_ctx {
     s.mlpW = mlpWbf.p;
     setGates(s);
     s.trkCand = trkCand.p;
+    s.trkKeep = trkKeep.p;
     s.trkSeg = dTrkSeg.p;
     s.tFS = tFS.p;
     s.tTP = tTP.p;
@@ -790,7 +792,8 @@ namespace {
 // edges and roads in the proportions 10 : 1 : 1 : 1 (a central event has
 // about 9 tracklets, 1 edge and 0.2 roads per cell).  Returns false if the
 // clusters alone do not leave room for them.
-bool carveNoAlloc(unsigned char* base, size_t bytes, int n, DevState& s, int*& planScratch)
+// keptLists: the filter is on, so the chunks' kept lists (trkKeep) need room.
+bool carveNoAlloc(unsigned char* base, size_t bytes, int n, bool keptLists, DevState& s, int*& planScratch)
 {
   size_t off = 0;
   bool fits = true;
@@ -833,6 +836,7 @@ bool carveNoAlloc(unsigned char* base, size_t bytes, int n, DevState& s, int*& p
   s.trkOff = static_cast<int*>(take(N, 4));
   s.trkKept = static_cast<int*>(take(N, 4));
   s.trkCand = static_cast<int*>(take(N * kTrkSlots, 4));
+  s.trkKeep = keptLists ? static_cast<int*>(take(N * kTrkSlots, 4)) : nullptr;
   s.chunkStart = static_cast<int*>(take((size_t)kNumStages * (kLayers + 1), 4));
   if (!fits) return false;
   // bytes per unit k of capacity: 10 tracklets, 1 cell, 1 edge, 1 road, and
@@ -909,7 +913,8 @@ itsca_status itsca_run_noalloc(itsca_ctx* ctx, const itsca_device_event* ev, int
     if (!ctx->scratchPtr) return ctx->fail(ITSCA_ERR_STATE, "itsca_run_noalloc: no device scratch");
     DevState s{};
     int* planScratch = nullptr;
-    if (!carveNoAlloc(ctx->scratchPtr, ITSCA_NOALLOC_SCRATCH_BYTES, (int)n, s, planScratch))
+    const bool filtered = ctx->filterOn && ctx->mlpWbf.p != nullptr;
+    if (!carveNoAlloc(ctx->scratchPtr, ITSCA_NOALLOC_SCRATCH_BYTES, (int)n, filtered, s, planScratch))
       return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_run_noalloc: the event does not fit the device scratch");
     s.W = 1;
     s.chunk = ctx->blockSize;
