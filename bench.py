@@ -94,6 +94,8 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ca_central_batch256_hbm_traffic.m
 
 
 HELD_OUT_SEED = 20000  # first seed of the held-out central events
+MIXED_SEED = 3000      # first seed of the mixed-multiplicity batch
+PILEUP_SEED = 50000    # first seed of the pile-up batch
 TRAFFIC_BATCH = 256     # the workload of the counter runs in TRAFFIC_FILE
 
 
@@ -293,7 +295,7 @@ def mixed_batch(itsca, steps, tuning_fixed=None):
     """256 events of log-uniform multiplicity in [50, 8000] tracks in one
     launch, plain form, launch shape chosen by the autotuner, graph replay;
     per-stage kernel times and algorithmic GB/s."""
-    evs = itsca.generate_events(3, 1000, 256)
+    evs = itsca.generate_events(3, MIXED_SEED, 256)
     t = itsca.GpuTracker(device=0)
     t.set_events(evs)
     t.run()
@@ -305,7 +307,7 @@ def mixed_batch(itsca, steps, tuning_fixed=None):
     s = time_form(t, steps, 2)
     counts = [t.counts(i) for i in range(t.num_items())]
     kernel_ms, _, _ = kernel_profile(t)
-    out = {"workload": "256 events, multiplicity log-uniform in [50, 8000] tracks (seeds 1000..1255), one launch",
+    out = {"workload": "256 events, multiplicity log-uniform in [50, 8000] tracks (seeds %d..%d), one launch" % (MIXED_SEED, MIXED_SEED + 255),
            "form": "plain, hipGraph replay", "events_per_s": round(256 * steps / s, 2),
            "ms_per_step": round(1e3 * s / steps, 4),
            "tuning": tuning_dict(tuning),
@@ -338,7 +340,7 @@ def binning_sweep(itsca, tracker, threshold, steps=10):
                              "kPrep_ms": round(km.get("kPrep", 0.0), 4)})
         return rows
     out = {"central": sweep(tracker, ("plain", "learned"))}
-    evs = itsca.generate_events(3, 1000, 256)
+    evs = itsca.generate_events(3, MIXED_SEED, 256)
     t = itsca.GpuTracker(device=0)
     t.set_events(evs)
     out["mixed"] = sweep(t, ("mixed",))
@@ -351,7 +353,7 @@ def pileup_batch(itsca, ft, params, threshold, gates, steps, events=64):
     each, 3 % noise), every vertex its own work item, plain and learned."""
     with tempfile.TemporaryDirectory() as tmp:
         ev_path, lab_path = os.path.join(tmp, "pileup.txt"), os.path.join(tmp, "pileup_labels.txt")
-        itsca.write_events(5, 40000, events, ev_path, lab_path)
+        itsca.write_events(5, PILEUP_SEED, events, ev_path, lab_path)
         evs = itsca.load_events(ev_path)
         t = itsca.GpuTracker(device=0)
         t.set_events(evs)
@@ -365,7 +367,7 @@ def pileup_batch(itsca, ft, params, threshold, gates, steps, events=64):
         q_learned = road_report(itsca, t, evs, ev_path, lab_path)
         quality = ft.evaluate(t, evs, threshold)
         t.close()
-    return {"workload": "%d pile-up events (4 vertices each, seeds 40000..%d), one launch" % (events, 40000 + events - 1),
+    return {"workload": "%d pile-up events (4 vertices each, seeds %d..%d), one launch" % (events, PILEUP_SEED, PILEUP_SEED + events - 1),
             "plain_events_per_s": round(events * steps / s_plain, 2),
             "learned_events_per_s": round(events * steps / s_learned, 2),
             "speedup": round(s_plain / s_learned, 4), "track_recall": round(quality["recall"], 6),

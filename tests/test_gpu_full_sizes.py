@@ -43,7 +43,7 @@ def _batch_vs_single(t, evs):
 
 def test_mixed_256_event_batch():
     say("gpu full size: 256 mixed-multiplicity events")
-    evs = itsca.generate_events(3, 1000, 256)
+    evs = itsca.generate_events(3, 3000, 256)
     sizes = [len(ev.x) for ev in evs]
     t = itsca.GpuTracker(device=0)
     _batch_vs_single(t, evs)
@@ -70,7 +70,7 @@ def test_central_256_event_batch_learned_within_plain():
 
 def test_pileup_64_event_batch():
     say("gpu full size: 64 pile-up events")
-    evs = itsca.generate_events(5, 40000, 64)
+    evs = itsca.generate_events(5, 50000, 64)
     assert all(len(ev.vertices) == 4 for ev in evs)
     t = itsca.GpuTracker(device=0)
     _batch_vs_single(t, evs)
