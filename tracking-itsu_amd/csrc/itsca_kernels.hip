@@ -1864,13 +1864,14 @@ __global__ void __launch_bounds__(B) kEdgesFill(DevState s)
 // kTransposeKeys.  Bucket placement by LDS atomics, then each edge is written
 // at bucket start + (number of bucket members with a smaller edge index).  The
 // fill writes the edges in ascending c, so that rank keeps the predecessors
-// ascending whatever order the atomics took.
-constexpr int kTransposeKeys = 8192;
+// ascending whatever order the atomics took.  One LDS array serves as counts,
+// bucket starts and insertion cursors: after the placement the cursor of
+// bucket k stands at its end, which is the start of bucket k + 1.
+constexpr int kTransposeKeys = 16384;
 
 __global__ void __launch_bounds__(1024) kEdgesTranspose(DevState s)
 {
   __shared__ int hist[kTransposeKeys];
-  __shared__ int start[kTransposeKeys];
   __shared__ int scratch[1024 / 64 + 1];
   const int w = blockIdx.x / (kCellLayers - 1), l = blockIdx.x % (kCellLayers - 1);
   const Seg cn = s.celSeg[w * kCellLayers + l + 1];
@@ -1881,6 +1882,7 @@ __global__ void __launch_bounds__(1024) kEdgesTranspose(DevState s)
   int carry = 0;
   for (int k0 = 0; k0 < cn.count; k0 += kTransposeKeys) {
     const int nk = min(kTransposeKeys, cn.count - k0);
+    const int passStart = carry;
     for (int k = threadIdx.x; k < nk; k += blockDim.x) hist[k] = 0;
     __syncthreads();
     for (int a = threadIdx.x; a < e.count; a += blockDim.x) {
@@ -1894,7 +1896,6 @@ __global__ void __launch_bounds__(1024) kEdgesTranspose(DevState s)
       int total;
       const int ex = blockExclusiveScan(v, scratch, total);
       if (k < nk) {
-        start[k] = carry + ex;
         s.edgeOff[cn.base + k0 + k] = carry + ex;
         s.edgeCnt[cn.base + k0 + k] = v;
         hist[k] = carry + ex;  // insertion cursor
@@ -1911,7 +1912,7 @@ __global__ void __launch_bounds__(1024) kEdgesTranspose(DevState s)
     for (int a = threadIdx.x; a < e.count; a += blockDim.x) {
       const int kk = to[a] - k0;
       if (kk < 0 || kk >= nk) continue;
-      const int lo = start[kk], hi = hist[kk];  // the cursor ended at the bucket end
+      const int lo = kk > 0 ? hist[kk - 1] : passStart, hi = hist[kk];  // cursors ended at the bucket ends
       int rank = 0;
       for (int b = lo; b < hi; ++b) rank += tmp[b] < a ? 1 : 0;
       s.pred[e.base + lo + rank] = from[a];
