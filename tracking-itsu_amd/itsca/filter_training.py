@@ -1,9 +1,9 @@
 """Training of the learned tracklet filter on the GPU, and its weights file.
 
 The filter has two parts, both fitted here:
-- per tracklet layer, a phi gate of the tracklet search: twice the largest
-  |dphi| of the positive tracklets of the first gate-sample events (capped at
-  the reference's 0.3 rad cut).  Filtered runs search and keep only the
+- per tracklet layer, a phi gate of the tracklet search: 1.1 times the
+  largest |dphi| of the positive tracklets of the first gate-sample events
+  (capped at the reference's 0.3 rad cut).  Filtered runs search and keep only the
   candidates inside it;
 - the network, which scores the gated candidates (its count features are the
   gated counts).
@@ -113,7 +113,10 @@ def collect(tracker: GpuTracker, preset: int, seeds: Sequence[int], stride: int,
     return tracker.train_size()
 
 
-def fit_gates(tracker: GpuTracker, preset: int, seeds: Sequence[int], margin: float = 2.0):
+GATE_MARGIN = 1.1  # gate = this x the largest |dphi| of a positive tracklet of the gate sample
+
+
+def fit_gates(tracker: GpuTracker, preset: int, seeds: Sequence[int], margin: float = GATE_MARGIN):
     """Per tracklet layer: margin x the largest |dphi| of the positive rows of
     plain runs over `seeds`, capped at the reference's 0.3 rad cut.  Read from
     the rows' features: layer = f0 / 0.2, dphi = f1 * 0.3 (BF16, 2^-9
@@ -256,7 +259,7 @@ def sweep(events, params: np.ndarray, thresholds: Sequence[float], gates=None) -
 def train_filter(tracker: GpuTracker, preset: int, seeds: Sequence[int], samples: float, val_seeds: Sequence[int],
                  epochs: int = 12, batch: int = 4096, lr: float = 3e-3, weight_decay: float = 1e-4,
                  pos_weight: float = 4.0, init_seed: int = 7, target_recall: float = 0.999, gate_events: int = 50,
-                 gate_margin: float = 2.0):
+                 gate_margin: float = GATE_MARGIN):
     """The whole training: the phi gates from plain runs over the first
     `gate_events` events of `seeds` (gate_margin <= 0: no gates), labels from
     gated runs over the events of `seeds` (about `samples` rows), AdamW epochs
@@ -328,7 +331,7 @@ def main(argv=None):
     ap.add_argument("--init-seed", type=int, default=7)
     ap.add_argument("--target-recall", type=float, default=0.999, help="road recall on the validation events")
     ap.add_argument("--gate-events", type=int, default=50, help="events whose positive tracklets set the phi gates")
-    ap.add_argument("--gate-margin", type=float, default=2.0,
+    ap.add_argument("--gate-margin", type=float, default=GATE_MARGIN,
                     help="phi gate = this x the largest |dphi| of a positive tracklet (<= 0: no gates)")
     args = ap.parse_args(argv)
 
