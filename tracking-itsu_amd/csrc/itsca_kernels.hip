@@ -1968,18 +1968,26 @@ __global__ void __launch_bounds__(B) kLevels(DevState s, int layer, int stage)
       int level = 1;
       uint32_t paths = 1;
       if (cnt > 0) {
+        // One pass over the predecessors: the highest level so far and the
+        // saturating sum of the path counts of the predecessors at that level
+        // (a higher level restarts the sum), each predecessor's level and
+        // count loaded together.  The sum saturates at 2^32 - 1, so its order
+        // does not matter.
         const int off = s.edgSeg[w * (kCellLayers - 1) + layer - 1].base + pr.y;
         int best = 0;
-        for (int a = 0; a < cnt; ++a) best = max(best, s.clevel[cp.base + s.pred[off + a]]);
-        level = best + 1;
         paths = 0;
         for (int a = 0; a < cnt; ++a) {
           const int c = cp.base + s.pred[off + a];
-          if (s.clevel[c] == level - 1) {
-            const uint32_t p = s.cpaths[c];
+          const int lv = s.clevel[c];
+          const uint32_t p = s.cpaths[c];
+          if (lv > best) {
+            best = lv;
+            paths = p;
+          } else if (lv == best) {
             paths = paths + p < paths ? 0xffffffffu : paths + p;  // saturate
           }
         }
+        level = best + 1;
       }
       s.clevel[n] = level;
       s.cpaths[n] = paths;
