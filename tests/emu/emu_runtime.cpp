@@ -376,6 +376,16 @@ hipError_t hipFree(void* p)
   std::free(p);
   return hipSuccess;
 }
+hipError_t hipHostMalloc(void** p, size_t bytes, unsigned)
+{
+  *p = std::malloc(bytes ? bytes : 1);
+  return *p ? hipSuccess : hipErrorInvalidValue;
+}
+hipError_t hipHostFree(void* p)
+{
+  std::free(p);
+  return hipSuccess;
+}
 hipError_t hipMemcpy(void* dst, const void* src, size_t bytes, hipMemcpyKind)
 {
   std::memcpy(dst, src, bytes);

@@ -164,6 +164,9 @@ hipError_t hipMalloc(T** p, size_t bytes)
   return hipMalloc(reinterpret_cast<void**>(p), bytes);
 }
 hipError_t hipFree(void* p);
+#define hipHostMallocDefault 0u
+hipError_t hipHostMalloc(void** p, size_t bytes, unsigned flags);  // host memory (no pinning here)
+hipError_t hipHostFree(void* p);
 // Device symbols are host objects here: the "device address" is the object's own.
 #define HIP_SYMBOL(sym) (&(sym))
 inline hipError_t hipGetSymbolAddress(void** p, const void* sym)

@@ -50,6 +50,41 @@ struct DBuf {
   }
 };
 
+// Pinned (page-locked) host staging, grow-only, contents kept on growth:
+// host-device copies from and to it run as direct DMA at the link's rate,
+// where pageable memory goes through the runtime's bounce buffers.  A
+// growth frees the old block, so no copy may be in flight from it then (the
+// entry points that fill it synchronize their stream before returning).
+template <typename T>
+struct PinnedBuf {
+  T* p = nullptr;
+  size_t n = 0, cap = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf()
+  {
+    if (p) (void)hipHostFree(p);
+  }
+  hipError_t reserve(size_t want)
+  {
+    if (want <= cap && p) return hipSuccess;
+    const size_t c = std::max<size_t>({want, cap + cap / 2, (size_t)4096});
+    T* q = nullptr;
+    const hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&q), c * sizeof(T), hipHostMallocDefault);
+    if (e != hipSuccess) return e;
+    if (n) std::memcpy(q, p, n * sizeof(T));
+    if (p) (void)hipHostFree(p);
+    p = q;
+    cap = c;
+    return hipSuccess;
+  }
+  void clear() { n = 0; }
+  size_t size() const { return n; }
+  T* data() { return p; }
+  const T* data() const { return p; }
+};
+
 struct HipError {
   hipError_t code;
   std::string where;
@@ -129,9 +164,11 @@ struct itsca_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
 
-  // staged host input
-  std::vector<float> hx, hy, hz, hvert;
-  std::vector<int> hid, hmc, hLayerOff, hEvVert;  // hLayerOff: 8 per event; hEvVert: first vertex of each event
+  // staged host input (the cluster arrays in pinned memory)
+  PinnedBuf<float> hx, hy, hz;
+  PinnedBuf<int> hid, hmc;
+  std::vector<float> hvert;
+  std::vector<int> hLayerOff, hEvVert;  // hLayerOff: 8 per event; hEvVert: first vertex of each event
 
   // batch shape
   int W = 0, E = 0;
@@ -145,7 +182,7 @@ struct itsca_ctx {
   // results of the last run
   std::vector<Seg> trkSeg, celSeg, edgSeg, roadSeg;
   int status[kNumStatus] = {};
-  std::vector<int> hostRoads;  // staging of itsca_download_roads
+  PinnedBuf<int> hostRoads;  // staging of itsca_download_roads
 
   // itsca_run_noalloc: the device address of `scratch` on this device, and
   // fixed host staging for its small inputs and read-backs
@@ -569,21 +606,43 @@ itsca_status itsca_add_event(itsca_ctx* ctx, const float* x, const float* y, con
   const int base = (int)ctx->hx.size();
   if ((long)base + total > 0x7fffffffL) return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_add_event: batch too large");
   const int o = layer_offsets[0];
-  // Non-finite coordinates have no defined bin: the reference's conversions
-  // of NaN are undefined behaviour in C++, so they are refused here.
-  for (int i = o; i < o + total; ++i)
-    if (!std::isfinite(x[i]) || !std::isfinite(y[i]) || !std::isfinite(z[i]))
-      return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_add_event: cluster " + std::to_string(i - o) +
-                                               " has a non-finite coordinate");
   for (int v = 0; v < 3 * num_vertices; ++v)
     if (!std::isfinite(vertices[v]))
       return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_add_event: vertex " + std::to_string(v / 3) +
                                                " has a non-finite coordinate");
-  ctx->hx.insert(ctx->hx.end(), x + o, x + o + total);
-  ctx->hy.insert(ctx->hy.end(), y + o, y + o + total);
-  ctx->hz.insert(ctx->hz.end(), z + o, z + o + total);
-  ctx->hid.insert(ctx->hid.end(), cluster_id + o, cluster_id + o + total);
-  ctx->hmc.insert(ctx->hmc.end(), mc_id + o, mc_id + o + total);
+  const size_t want = (size_t)base + (size_t)total;
+  hipError_t e = ctx->hx.reserve(want);
+  if (e == hipSuccess) e = ctx->hy.reserve(want);
+  if (e == hipSuccess) e = ctx->hz.reserve(want);
+  if (e == hipSuccess) e = ctx->hid.reserve(want);
+  if (e == hipSuccess) e = ctx->hmc.reserve(want);
+  if (e != hipSuccess)
+    return ctx->fail(ITSCA_ERR_HIP, std::string("itsca_add_event: pinned host staging: ") + hipGetErrorString(e));
+  // The coordinates are copied and checked in one pass and committed only
+  // when all are finite.  Non-finite coordinates have no defined bin: the
+  // reference's conversions of NaN are undefined behaviour in C++, so they
+  // are refused here.
+  float* dx = ctx->hx.data() + base;
+  float* dy = ctx->hy.data() + base;
+  float* dz = ctx->hz.data() + base;
+  bool finite = true;
+  for (int i = 0; i < total; ++i) {
+    const float a = x[o + i], b = y[o + i], c = z[o + i];
+    dx[i] = a;
+    dy[i] = b;
+    dz[i] = c;
+    finite = finite & std::isfinite(a) & std::isfinite(b) & std::isfinite(c);
+  }
+  if (!finite)
+    for (int i = 0; i < total; ++i)
+      if (!std::isfinite(dx[i]) || !std::isfinite(dy[i]) || !std::isfinite(dz[i]))
+        return ctx->fail(ITSCA_ERR_ARGUMENT, "itsca_add_event: cluster " + std::to_string(i) +
+                                                 " has a non-finite coordinate");
+  if (total > 0) {
+    std::memcpy(ctx->hid.data() + base, cluster_id + o, (size_t)total * sizeof(int));
+    std::memcpy(ctx->hmc.data() + base, mc_id + o, (size_t)total * sizeof(int));
+  }
+  ctx->hx.n = ctx->hy.n = ctx->hz.n = ctx->hid.n = ctx->hmc.n = want;
   for (int l = 0; l < 8; ++l) ctx->hLayerOff.push_back(base + layer_offsets[l] - o);
   ctx->hEvVert.push_back((int)ctx->hvert.size() / 3);
   ctx->hvert.insert(ctx->hvert.end(), vertices, vertices + 3 * num_vertices);
@@ -750,7 +809,8 @@ itsca_status itsca_download_roads(itsca_ctx* ctx, int32_t* rows, size_t cap_rows
       if ((size_t)ctx->roadSeg[w].base != base + (size_t)first_road[w])
         return ctx->fail(ITSCA_ERR_STATE, "itsca_download_roads: roads are not in item order");
     HIP_TRY(hipSetDevice(ctx->device));
-    ctx->hostRoads.resize(n * (kCellLayers + 2));
+    HIP_TRY(ctx->hostRoads.reserve(n * (kCellLayers + 2)));
+    ctx->hostRoads.n = n * (kCellLayers + 2);
     int* cells = ctx->hostRoads.data();
     int* lab = cells + n * kCellLayers;
     int* fake = lab + n;
