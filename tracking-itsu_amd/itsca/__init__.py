@@ -472,7 +472,7 @@ class GpuTracker:
         first = np.zeros(self.num_items() + 1, dtype=np.int32)
         total = ctypes.c_size_t(0)
         st = self._lib.itsca_download_roads(self.ctx, None, 0, first.ctypes.data, ctypes.byref(total))
-        rows = np.zeros((max(total.value, 1), 8), dtype=np.int32)
+        rows = np.empty((max(total.value, 1), 8), dtype=np.int32)  # every returned row is written
         if st != 0 and total.value == 0:
             self._check(st, "itsca_download_roads")
         self._check(self._lib.itsca_download_roads(self.ctx, rows.ctypes.data, total.value, first.ctypes.data,
