@@ -299,6 +299,19 @@ try:
     raise SystemExit("a NaN coordinate was accepted")
 except itsca.ItscaError as e:
     assert "non-finite" in str(e), e
+# a refused event leaves the events staged before it intact, and the next one is staged after them
+L = t._lib
+F = lambda a: np.ascontiguousarray(a, np.float32).ctypes.data_as(ctypes.POINTER(ctypes.c_float))  # noqa: E731
+I = lambda a: np.ascontiguousarray(a, np.int32).ctypes.data_as(ctypes.POINTER(ctypes.c_int32))  # noqa: E731
+def add(e):
+    return L.itsca_add_event(t.ctx, F(e.x), F(e.y), F(e.z), I(e.cluster_id), I(e.mc), I(e.layer_offsets),
+                             F(np.asarray(e.vertices, np.float32).reshape(-1)), len(e.vertices), None)
+assert L.itsca_clear_events(t.ctx) == 0
+assert add(ev) == 0 and add(bad) != 0 and add(ev) == 0
+assert L.itsca_upload(t.ctx) == 0 and L.itsca_run(t.ctx) == 0
+want = pyoracle.OracleVertex(ev, 0, stable=True).dump()
+assert t.num_items() == 2
+assert itsca.dump_item(t, 0, ev.vertices[0]) == want and itsca.dump_item(t, 1, ev.vertices[0]) == want
 t.close()
 print("non-finite input refused ok", flush=True)
 # search-index binnings outside the 16-byte sub-bin entries are refused
