@@ -178,6 +178,11 @@ int shflXorInt(int v, int mask, int width)
 
 void waveBarrier() { cur->waves[tIdx.x / 64].wait(); }
 
+void globalLoadLds(const void* src, void* dst, unsigned size, int offset)
+{
+  std::memcpy(static_cast<char*>(dst) + offset + (size_t)size * (tIdx.x % 64), src, size);
+}
+
 emu_f32x4 mfma16x16x16bf16(emu_bf16x4 a, emu_bf16x4 b, emu_f32x4 c)
 {
   const int lane = (int)(tIdx.x % 64), wave = (int)(tIdx.x / 64);

@@ -52,6 +52,9 @@ int shflXorInt(int v, int mask, int width);
 int shflInt(int v, int src, int width);
 unsigned long long ballot(bool pred);
 void waveBarrier();
+// global_load_lds (LDS DMA): this lane's `size` bytes at src go to dst + offset + size * lane,
+// dst being the wavefront's uniform LDS base
+void globalLoadLds(const void* src, void* dst, unsigned size, int offset);
 // v_mfma_f32_16x16x16_bf16 as documented for CDNA wave64: lane i holds
 // A[i%16][4*(i/16)+r], B[4*(i/16)+r][i%16] and C/D[4*(i/16)+r][i%16].
 emu_f32x4 mfma16x16x16bf16(emu_bf16x4 a, emu_bf16x4 b, emu_f32x4 c);
@@ -99,6 +102,8 @@ inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
 #define __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, cbsz, abid, blgp) emu::mfma16x16x32bf16(a, b, c)
 #define __builtin_amdgcn_fence(order, scope) __atomic_thread_fence(__ATOMIC_SEQ_CST)
 #define __builtin_amdgcn_wave_barrier() emu::waveBarrier()
+#define __builtin_amdgcn_global_load_lds(src, dst, size, offset, aux) emu::globalLoadLds(src, dst, size, offset)
+#define __builtin_amdgcn_s_waitcnt(imm) ((void)(imm))  // the emulated loads complete at once
 inline float log1pf(float x) { return std::log1p(x); }
 inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline int atomicMin(int* p, int v)

@@ -1016,12 +1016,22 @@ __device__ __forceinline__ int stageUnion(const TrackletSearch& t, int rowRef, c
   __syncthreads();
   const int nrows = L.copyRows;
   if (nrows > 0) {
+    // Direct global-to-LDS loads (global_load_lds_dwordx4: lane L of a
+    // wavefront writes 16 bytes at the wavefront's LDS base + 16 L): no
+    // registers hold the records, and every load of the copy is in flight at
+    // once.  A lane past the end issues none.
     const int total = L.rowStart[nrows];
+    const int waveFirst = threadIdx.x & ~63;
     int ri = 0;  // k only grows, so the row index only moves forward
-    for (int k = threadIdx.x; k < total; k += blockDim.x) {
-      while (ri + 1 < nrows && L.rowStart[ri + 1] <= k) ++ri;
-      stage[k] = src[L.rowLo[ri] + (k - L.rowStart[ri])];
+    for (int k0 = 0; k0 < total; k0 += blockDim.x) {
+      const int k = k0 + threadIdx.x;
+      if (k < total) {
+        while (ri + 1 < nrows && L.rowStart[ri + 1] <= k) ++ri;
+        __builtin_amdgcn_global_load_lds(src + (L.rowLo[ri] + (k - L.rowStart[ri])), stage + (k0 + waveFirst), 16, 0,
+                                         0);
+      }
     }
+    __builtin_amdgcn_s_waitcnt(0);  // the copy is in LDS
     __syncthreads();
   }
   return nrows;
