@@ -292,11 +292,17 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
       return *reinterpret_cast<const int*>(static_cast<const char*>(b) + 4u * (unsigned)i);
     };
     auto st = [](void* b, int i, int v) { *reinterpret_cast<int*>(static_cast<char*>(b) + 4u * (unsigned)i) = v; };
+    // The loads of a field are issued together (indices past the layer
+    // clamped to its last cluster), then stored: a conditional load per
+    // element would wait for each one before its store.
     auto stageField = [&](const void* src) {
+      if (n > 0) {
+        int v[kPrepPer];
 #pragma unroll
-      for (int k = 0; k < kPrepPer; ++k) {
-        const int i = threadIdx.x + k * kPrepThreads;
-        if (i < n) lBucket[(pos2[k >> 1] >> (16 * (k & 1))) & 0xffffu] = ld(src, i);
+        for (int k = 0; k < kPrepPer; ++k) v[k] = ld(src, min((int)threadIdx.x + k * kPrepThreads, n - 1));
+#pragma unroll
+        for (int k = 0; k < kPrepPer; ++k)
+          if ((int)threadIdx.x + k * kPrepThreads < n) lBucket[(pos2[k >> 1] >> (16 * (k & 1))) & 0xffffu] = v[k];
       }
       __syncthreads();
     };
