@@ -134,6 +134,7 @@ struct DevState {
   // neighbour edges (c of layer l -> n of layer l+1), first in (c, n) order,
   // then transposed into predecessor lists
   int *succCnt, *succCand;  // per cell: accepted successors, kEdgeSlots slots
+  int* cFirst;  // per cell: its only predecessor at the highest level, else -1 (succCnt's memory, free after the edges)
   int *eFrom, *eTo, *eTmp;  // compact, edgSeg order
   int *edgeCnt, *edgeOff;   // per cell of layer l+1: its predecessor list
   Seg* edgSeg;              // W*4

@@ -1983,6 +1983,7 @@ __global__ void __launch_bounds__(B) kLevels(DevState s, int layer, int stage)
       const int cnt = pr.x;
       int level = 1;
       uint32_t paths = 1;
+      int only = -1;  // the only predecessor at the highest level, if there is one
       if (cnt > 0) {
         // One pass over the predecessors: the highest level so far and the
         // saturating sum of the path counts of the predecessors at that level
@@ -1990,23 +1991,29 @@ __global__ void __launch_bounds__(B) kLevels(DevState s, int layer, int stage)
         // count loaded together.  The sum saturates at 2^32 - 1, so its order
         // does not matter.
         const int off = s.edgSeg[w * (kCellLayers - 1) + layer - 1].base + pr.y;
-        int best = 0;
+        int best = 0, atBest = 0;
         paths = 0;
         for (int a = 0; a < cnt; ++a) {
-          const int c = cp.base + s.pred[off + a];
+          const int pi = s.pred[off + a];
+          const int c = cp.base + pi;
           const int lv = s.clevel[c];
           const uint32_t p = s.cpaths[c];
           if (lv > best) {
             best = lv;
             paths = p;
+            only = pi;
+            atBest = 1;
           } else if (lv == best) {
             paths = paths + p < paths ? 0xffffffffu : paths + p;  // saturate
+            ++atBest;
           }
         }
         level = best + 1;
+        if (atBest != 1) only = -1;
       }
       s.clevel[n] = level;
       s.cpaths[n] = paths;
+      s.cFirst[n] = only;
       if (layer == kCellLayers - 1 && level == kCellsMinLevel) {
         // A saturated count stays saturated through every later sum, so a
         // root reached by one exceeds the bound too.
@@ -2068,6 +2075,13 @@ __global__ void __launch_bounds__(B) kRoadsFill(DevState s)
       uint32_t q = (uint32_t)q0;
       for (int layer = L4; layer > 0; --layer) {
         const int n = cs[layer].base + cell[layer];
+        // A cell with one predecessor at the level below its own: every chain
+        // through it continues there, whatever q.
+        const int only = s.cFirst[n];
+        if (only >= 0) {
+          cell[layer - 1] = only;
+          continue;
+        }
         const int lvl = s.clevel[n];
         const int off = s.edgSeg[w * (kCellLayers - 1) + layer - 1].base + s.edgeOff[n];
         const int cnt2 = s.edgeCnt[n];

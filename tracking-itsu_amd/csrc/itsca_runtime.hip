@@ -450,6 +450,7 @@ struct itsca_ctx {
     s.cpaths = cpaths.p;
     s.cCap = cCap;
     s.succCnt = succCnt.p;
+    s.cFirst = succCnt.p;
     s.succCand = succCand.p;
     s.eFrom = eFrom.p;
     s.eTo = eTo.p;
@@ -932,6 +933,7 @@ bool carveNoAlloc(unsigned char* base, size_t bytes, int n, bool keptLists, DevS
   s.edgeOff = static_cast<int*>(take(s.cCap, 4));
   s.roadOff = static_cast<int*>(take(s.cCap, 4));
   s.succCnt = static_cast<int*>(take(s.cCap, 4));
+  s.cFirst = s.succCnt;
   s.succCand = static_cast<int*>(take((size_t)s.cCap * kEdgeSlots, 4));
   s.pred = static_cast<int*>(take(s.eCap, 4));
   s.eFrom = static_cast<int*>(take(s.eCap, 4));
