@@ -1883,7 +1883,9 @@ __global__ void __launch_bounds__(B) kEdgesFill(DevState s)
 // ascending whatever order the atomics took.  One LDS array serves as counts,
 // bucket starts and insertion cursors: after the placement the cursor of
 // bucket k stands at its end, which is the start of bucket k + 1.
-constexpr int kTransposeKeys = 16384;
+// 76 KB of LDS, two workgroups of 1024 per CU: the largest cell layer of a
+// central event (about 18k cells) takes one pass over its edges.
+constexpr int kTransposeKeys = 19456;
 
 __global__ void __launch_bounds__(1024) kEdgesTranspose(DevState s)
 {
