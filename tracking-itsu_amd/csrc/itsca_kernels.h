@@ -39,6 +39,7 @@ struct LaunchConfig {
   bool filter;            // learned tracklet filter after the tracklet count
   float threshold;        // its logit threshold
   bool fusedFilter;       // the filter inside the tracklet search (kTrackletsSearchScore), one launch per layer
+  int cus;                // compute units of the device (0: unknown)
 };
 
 void launchPipeline(const DevState& s, const LaunchConfig& cfg, hipStream_t stream, KernelTimer* timer);

@@ -926,6 +926,7 @@ __device__ __forceinline__ void forEachTracklet(const DevState& s, int rw, int l
 // few phi rows times a range of z bins; those clusters of the next layer are
 // copied once per chunk and read from LDS by every cluster of the chunk.
 constexpr int kStageCap = 1984;  // next-layer clusters per chunk (31 KB: five workgroups of 256 per CU)
+constexpr int kTrackletsCountPerCu = 5;  // workgroups of kTrackletsCount resident per CU (its LDS)
 constexpr int kStageRows = 8;    // phi rows of the copy
 
 // The LDS copy of one chunk: the union of its windows, its rows in the copy.
@@ -2181,6 +2182,12 @@ void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t str
     return (unsigned)std::min<long>(512, (chunks + 255) / 256);
   };
   const int G = cfg.persistentBlocks;
+  // Some kernels run faster with at most the workgroups that are resident at
+  // once (perCu per CU; measured kernel by kernel): a grid of several rounds
+  // runs its last, partial round on a partly idle chip.  The others keep the
+  // tuned grid.
+  auto resident = [&](int perCu) { return cfg.cus > 0 ? std::min(G, perCu * cfg.cus) : G; };
+  const int G8 = resident(2048 / B);  // eight wavefronts per SIMD
   int* counts = cfg.planScratch;
   auto done = [&](const char* name, int stage) {
     if (timer) timer->after(name, stage, stream);
@@ -2204,13 +2211,14 @@ void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t str
     launchCommitKept(s, stream);
     done("kCommitKept", 1);
   } else {
+    const int Gt = resident(kTrackletsCountPerCu);
     if (s.zSub * s.phiSub > 1)
-      hipLaunchKernelGGL((kTrackletsCount<B, true>), dim3(G), dim3(B), 0, stream, s);
+      hipLaunchKernelGGL((kTrackletsCount<B, true>), dim3(Gt), dim3(B), 0, stream, s);
     else
-      hipLaunchKernelGGL((kTrackletsCount<B, false>), dim3(G), dim3(B), 0, stream, s);
+      hipLaunchKernelGGL((kTrackletsCount<B, false>), dim3(Gt), dim3(B), 0, stream, s);
     done("kTrackletsCount", 1);
     if (cfg.filter) {
-      launchTrackletFilter(s, B, G, cfg.threshold, stream);
+      launchTrackletFilter(s, B, resident(1024 / B), cfg.threshold, stream);  // four wavefronts per SIMD (its LDS)
       done("kTrackletsScore", 1);
     }
   }
@@ -2233,13 +2241,13 @@ void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t str
   hipLaunchKernelGGL(kScanChunks, dim3(1), dim3(1024), 0, stream, s, (int)kStCells, s.W * kCellLayers, s.celSeg,
                      s.cCap, (int)kStatusOverflowCells);
   done("kScanChunks", 2);
-  hipLaunchKernelGGL(kCellsFill<B>, dim3(G), dim3(B), 0, stream, s);
+  hipLaunchKernelGGL(kCellsFill<B>, dim3(G8), dim3(B), 0, stream, s);
   done("kCellsFill", 2);
   hipLaunchKernelGGL(kPlanEdges, dim3(1), dim3(1024), 0, stream, s, counts);
   hipLaunchKernelGGL(kChunkSegs, dim3(segBlocks(s.cCap)), dim3(256), 0, stream, s, (int)kStEdges, s.W * (kCellLayers - 1), 2,
                      0);
   done("kPlanEdges", 3);
-  hipLaunchKernelGGL(kEdgesCount<B>, dim3(G), dim3(B), 0, stream, s);
+  hipLaunchKernelGGL(kEdgesCount<B>, dim3(G8), dim3(B), 0, stream, s);
   done("kEdgesCount", 3);
   hipLaunchKernelGGL(kScanChunks, dim3(1), dim3(1024), 0, stream, s, (int)kStEdges, s.W * (kCellLayers - 1),
                      s.edgSeg, s.eCap, (int)kStatusOverflowEdges);
@@ -2255,7 +2263,7 @@ void launchPipelineB(const DevState& s, const LaunchConfig& cfg, hipStream_t str
     hipLaunchKernelGGL(kPlanLayer, dim3(1), dim3(1024), 0, stream, s, counts, layer, stage);
     hipLaunchKernelGGL(kChunkSegs, dim3(segBlocks(s.cCap)), dim3(256), 0, stream, s, stage, s.W, 3, layer);
     done("kPlanLayer", 4);
-    hipLaunchKernelGGL(kLevels<B>, dim3(G), dim3(B), 0, stream, s, layer, stage);
+    hipLaunchKernelGGL(kLevels<B>, dim3(G8), dim3(B), 0, stream, s, layer, stage);
     done("kLevels", 4);
   }
   hipLaunchKernelGGL(kScanChunks, dim3(1), dim3(1024), 0, stream, s, (int)kStRoads, s.W, s.roadSeg, s.rCap,

@@ -479,7 +479,7 @@ struct itsca_ctx {
   {
     int maxN = 0;
     for (const Seg& g : clSeg) maxN = std::max(maxN, g.count);
-    LaunchConfig cfg{persistentBlocks, planScratch.p, maxN, filterOn, threshold, filterFused};
+    LaunchConfig cfg{persistentBlocks, planScratch.p, maxN, filterOn, threshold, filterFused, cus};
     if (profiling && !timerReady) {
       for (auto& e : timer.ev) HIP_TRY(hipEventCreate(&e));
       timerReady = true;
@@ -1003,7 +1003,7 @@ itsca_status itsca_run_noalloc(itsca_ctx* ctx, const itsca_device_event* ev, int
     HIP_TRY(hipMemcpyAsync(const_cast<int*>(s.itemEvent), ctx->naZero, 4, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(const_cast<int*>(s.recItem), ctx->naZero, 4, hipMemcpyHostToDevice, ctx->stream));
     LaunchConfig cfg{ctx->persistentBlocks, planScratch, maxLayer, ctx->filterOn && ctx->mlpWbf.p != nullptr,
-                     ctx->threshold, ctx->filterFused};
+                     ctx->threshold, ctx->filterFused, ctx->cus};
     launchPipeline(s, cfg, ctx->stream, nullptr);
     if (roads_cap > 0) launchRoadsOut(s, 0, roads, roads_cap, ctx->stream);
     HIP_TRY(hipGetLastError());
