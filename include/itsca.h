@@ -41,7 +41,8 @@ enum {
 
 typedef struct itsca_config {
   int device;            /* HIP device ordinal */
-  int persistent_blocks; /* workgroups of the persistent stage kernels; 0 = 8 per CU */
+  int persistent_blocks; /* workgroups of the persistent stage kernels (some take at most their resident
+                            workgroups, two take twice as many); 0 = 8 per CU */
   double capacity_scale; /* multiplier on the built-in output-size estimates; 0 = 1 */
 } itsca_config;
 
@@ -131,7 +132,9 @@ itsca_status itsca_download(itsca_ctx* ctx, int item, int what, int layer, void*
 /* Launch shape of the stage kernels and binning of the tracklet search index.
  * None of it changes a result, only the time (itsca_autotune searches it).
  *   block_size:        workgroup size and chunk length, 64, 128 or 256
- *   persistent_blocks: workgroups of each persistent stage kernel (0 = 8 per CU)
+ *   persistent_blocks: workgroups of the persistent stage kernels (0 = 8 per CU); the tracklet count, the
+ *     scoring, the cell fill, the edge count and the levels launch at most their resident workgroups, the
+ *     edge fill and the road fill twice this number
  *   z_subbins:         z sub-bins per z bin of the search index, 1..8 (0 = 8);
  *                      the search tests 20 * z_subbins z slices per layer instead
  *                      of the reference's 20 bins
